@@ -1,0 +1,105 @@
+"""ctypes binding of libfiuncho_amd.so (the C ABI in include/fiuncho_amd.h).
+
+The library is built in-tree (``fiuncho_amd/libfiuncho_amd.so``) by
+``__graft_entry__.build()`` / ``make -C fiuncho_amd/csrc``.  There is no CPU
+fallback: if the library is missing, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfiuncho_amd.so")
+
+FIU_OK = 0
+FIU_ERR_INVALID = 1
+FIU_ERR_DEVICE = 2
+FIU_ERR_NOMEM = 3
+FIU_ERR_UNSUPPORTED = 4
+FIU_ERR_IO = 5
+FIU_ERR_REJECTED = 6
+FIU_ERR_CANCELLED = 7
+FIU_ERR_PENDING = 8
+
+ALGO_AUTO, ALGO_DIRECT, ALGO_MARGINAL = 0, 1, 2
+
+# Every symbol include/fiuncho_amd.h declares (checked by the CPU tests).
+EXPORTS = (
+    "fiu_last_error", "fiu_version", "fiu_set_device",
+    "fiu_dataset_create", "fiu_dataset_read", "fiu_dataset_info", "fiu_dataset_words",
+    "fiu_dataset_destroy", "fiu_config_init", "fiu_search", "fiu_ct_size",
+    "fiu_combine_batch", "fiu_contingency_batch", "fiu_mi_batch",
+    "fiu_synth_spec_init", "fiu_synth_generate", "fiu_dataset_synth",
+)
+
+
+class FiuError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"fiuncho_amd error {status}: {message}")
+        self.status = status
+
+
+class Config(C.Structure):
+    _fields_ = [("struct_size", C.c_uint32), ("algorithm", C.c_int32),
+                ("chunk_combos", C.c_uint64), ("candidate_cap", C.c_uint32),
+                ("reserved0", C.c_uint32), ("stream", C.c_void_p)]
+
+
+class SearchStats(C.Structure):
+    _fields_ = [("combinations", C.c_uint64), ("launches", C.c_uint64),
+                ("redo_chunks", C.c_uint64), ("scan_ms", C.c_double),
+                ("total_ms", C.c_double), ("algorithm", C.c_int32), ("reserved", C.c_int32)]
+
+
+class SynthSpec(C.Structure):
+    _fields_ = [("struct_size", C.c_uint32), ("cases", C.c_uint32), ("ctrls", C.c_uint32),
+                ("planted_order", C.c_uint32), ("snps", C.c_uint64),
+                ("maf_lo", C.c_double), ("maf_hi", C.c_double),
+                ("low_frac", C.c_double), ("low_lo", C.c_double), ("low_hi", C.c_double),
+                ("seed", C.c_uint64), ("planted", C.c_int64 * 8),
+                ("penetrance_seed", C.c_uint64), ("pen_lo", C.c_double), ("pen_hi", C.c_double),
+                ("threads", C.c_int32), ("reserved", C.c_int32)]
+
+
+_lib = None
+
+
+def lib():
+    """Load the library (raises when it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FiuError(FIU_ERR_DEVICE, f"{LIB_PATH} is not built (run __graft_entry__.build())")
+    L = C.CDLL(LIB_PATH)
+    vp, sz, u32, i32, u16 = C.c_void_p, C.c_size_t, C.c_uint32, C.c_int32, C.c_uint16
+    L.fiu_last_error.restype = C.c_char_p
+    L.fiu_version.restype = C.c_char_p
+    L.fiu_set_device.argtypes = [C.c_int]
+    L.fiu_dataset_create.argtypes = [vp, sz, sz, sz, u32, u32, C.POINTER(vp)]
+    L.fiu_dataset_read.argtypes = [C.POINTER(C.c_char_p), sz, C.POINTER(vp)]
+    L.fiu_dataset_info.argtypes = [vp, C.POINTER(sz), C.POINTER(u32), C.POINTER(u32),
+                                   C.POINTER(sz), C.POINTER(sz), C.POINTER(sz)]
+    L.fiu_dataset_words.argtypes = [vp, vp]
+    L.fiu_dataset_destroy.argtypes = [vp]
+    L.fiu_config_init.argtypes = [C.POINTER(Config)]
+    L.fiu_config_init.restype = None
+    L.fiu_search.argtypes = [vp, u16, i32, i32, u32, C.POINTER(Config), vp, vp,
+                             C.POINTER(u32), C.POINTER(SearchStats)]
+    L.fiu_ct_size.restype = sz
+    L.fiu_ct_size.argtypes = [u16]
+    L.fiu_combine_batch.argtypes = [vp, vp, u16, sz, vp]
+    L.fiu_contingency_batch.argtypes = [vp, vp, u16, sz, vp, vp]
+    L.fiu_mi_batch.argtypes = [vp, sz, u16, u32, u32, vp]
+    L.fiu_synth_spec_init.argtypes = [C.POINTER(SynthSpec)]
+    L.fiu_synth_spec_init.restype = None
+    L.fiu_synth_generate.argtypes = [C.POINTER(SynthSpec), vp, vp, C.POINTER(sz), C.POINTER(sz)]
+    L.fiu_dataset_synth.argtypes = [C.POINTER(SynthSpec), C.POINTER(vp)]
+    _lib = L
+    return L
+
+
+def check(status):
+    if status != FIU_OK:
+        raise FiuError(status, lib().fiu_last_error().decode())

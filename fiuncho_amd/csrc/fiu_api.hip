@@ -1,0 +1,624 @@
+// C ABI of fiuncho_amd (declared in include/fiuncho_amd.h): dataset upload,
+// the chunked search driver and the batch operators.
+#include "../../include/fiuncho_amd.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "fiu_kernels.h"
+#include "fiu_loader.h"
+#include "fiu_synth.h"
+#include <thread>
+
+using namespace fiu;
+
+namespace {
+
+thread_local std::string g_error;
+thread_local int g_device = 0;
+
+struct Failure {
+    fiu_status status;
+    std::string msg;
+};
+
+[[noreturn]] void fail(fiu_status s, const std::string &m) { throw Failure{s, m}; }
+
+void check(hipError_t e, const char *what)
+{
+    if (e != hipSuccess) fail(FIU_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <class F> fiu_status guard(F &&f)
+{
+    try {
+        f();
+        g_error.clear();
+        return FIU_OK;
+    } catch (const Failure &e) {
+        g_error = e.msg;
+        return e.status;
+    } catch (const LoadError &e) {
+        g_error = e.what();
+        return FIU_ERR_IO;
+    } catch (const std::bad_alloc &) {
+        g_error = "host allocation failed";
+        return FIU_ERR_NOMEM;
+    } catch (const std::exception &e) {
+        g_error = e.what();
+        return FIU_ERR_INVALID;
+    }
+}
+
+// Device buffer with RAII.
+template <class T> struct DBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    ~DBuf() { release(); }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void alloc(size_t count)
+    {
+        if (count <= n && p) return;
+        release();
+        if (hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)) != hipSuccess) {
+            p = nullptr;
+            (void)hipGetLastError();
+            fail(FIU_ERR_NOMEM, "device allocation of " + std::to_string(count * sizeof(T)) + " bytes failed");
+        }
+        n = count;
+    }
+};
+
+uint64_t binom_sat(uint64_t n, uint64_t k)
+{
+    if (k > n) return 0;
+    k = std::min(k, n - k);
+    unsigned __int128 r = 1;
+    for (uint64_t i = 1; i <= k; ++i) {
+        r = r * (n - k + i) / i;
+        if (r > (unsigned __int128)UINT64_MAX) return UINT64_MAX;
+    }
+    return (uint64_t)r;
+}
+
+int bit_width(uint64_t x)
+{
+    int b = 0;
+    while (x) ++b, x >>= 1;
+    return std::max(b, 1);
+}
+
+}  // namespace
+
+struct fiu_dataset {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    size_t snps = 0, cw = 0, tw = 0;
+    uint32_t cases = 0, ctrls = 0;
+    std::vector<uint64_t> host_words;
+    DBuf<uint4> bits;
+    DBuf<uint32_t> singles;
+    DBuf<uint32_t> flags;
+    DevData dev{};
+    // search scratch (grown on demand, kept for later calls)
+    DBuf<uint64_t> binom;
+    int binom_m = -1;
+    DBuf<Cand> cand, top;
+    DBuf<TopState> state;
+    DBuf<uint32_t> redo;
+    std::vector<hipEvent_t> events;
+
+    ~fiu_dataset()
+    {
+        for (auto e : events) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+void upload(fiu_dataset *d, const uint64_t *words)
+{
+    if (d->snps > (size_t)INT32_MAX - 64) fail(FIU_ERR_UNSUPPORTED, "more SNPs than INT_MAX");
+    check(hipSetDevice(d->device), "hipSetDevice");
+    check(hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking), "hipStreamCreate");
+    const int n = (int)d->snps, n_pad = (n + 63) / 64 * 64;
+    const int q0 = (int)((d->cw + 1) / 2), q1 = (int)((d->tw + 1) / 2);
+    const uint64_t off1 = (uint64_t)q0 * 3 * n_pad;
+    const uint64_t total = off1 + (uint64_t)q1 * 3 * n_pad;
+    d->bits.alloc(total);
+    const size_t nwords = d->snps * 3 * (d->cw + d->tw);
+    {
+        DBuf<uint64_t> staging;
+        staging.alloc(nwords);
+        if (nwords)
+            check(hipMemcpyAsync(staging.p, words, nwords * 8, hipMemcpyHostToDevice, d->stream), "upload");
+        check(launch_transpose(staging.p, d->bits.p, n, n_pad, 0, d->cw, d->tw, q0, 0, d->stream), "transpose");
+        check(launch_transpose(staging.p, d->bits.p, n, n_pad, 1, d->cw, d->tw, q1, off1, d->stream), "transpose");
+        check(hipStreamSynchronize(d->stream), "transpose sync");
+    }
+    DevData &D = d->dev;
+    D.bits = d->bits.p;
+    D.class_off[0] = 0;
+    D.class_off[1] = off1;
+    D.quads[0] = q0;
+    D.quads[1] = q1;
+    D.n = n;
+    D.n_pad = n_pad;
+    D.cases = d->cases;
+    D.ctrls = d->ctrls;
+    d->singles.alloc((size_t)n_pad * 6);
+    d->flags.alloc(4);
+    uint32_t zero[4] = {0, 0, 0, 0};
+    check(hipMemcpyAsync(d->flags.p, zero, sizeof(zero), hipMemcpyHostToDevice, d->stream), "flags");
+    D.singles = d->singles.p;
+    if (n > 0) check(launch_singles(d->bits.p, D, d->singles.p, d->flags.p, d->stream), "singles");
+    uint32_t fl[4];
+    check(hipMemcpyAsync(fl, d->flags.p, sizeof(fl), hipMemcpyDeviceToHost, d->stream), "flags");
+    check(hipStreamSynchronize(d->stream), "singles sync");
+    D.uniform = (n > 0 && fl[0] == 0) ? 1 : 0;
+    D.total[0] = fl[1];
+    D.total[1] = fl[2];
+}
+
+fiu_dataset *create(const uint64_t *words, size_t snps, size_t cw, size_t tw, uint32_t cases, uint32_t ctrls)
+{
+    if (!words && snps) fail(FIU_ERR_INVALID, "words is NULL");
+    if (snps && (cw == 0 || tw == 0) && (cases || ctrls)) fail(FIU_ERR_INVALID, "zero words per row");
+    if ((uint64_t)cases > cw * 64 || (uint64_t)ctrls > tw * 64)
+        fail(FIU_ERR_INVALID, "more samples than bits per row");
+    auto d = std::make_unique<fiu_dataset>();
+    d->device = g_device;
+    d->snps = snps;
+    d->cw = cw;
+    d->tw = tw;
+    d->cases = cases;
+    d->ctrls = ctrls;
+    d->host_words.assign(words, words + snps * 3 * (cw + tw));
+    upload(d.get(), d->host_words.data());
+    return d.release();
+}
+
+hipStream_t stream_of(const fiu_dataset *d, const fiu_config *cfg)
+{
+    return cfg && cfg->stream ? (hipStream_t)cfg->stream : d->stream;
+}
+
+void mi_constants(uint32_t cases, uint32_t ctrls, ScanParams &P)
+{
+    const double n = (double)cases + (double)ctrls;
+    double hy = 0.0;
+    if (cases) hy -= ((double)cases / n) * std::log((double)cases / n);
+    if (ctrls) hy -= ((double)ctrls / n) * std::log((double)ctrls / n);
+    P.n64 = n;
+    P.hy64 = hy;
+    P.hy = (float)hy;
+    P.inv_n = (float)(1.0 / n);
+    // bound on |screen - exact| (a few 1e-6 nats) with a 5x margin
+    P.delta = 2e-5f;
+}
+
+void ensure_binom(fiu_dataset *d, int m, hipStream_t s)
+{
+    if (d->binom_m >= m) return;
+    const uint64_t n = d->snps, stride = n + 1;
+    std::vector<uint64_t> h((uint64_t)(m + 1) * stride);
+    for (int j = 0; j <= m; ++j)
+        for (uint64_t x = 0; x <= n; ++x) h[j * stride + x] = binom_sat(x, (uint64_t)j);
+    d->binom.alloc(h.size());
+    check(hipMemcpyAsync(d->binom.p, h.data(), h.size() * 8, hipMemcpyHostToDevice, s), "binom upload");
+    check(hipStreamSynchronize(s), "binom sync");
+    d->binom_m = m;
+}
+
+struct Chunk {
+    int64_t j0, j1;
+};
+
+void search(fiu_dataset *d, int order, int32_t step, int32_t offset, uint32_t outputs, const fiu_config *cfg,
+            int32_t *out_combos, float *out_mi, uint32_t *out_count, fiu_search_stats *stats)
+{
+    const auto t_start = std::chrono::steady_clock::now();
+    if (!out_count) fail(FIU_ERR_INVALID, "out_count is NULL");
+    *out_count = 0;
+    if (order < 2) fail(FIU_ERR_INVALID, "order must be >= 2");
+    if (step < 1) fail(FIU_ERR_INVALID, "distribution step must be >= 1");
+    if (offset < 0) fail(FIU_ERR_INVALID, "distribution offset must be >= 0");
+    if (outputs > 0 && (!out_combos || !out_mi)) fail(FIU_ERR_INVALID, "output pointers are NULL");
+    if (order > kMaxOrder) fail(FIU_ERR_UNSUPPORTED, "order " + std::to_string(order) + " is above this build's limit");
+    if (outputs > (uint32_t)kMaxOutputs)
+        fail(FIU_ERR_UNSUPPORTED, "outputs above " + std::to_string(kMaxOutputs));
+    const int algo_req = cfg ? cfg->algorithm : FIU_ALGO_AUTO;
+    if (algo_req == FIU_ALGO_MARGINAL && !d->dev.uniform)
+        fail(FIU_ERR_INVALID, "marginal algorithm needs every SNP to cover the same samples");
+    fiu_search_stats st{};
+    st.algorithm = FIU_ALGO_DIRECT;
+    const int n = (int)d->snps, M = order - 1;
+    if (n < order || outputs == 0) {
+        if (stats) *stats = st;
+        return;
+    }
+    const int bits = bit_width((uint64_t)n - 1);
+    if (bits * order > 64) fail(FIU_ERR_UNSUPPORTED, "combination does not pack in 64 bits");
+    const uint64_t total_prefixes = binom_sat((uint64_t)n, (uint64_t)M);
+    if (total_prefixes == UINT64_MAX) fail(FIU_ERR_UNSUPPORTED, "too many prefixes");
+    const uint64_t n_pref = total_prefixes > (uint64_t)offset
+                                ? (total_prefixes - (uint64_t)offset + (uint64_t)step - 1) / (uint64_t)step
+                                : 0;
+    hipStream_t s = stream_of(d, cfg);
+    check(hipSetDevice(d->device), "hipSetDevice");
+    ensure_binom(d, M, s);
+
+    const uint32_t cap = cfg && cfg->candidate_cap ? cfg->candidate_cap : (1u << 20);
+    d->cand.alloc(cap);
+    d->top.alloc(kMaxOutputs);
+    d->state.alloc(1);
+    // chunk plan: a small first chunk sets the threshold, then even chunks
+    const uint64_t per_max = (uint64_t)(n - M);  // combinations of the lightest-loaded prefix bound
+    const double per_avg = (double)(n - M) / (double)(M + 1);
+    const uint64_t warm = std::min<uint64_t>(cap / 2, std::max<uint64_t>(16ull * outputs, 1ull << 16));
+    const uint64_t target = cfg && cfg->chunk_combos ? cfg->chunk_combos : (1ull << 28);
+    std::vector<Chunk> chunks;
+    {
+        int64_t j = 0;
+        const int64_t first = (int64_t)std::max<uint64_t>(1, warm / std::max<uint64_t>(per_max, 1));
+        const int64_t later = (int64_t)std::max<double>(1.0, (double)target / std::max(per_avg, 1.0));
+        while ((uint64_t)j < n_pref) {
+            const int64_t len = chunks.empty() ? first : later;
+            const int64_t e = std::min<int64_t>((int64_t)n_pref, j + len);
+            chunks.push_back({j, e});
+            j = e;
+        }
+    }
+    d->redo.alloc(std::max<size_t>(chunks.size(), 1));
+    TopState init{};
+    init.tau = -INFINITY;
+    init.tau_combo = ~0ull;
+    check(hipMemcpyAsync(d->state.p, &init, sizeof(init), hipMemcpyHostToDevice, s), "state init");
+
+    ScanParams P{};
+    P.step = step;
+    P.offset = offset;
+    P.total_prefixes = total_prefixes;
+    P.binom = d->binom.p;
+    P.cand = d->cand.p;
+    P.cand_cap = cap;
+    P.combo_bits = bits;
+    P.tag = 0;
+    mi_constants(d->cases, d->ctrls, P);
+
+    while (d->events.size() < 2 * chunks.size() + 2) {
+        hipEvent_t e;
+        check(hipEventCreate(&e), "hipEventCreate");
+        d->events.push_back(e);
+    }
+    size_t ev = 0;
+    auto run_chunk = [&](size_t ci, bool timed) {
+        P.j0 = chunks[ci].j0;
+        P.j1 = chunks[ci].j1;
+        if (timed) check(hipEventRecord(d->events[ev], s), "event");
+        check(launch_scan_direct(order, d->bits.p, d->dev, P, d->state.p, s), "scan launch");
+        if (timed) check(hipEventRecord(d->events[ev + 1], s), "event");
+        if (timed) ev += 2;
+        check(launch_merge(d->state.p, d->top.p, d->cand.p, outputs, cap, (uint32_t)ci, d->redo.p, s), "merge launch");
+        ++st.launches;
+    };
+    for (size_t ci = 0; ci < chunks.size(); ++ci) run_chunk(ci, true);
+    // chunks whose candidates overflowed the buffer are scanned again with
+    // the (higher) final threshold until none overflows
+    for (;;) {
+        TopState hs;
+        check(hipMemcpyAsync(&hs, d->state.p, sizeof(hs), hipMemcpyDeviceToHost, s), "state read");
+        check(hipStreamSynchronize(s), "search sync");
+        if (hs.redo_count == 0) break;
+        std::vector<uint32_t> redo(hs.redo_count);
+        check(hipMemcpy(redo.data(), d->redo.p, redo.size() * 4, hipMemcpyDeviceToHost), "redo read");
+        hs.redo_count = 0;
+        check(hipMemcpyAsync(d->state.p, &hs, sizeof(hs), hipMemcpyHostToDevice, s), "state reset");
+        for (uint32_t ci : redo) {
+            run_chunk(ci, false);
+            ++st.redo_chunks;
+        }
+    }
+    double scan_ms = 0.0;
+    for (size_t i = 0; i + 1 < ev; i += 2) {
+        float ms = 0.0f;
+        check(hipEventElapsedTime(&ms, d->events[i], d->events[i + 1]), "event time");
+        scan_ms += ms;
+    }
+    TopState hs;
+    check(hipMemcpy(&hs, d->state.p, sizeof(hs), hipMemcpyDeviceToHost), "state read");
+    std::vector<Cand> top(hs.n_top);
+    if (hs.n_top) check(hipMemcpy(top.data(), d->top.p, hs.n_top * sizeof(Cand), hipMemcpyDeviceToHost), "top read");
+    const uint64_t mask = bits == 64 ? ~0ull : ((1ull << bits) - 1);
+    for (uint32_t i = 0; i < hs.n_top; ++i) {
+        for (int q = 0; q < order; ++q)
+            out_combos[(size_t)i * order + q] = (int32_t)((top[i].combo >> (bits * (order - 1 - q))) & mask);
+        out_mi[i] = top[i].mi;
+    }
+    *out_count = hs.n_top;
+    // combinations evaluated: sum over the distribution's prefixes
+    st.combinations = 0;
+    {
+        // exact count by walking prefixes is costly for huge scans; use the
+        // closed form when the distribution covers every prefix
+        if (step == 1 && offset == 0) st.combinations = binom_sat((uint64_t)n, (uint64_t)order);
+    }
+    st.scan_ms = scan_ms;
+    st.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    if (stats) *stats = st;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *fiu_last_error(void) { return g_error.c_str(); }
+
+const char *fiu_version(void) { return "fiuncho_amd 0.1 (gfx950)"; }
+
+fiu_status fiu_set_device(int device)
+{
+    return guard([&] {
+        int count = 0;
+        check(hipGetDeviceCount(&count), "hipGetDeviceCount");
+        if (device < 0 || device >= count) fail(FIU_ERR_INVALID, "no such device");
+        g_device = device;
+    });
+}
+
+void fiu_config_init(fiu_config *cfg)
+{
+    if (!cfg) return;
+    std::memset(cfg, 0, sizeof(*cfg));
+    cfg->struct_size = sizeof(fiu_config);
+    cfg->algorithm = FIU_ALGO_AUTO;
+}
+
+fiu_status fiu_dataset_create(const uint64_t *words, size_t snps, size_t cases_words, size_t ctrls_words,
+                              uint32_t cases, uint32_t ctrls, fiu_dataset **out)
+{
+    return guard([&] {
+        if (!out) fail(FIU_ERR_INVALID, "out is NULL");
+        *out = create(words, snps, cases_words, ctrls_words, cases, ctrls);
+    });
+}
+
+fiu_status fiu_dataset_read(const char *const *paths, size_t npaths, fiu_dataset **out)
+{
+    return guard([&] {
+        if (!out || (!paths && npaths)) fail(FIU_ERR_INVALID, "NULL argument");
+        std::vector<std::string> p;
+        for (size_t i = 0; i < npaths; ++i) p.emplace_back(paths[i] ? paths[i] : "");
+        PackedDataset pk;
+        read_files(p, pk);
+        *out = create(pk.words.data(), pk.snps, pk.cases_words, pk.ctrls_words, (uint32_t)pk.cases,
+                      (uint32_t)pk.ctrls);
+    });
+}
+
+fiu_status fiu_dataset_info(const fiu_dataset *ds, size_t *snps, uint32_t *cases, uint32_t *ctrls,
+                            size_t *cases_words, size_t *ctrls_words, size_t *device_bytes)
+{
+    return guard([&] {
+        if (!ds) fail(FIU_ERR_INVALID, "dataset is NULL");
+        if (snps) *snps = ds->snps;
+        if (cases) *cases = ds->cases;
+        if (ctrls) *ctrls = ds->ctrls;
+        if (cases_words) *cases_words = ds->cw;
+        if (ctrls_words) *ctrls_words = ds->tw;
+        if (device_bytes) *device_bytes = ds->bits.n * sizeof(uint4) + ds->singles.n * 4;
+    });
+}
+
+fiu_status fiu_dataset_words(const fiu_dataset *ds, uint64_t *out)
+{
+    return guard([&] {
+        if (!ds || !out) fail(FIU_ERR_INVALID, "NULL argument");
+        std::memcpy(out, ds->host_words.data(), ds->host_words.size() * 8);
+    });
+}
+
+fiu_status fiu_dataset_destroy(fiu_dataset *ds)
+{
+    return guard([&] {
+        if (ds) {
+            (void)hipSetDevice(ds->device);
+            if (ds->stream) (void)hipStreamSynchronize(ds->stream);
+        }
+        delete ds;
+    });
+}
+
+fiu_status fiu_search(const fiu_dataset *ds, uint16_t order, int32_t dist_step, int32_t dist_offset,
+                      uint32_t outputs, const fiu_config *cfg, int32_t *out_combos, float *out_mi,
+                      uint32_t *out_count, fiu_search_stats *stats)
+{
+    return guard([&] {
+        if (!ds) fail(FIU_ERR_INVALID, "dataset is NULL");
+        if (cfg && cfg->struct_size != sizeof(fiu_config)) fail(FIU_ERR_INVALID, "fiu_config.struct_size mismatch");
+        search(const_cast<fiu_dataset *>(ds), order, dist_step, dist_offset, outputs, cfg, out_combos, out_mi,
+               out_count, stats);
+    });
+}
+
+size_t fiu_ct_size(uint16_t order)
+{
+    size_t s = 1;
+    for (int i = 0; i < order; ++i) s *= 3;
+    return (s + 15) / 16 * 16;
+}
+
+static void check_combos(const fiu_dataset *ds, const int32_t *combos, int order, size_t n)
+{
+    if (!combos && n) fail(FIU_ERR_INVALID, "combos is NULL");
+    for (size_t i = 0; i < n; ++i)
+        for (int q = 0; q < order; ++q) {
+            const int32_t c = combos[i * order + q];
+            if (c < 0 || (size_t)c >= ds->snps) fail(FIU_ERR_INVALID, "SNP index out of range");
+            if (q && c <= combos[i * order + q - 1]) fail(FIU_ERR_INVALID, "combination not strictly ascending");
+        }
+}
+
+fiu_status fiu_combine_batch(const fiu_dataset *ds, const int32_t *combos, uint16_t order, size_t n, uint64_t *out)
+{
+    return guard([&] {
+        if (!ds || (!out && n)) fail(FIU_ERR_INVALID, "NULL argument");
+        if (order < 1 || order > 12) fail(FIU_ERR_UNSUPPORTED, "order outside 1..12");
+        check_combos(ds, combos, order, n);
+        if (!n) return;
+        check(hipSetDevice(ds->device), "hipSetDevice");
+        uint64_t R = 1;
+        for (int i = 0; i < order; ++i) R *= 3;
+        const size_t per = R * (ds->cw + ds->tw);
+        DBuf<int32_t> dc;
+        DBuf<uint64_t> dout;
+        dc.alloc(n * order);
+        dout.alloc(n * per);
+        check(hipMemcpyAsync(dc.p, combos, n * order * 4, hipMemcpyHostToDevice, ds->stream), "copy");
+        check(launch_combine_batch(ds->bits.p, ds->dev, dc.p, order, n, ds->cw, ds->tw, dout.p, ds->stream), "combine");
+        check(hipMemcpyAsync(out, dout.p, n * per * 8, hipMemcpyDeviceToHost, ds->stream), "copy back");
+        check(hipStreamSynchronize(ds->stream), "sync");
+    });
+}
+
+fiu_status fiu_contingency_batch(const fiu_dataset *ds, const int32_t *combos, uint16_t order, size_t n,
+                                 uint32_t *out_counts, float *out_mi)
+{
+    return guard([&] {
+        if (!ds || (!out_counts && n)) fail(FIU_ERR_INVALID, "NULL argument");
+        if (order < 1 || order > 8) fail(FIU_ERR_UNSUPPORTED, "order outside 1..8");
+        check_combos(ds, combos, order, n);
+        if (!n) return;
+        check(hipSetDevice(ds->device), "hipSetDevice");
+        const size_t cts = fiu_ct_size(order);
+        DBuf<int32_t> dc;
+        DBuf<uint32_t> dout;
+        DBuf<float> dmi;
+        dc.alloc(n * order);
+        dout.alloc(n * 2 * cts);
+        check(hipMemcpyAsync(dc.p, combos, n * order * 4, hipMemcpyHostToDevice, ds->stream), "copy");
+        check(launch_contingency_batch(ds->bits.p, ds->dev, dc.p, order, n, cts, dout.p, ds->stream), "contingency");
+        if (out_mi) {
+            ScanParams P{};
+            mi_constants(ds->cases, ds->ctrls, P);
+            dmi.alloc(n);
+            check(launch_mi_batch(dout.p, n, cts, P.n64, P.hy64, dmi.p, ds->stream), "mi");
+            check(hipMemcpyAsync(out_mi, dmi.p, n * 4, hipMemcpyDeviceToHost, ds->stream), "copy back");
+        }
+        check(hipMemcpyAsync(out_counts, dout.p, n * 2 * cts * 4, hipMemcpyDeviceToHost, ds->stream), "copy back");
+        check(hipStreamSynchronize(ds->stream), "sync");
+    });
+}
+
+fiu_status fiu_mi_batch(const uint32_t *tables, size_t n, uint16_t order, uint32_t cases, uint32_t ctrls,
+                        float *out_mi)
+{
+    return guard([&] {
+        if ((!tables || !out_mi) && n) fail(FIU_ERR_INVALID, "NULL argument");
+        if (order < 1 || order > 12) fail(FIU_ERR_UNSUPPORTED, "order outside 1..12");
+        if (!n) return;
+        check(hipSetDevice(g_device), "hipSetDevice");
+        const size_t cts = fiu_ct_size(order);
+        hipStream_t s;
+        check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "stream");
+        std::unique_ptr<void, void (*)(void *)> sg((void *)s, [](void *p) { (void)hipStreamDestroy((hipStream_t)p); });
+        DBuf<uint32_t> dt;
+        DBuf<float> dm;
+        dt.alloc(n * 2 * cts);
+        dm.alloc(n);
+        check(hipMemcpyAsync(dt.p, tables, n * 2 * cts * 4, hipMemcpyHostToDevice, s), "copy");
+        ScanParams P{};
+        mi_constants(cases, ctrls, P);
+        check(launch_mi_batch(dt.p, n, cts, P.n64, P.hy64, dm.p, s), "mi");
+        check(hipMemcpyAsync(out_mi, dm.p, n * 4, hipMemcpyDeviceToHost, s), "copy back");
+        check(hipStreamSynchronize(s), "sync");
+    });
+}
+
+static SynthSpec to_spec(const fiu_synth_spec *sp)
+{
+    if (!sp) fail(FIU_ERR_INVALID, "spec is NULL");
+    if (sp->struct_size != sizeof(fiu_synth_spec)) fail(FIU_ERR_INVALID, "fiu_synth_spec.struct_size mismatch");
+    if (sp->planted_order > 8) fail(FIU_ERR_INVALID, "planted_order above 8");
+    SynthSpec s;
+    s.snps = sp->snps;
+    s.cases = sp->cases;
+    s.ctrls = sp->ctrls;
+    s.maf_lo = sp->maf_lo;
+    s.maf_hi = sp->maf_hi;
+    s.low_frac = sp->low_frac;
+    s.low_lo = sp->low_lo;
+    s.low_hi = sp->low_hi;
+    s.seed = sp->seed;
+    s.planted_order = (int)sp->planted_order;
+    for (int i = 0; i < 8; ++i) s.planted[i] = sp->planted[i];
+    for (uint32_t i = 0; i < sp->planted_order; ++i) {
+        if (sp->planted[i] < 0 || (uint64_t)sp->planted[i] >= sp->snps) fail(FIU_ERR_INVALID, "planted SNP out of range");
+        if (i && sp->planted[i] <= sp->planted[i - 1]) fail(FIU_ERR_INVALID, "planted SNPs not ascending");
+    }
+    s.pen_seed = sp->penetrance_seed;
+    s.pen_lo = sp->pen_lo;
+    s.pen_hi = sp->pen_hi;
+    return s;
+}
+
+static int synth_threads(const fiu_synth_spec *sp)
+{
+    return sp->threads > 0 ? sp->threads : (int)std::max(1u, std::thread::hardware_concurrency());
+}
+
+void fiu_synth_spec_init(fiu_synth_spec *spec)
+{
+    if (!spec) return;
+    std::memset(spec, 0, sizeof(*spec));
+    spec->struct_size = sizeof(fiu_synth_spec);
+    spec->maf_lo = 0.05;
+    spec->maf_hi = 0.5;
+    spec->low_lo = 0.01;
+    spec->low_hi = 0.05;
+    spec->seed = 1;
+    spec->penetrance_seed = 2;
+    spec->pen_lo = 0.3;
+    spec->pen_hi = 0.7;
+}
+
+fiu_status fiu_synth_generate(const fiu_synth_spec *spec, uint64_t *words, int8_t *geno, size_t *cases_words,
+                              size_t *ctrls_words)
+{
+    return guard([&] {
+        const SynthSpec s = to_spec(spec);
+        if (cases_words) *cases_words = synth_words(s.cases);
+        if (ctrls_words) *ctrls_words = synth_words(s.ctrls);
+        if (words || geno) synth_generate(s, words, geno, synth_threads(spec));
+    });
+}
+
+fiu_status fiu_dataset_synth(const fiu_synth_spec *spec, fiu_dataset **out)
+{
+    return guard([&] {
+        if (!out) fail(FIU_ERR_INVALID, "out is NULL");
+        const SynthSpec s = to_spec(spec);
+        const size_t cw = synth_words(s.cases), tw = synth_words(s.ctrls);
+        std::vector<uint64_t> words(s.snps * 3 * (cw + tw));
+        synth_generate(s, words.data(), nullptr, synth_threads(spec));
+        *out = create(words.data(), s.snps, cw, tw, s.cases, s.ctrls);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,79 @@
+// Device-side data structures shared by the gfx950 kernels and the host
+// runtime of fiuncho_amd.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fiu {
+
+constexpr int kWave = 64;            // CDNA wavefront width
+constexpr int kBlock = 256;          // threads per scan workgroup (4 waves)
+constexpr int kMaxOrder = 4;         // orders with a scan kernel
+constexpr int kMaxPrefix = 7;        // longest prefix the unranker handles
+constexpr int kMergeSlots = 8192;    // LDS entries of the top-N merge
+constexpr int kMaxOutputs = kMergeSlots / 2;
+
+// Device image of a dataset.
+//
+// Bits are stored "SNP-minor": for class c (0 cases, 1 controls), the 128-bit
+// quad q (32-bit words 4q..4q+3 of the reference row) of genotype row z of
+// SNP s lives at bits[class_off[c] + (q*3 + z)*n_pad + s].  A wavefront whose
+// 64 lanes own 64 consecutive SNPs therefore reads one contiguous 1 KiB line
+// per (q, z), and a wave-uniform SNP is one 16-byte scalar load.
+struct DevData {
+    const uint4 *bits;
+    uint64_t class_off[2];   // uint4 offset of each class
+    int32_t quads[2];        // 128-bit quads per row, per class
+    int32_t n;               // SNPs
+    int32_t n_pad;           // SNPs rounded up to a multiple of 64
+    uint32_t cases, ctrls;
+    const uint32_t *singles; // [n_pad][2][3] popcount of each row
+    uint32_t total[2];       // samples covered per class (uniform datasets)
+    int32_t uniform;         // 1 if every SNP covers the same samples
+    int32_t pad_;
+};
+
+// A top-N candidate: float MI, then the combination packed with the first
+// SNP in the most significant field, so that numeric order of `combo` is the
+// lexicographic order of combinations.
+struct Cand {
+    float mi;
+    uint32_t tag;            // dataset / request id in batched scans
+    uint64_t combo;
+};
+
+// Device state of one running top-N selection.
+struct TopState {
+    float tau;               // MI of the current N-th best (or -inf)
+    uint32_t n_top;          // entries in the top list
+    uint64_t tau_combo;      // combination of the current N-th best
+    uint32_t cand_count;     // candidates pushed since the last merge
+    uint32_t overflow;       // a push did not fit in the buffer
+    uint32_t redo_count;     // chunks that must be scanned again
+    uint32_t pad_;
+};
+
+struct ScanParams {
+    int64_t j0, j1;          // prefix indices [j0, j1) of the distribution
+    int64_t step, offset;    // Distribution step / offset (prefix lex ranks)
+    uint64_t total_prefixes; // C(n, order-1)
+    const uint64_t *binom;   // binom[j*(n+1)+d] = C(d, j), j <= order-1
+    Cand *cand;              // candidate buffer
+    uint32_t cand_cap;
+    int32_t combo_bits;      // bits per SNP index in Cand::combo
+    float delta;             // screening margin on MI
+    float hy;                // H(Y) in nats (from FP64)
+    float inv_n;             // 1 / (cases + ctrls)
+    uint32_t tag;
+    double hy64, n64;        // H(Y) and cases+ctrls in FP64
+    const uint32_t *pair;    // marginal pair table (order 3), or null
+};
+
+__host__ __device__ inline uint64_t pair_index(int64_t b, int64_t c, int64_t n)
+{
+    // row-major strict upper triangle: pairs (b, c), b < c < n
+    return (uint64_t)(b * (2 * n - b - 1) / 2 + (c - b - 1));
+}
+
+}  // namespace fiu

@@ -1,0 +1,22 @@
+// Host-side launchers of the gfx950 kernels (defined in fiu_kernels.hip).
+#pragma once
+
+#include "fiu_device.h"
+
+namespace fiu {
+
+hipError_t launch_scan_direct(int order, const uint4 *bits, const DevData &D, const ScanParams &P, TopState *st,
+                              hipStream_t s);
+hipError_t launch_merge(TopState *st, Cand *top, const Cand *cand, uint32_t N, uint32_t cap, uint32_t chunk_id,
+                        uint32_t *redo, hipStream_t s);
+hipError_t launch_transpose(const uint64_t *src, uint4 *dst, int n, int n_pad, int cls, uint64_t cw, uint64_t tw,
+                            int quads, uint64_t class_off, hipStream_t s);
+hipError_t launch_singles(const uint4 *bits, const DevData &D, uint32_t *singles, uint32_t *flags, hipStream_t s);
+hipError_t launch_combine_batch(const uint4 *bits, const DevData &D, const int32_t *combos, int k, uint64_t n,
+                                uint64_t cw, uint64_t tw, uint64_t *out, hipStream_t s);
+hipError_t launch_contingency_batch(const uint4 *bits, const DevData &D, const int32_t *combos, int k, uint64_t n,
+                                    uint64_t ct_size, uint32_t *out, hipStream_t s);
+hipError_t launch_mi_batch(const uint32_t *tables, uint64_t n, uint64_t ct_size, double n64, double hy64, float *out,
+                           hipStream_t s);
+
+}  // namespace fiu
