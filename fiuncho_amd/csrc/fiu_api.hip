@@ -8,6 +8,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <string>
 #include <vector>
@@ -121,6 +122,8 @@ struct fiu_dataset {
     DBuf<Cand> cand, top;
     DBuf<TopState> state;
     DBuf<uint32_t> redo;
+    DBuf<uint32_t> pair;        // marginal pair table (order-3 marginal scans)
+    bool pair_ready = false;
     std::vector<hipEvent_t> events;
 
     ~fiu_dataset()
@@ -230,6 +233,65 @@ struct Chunk {
     int64_t j0, j1;
 };
 
+// Pair counts f(x, y), x, y in {0, 1}, of every SNP pair, both classes
+// (32 bytes per pair).  Returns false when the table does not fit in the
+// free device memory.
+bool ensure_pair_table(fiu_dataset *d, hipStream_t s)
+{
+    if (d->pair_ready) return true;
+    const uint64_t n = d->snps, pairs = n * (n - 1) / 2;
+    size_t free_b = 0, total_b = 0;
+    check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+    if (pairs * 32 + (64ull << 20) > free_b) return false;
+    ensure_binom(d, 1, s);
+    d->pair.alloc(pairs * 8);
+    ScanParams P{};
+    P.j0 = 0;
+    P.j1 = (int64_t)n;
+    P.step = 1;
+    P.offset = 0;
+    P.total_prefixes = n;
+    P.binom = d->binom.p;
+    check(launch_pair_table(d->bits.p, d->dev, P, d->pair.p, s), "pair table launch");
+    check(hipStreamSynchronize(s), "pair table sync");
+    d->pair_ready = true;
+    return true;
+}
+
+// Exact number of combinations whose (m)-prefix has lexicographic rank
+// = offset (mod step): sum over prefixes of n - 1 - last.  The prefixes that
+// share their first m-1 indices form a run of consecutive ranks whose last
+// index increases by one; each run is summed in closed form.
+uint64_t count_combinations(uint64_t n, int m, uint64_t step, uint64_t offset)
+{
+    uint64_t total = 0, rank = 0;
+    std::vector<int64_t> c(m);
+    // iterate the (m-1)-prefixes of the m-prefixes in lexicographic order
+    std::function<void(int, int64_t)> rec = [&](int depth, int64_t start) {
+        if (depth == m - 1) {
+            const int64_t first_last = depth ? c[depth - 1] + 1 : 0;
+            if (first_last >= (int64_t)n) return;
+            const uint64_t L = n - (uint64_t)first_last;  // runs of last = first_last .. n-1
+            // first local index whose rank is offset + j*step, j >= 0
+            const uint64_t first =
+                rank >= offset ? (step - (rank - offset) % step) % step : offset - rank;
+            if (first < L) {
+                const uint64_t cnt = (L - 1 - first) / step + 1, t_last = first + (cnt - 1) * step;
+                // combos for local index t: n - 1 - (first_last + t) = L - 1 - t
+                total += cnt * (L - 1) - (first + t_last) * cnt / 2;
+            }
+            rank += L;
+            return;
+        }
+        for (int64_t v = start; v < (int64_t)n; ++v) {
+            c[depth] = v;
+            rec(depth + 1, v + 1);
+        }
+    };
+    if (m >= 1) rec(0, 0);
+    return total;
+}
+
 void search(fiu_dataset *d, int order, int32_t step, int32_t offset, uint32_t outputs, const fiu_config *cfg,
             int32_t *out_combos, float *out_mi, uint32_t *out_count, fiu_search_stats *stats)
 {
@@ -263,6 +325,12 @@ void search(fiu_dataset *d, int order, int32_t step, int32_t offset, uint32_t ou
     hipStream_t s = stream_of(d, cfg);
     check(hipSetDevice(d->device), "hipSetDevice");
     ensure_binom(d, M, s);
+    int algo = FIU_ALGO_DIRECT;
+    if (algo_req != FIU_ALGO_DIRECT && d->dev.uniform && (order == 2 || order == 3)) {
+        algo = FIU_ALGO_MARGINAL;
+        if (order == 3 && !ensure_pair_table(d, s)) algo = FIU_ALGO_DIRECT;  // table does not fit
+    }
+    st.algorithm = algo;
 
     const uint32_t cap = cfg && cfg->candidate_cap ? cfg->candidate_cap : (1u << 20);
     d->cand.alloc(cap);
@@ -300,6 +368,7 @@ void search(fiu_dataset *d, int order, int32_t step, int32_t offset, uint32_t ou
     P.cand_cap = cap;
     P.combo_bits = bits;
     P.tag = 0;
+    P.pair = d->pair_ready ? d->pair.p : nullptr;
     mi_constants(d->cases, d->ctrls, P);
 
     while (d->events.size() < 2 * chunks.size() + 2) {
@@ -312,7 +381,10 @@ void search(fiu_dataset *d, int order, int32_t step, int32_t offset, uint32_t ou
         P.j0 = chunks[ci].j0;
         P.j1 = chunks[ci].j1;
         if (timed) check(hipEventRecord(d->events[ev], s), "event");
-        check(launch_scan_direct(order, d->bits.p, d->dev, P, d->state.p, s), "scan launch");
+        if (algo == FIU_ALGO_MARGINAL)
+            check(launch_scan_marginal(order, d->bits.p, d->dev, P, d->state.p, s), "scan launch");
+        else
+            check(launch_scan_direct(order, d->bits.p, d->dev, P, d->state.p, s), "scan launch");
         if (timed) check(hipEventRecord(d->events[ev + 1], s), "event");
         if (timed) ev += 2;
         check(launch_merge(d->state.p, d->top.p, d->cand.p, outputs, cap, (uint32_t)ci, d->redo.p, s), "merge launch");
@@ -352,13 +424,10 @@ void search(fiu_dataset *d, int order, int32_t step, int32_t offset, uint32_t ou
         out_mi[i] = top[i].mi;
     }
     *out_count = hs.n_top;
-    // combinations evaluated: sum over the distribution's prefixes
-    st.combinations = 0;
-    {
-        // exact count by walking prefixes is costly for huge scans; use the
-        // closed form when the distribution covers every prefix
-        if (step == 1 && offset == 0) st.combinations = binom_sat((uint64_t)n, (uint64_t)order);
-    }
+    // combinations evaluated: closed form for the whole space, else a walk
+    // over the runs of prefixes (O(C(n, order-2)))
+    st.combinations = (step == 1 && offset == 0) ? binom_sat((uint64_t)n, (uint64_t)order)
+                                                 : count_combinations((uint64_t)n, M, (uint64_t)step, (uint64_t)offset);
     st.scan_ms = scan_ms;
     st.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
     if (stats) *stats = st;

@@ -11,6 +11,8 @@ constexpr int kWave = 64;            // CDNA wavefront width
 constexpr int kBlock = 256;          // threads per scan workgroup (4 waves)
 constexpr int kMaxOrder = 4;         // orders with a scan kernel
 constexpr int kMaxPrefix = 7;        // longest prefix the unranker handles
+constexpr int kMarginalPW = 4;       // prefixes per wavefront in marginal scans
+constexpr int kWindowQuads = 8;      // quads per LDS-staged window of a tile
 constexpr int kMergeSlots = 8192;    // LDS entries of the top-N merge
 constexpr int kMaxOutputs = kMergeSlots / 2;
 

@@ -7,6 +7,10 @@ namespace fiu {
 
 hipError_t launch_scan_direct(int order, const uint4 *bits, const DevData &D, const ScanParams &P, TopState *st,
                               hipStream_t s);
+hipError_t launch_scan_marginal(int order, const uint4 *bits, const DevData &D, const ScanParams &P, TopState *st,
+                                hipStream_t s);
+hipError_t launch_pair_table(const uint4 *bits, const DevData &D, const ScanParams &P, uint32_t *pair_out,
+                             hipStream_t s);
 hipError_t launch_merge(TopState *st, Cand *top, const Cand *cand, uint32_t N, uint32_t cap, uint32_t chunk_id,
                         uint32_t *redo, hipStream_t s);
 hipError_t launch_transpose(const uint64_t *src, uint4 *dst, int n, int n_pad, int cls, uint64_t cw, uint64_t tw,

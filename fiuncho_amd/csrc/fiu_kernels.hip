@@ -28,6 +28,105 @@ __device__ __forceinline__ uint32_t comp(const uint4 &v, int w)
     return w == 0 ? v.x : w == 1 ? v.y : w == 2 ? v.z : v.w;
 }
 
+// acc += popcount(s & v) as exactly two VALU instructions: v_and_b32 with the
+// wave-uniform operand in an SGPR, then the accumulating v_bcnt_u32_b32.
+// (Left to itself the compiler re-associates the sums into
+// v_bcnt(x, 0) + v_add3_u32, one extra instruction per two cells.)
+__device__ __forceinline__ void and_bcnt(uint32_t &acc, uint32_t s, uint32_t v)
+{
+    uint32_t t;
+    asm("v_and_b32 %0, %1, %2" : "=v"(t) : "s"(s), "v"(v));
+    asm("v_bcnt_u32_b32 %0, %1, %0" : "+v"(acc) : "v"(t));
+}
+
+// Buffer resource over the three genotype rows of one quad (all SNPs of one
+// class); loads through it are in-order vector loads whose offset can live in
+// an SGPR (wave-uniform prefix rows) or a VGPR (the lanes' own SNPs), with no
+// 64-bit address arithmetic, and reads past the rows return zero.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t quad_rsrc(const uint4 *row, int np)
+{
+    return __builtin_amdgcn_make_buffer_rsrc((void *)row, (short)0, 3 * np * 16, 0x00020000);
+}
+
+// Row z of SNP s (wave-uniform s): 16 bytes broadcast to every lane.
+__device__ __forceinline__ uint4 ld_uniform(__amdgpu_buffer_rsrc_t r, int np, int z, int s)
+{
+    const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, 0, (z * np + s) * 16, 0);
+    return make_uint4(v[0], v[1], v[2], v[3]);
+}
+
+// Row z of each lane's SNP (voff = 16 * SNP index).
+__device__ __forceinline__ uint4 ld_lane(__amdgpu_buffer_rsrc_t r, int np, int z, int voff)
+{
+    const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, voff, z * np * 16, 0);
+    return make_uint4(v[0], v[1], v[2], v[3]);
+}
+
+// acc_k += popcount(x_k & y_k & z_k) for four independent cells: one
+// v_bitop3_b32 (3-input AND) and one accumulating v_bcnt_u32_b32 per cell,
+// the four ANDs issued before the four popcounts.
+__device__ __forceinline__ void and3_bcnt4(uint32_t &a0, uint32_t &a1, uint32_t &a2, uint32_t &a3, uint32_t x0,
+                                           uint32_t y0, uint32_t z0, uint32_t x1, uint32_t y1, uint32_t z1,
+                                           uint32_t x2, uint32_t y2, uint32_t z2, uint32_t x3, uint32_t y3,
+                                           uint32_t z3)
+{
+    uint32_t t0, t1, t2, t3;
+    asm("v_bitop3_b32 %0, %4, %5, %6 bitop3:0x80\n\t"
+        "v_bitop3_b32 %1, %7, %8, %9 bitop3:0x80\n\t"
+        "v_bitop3_b32 %2, %10, %11, %12 bitop3:0x80\n\t"
+        "v_bitop3_b32 %3, %13, %14, %15 bitop3:0x80"
+        : "=&v"(t0), "=&v"(t1), "=&v"(t2), "=&v"(t3)
+        : "v"(x0), "v"(y0), "v"(z0), "v"(x1), "v"(y1), "v"(z1), "v"(x2), "v"(y2), "v"(z2), "v"(x3), "v"(y3),
+          "v"(z3));
+    asm("v_bcnt_u32_b32 %0, %4, %0\n\t"
+        "v_bcnt_u32_b32 %1, %5, %1\n\t"
+        "v_bcnt_u32_b32 %2, %6, %2\n\t"
+        "v_bcnt_u32_b32 %3, %7, %3"
+        : "+v"(a0), "+v"(a1), "+v"(a2), "+v"(a3)
+        : "v"(t0), "v"(t1), "v"(t2), "v"(t3));
+}
+
+// acc_k += popcount(x_k & y_k), four independent cells, VGPR operands.
+__device__ __forceinline__ void and2_bcnt4(uint32_t &a0, uint32_t &a1, uint32_t &a2, uint32_t &a3, uint32_t x0,
+                                           uint32_t y0, uint32_t x1, uint32_t y1, uint32_t x2, uint32_t y2,
+                                           uint32_t x3, uint32_t y3)
+{
+    uint32_t t0, t1, t2, t3;
+    asm("v_and_b32 %0, %4, %5\n\t"
+        "v_and_b32 %1, %6, %7\n\t"
+        "v_and_b32 %2, %8, %9\n\t"
+        "v_and_b32 %3, %10, %11"
+        : "=&v"(t0), "=&v"(t1), "=&v"(t2), "=&v"(t3)
+        : "v"(x0), "v"(y0), "v"(x1), "v"(y1), "v"(x2), "v"(y2), "v"(x3), "v"(y3));
+    asm("v_bcnt_u32_b32 %0, %4, %0\n\t"
+        "v_bcnt_u32_b32 %1, %5, %1\n\t"
+        "v_bcnt_u32_b32 %2, %6, %2\n\t"
+        "v_bcnt_u32_b32 %3, %7, %3"
+        : "+v"(a0), "+v"(a1), "+v"(a2), "+v"(a3)
+        : "v"(t0), "v"(t1), "v"(t2), "v"(t3));
+}
+
+// Four independent cells at once: the four ANDs are issued before the four
+// popcounts, so no popcount waits on the AND just before it.
+__device__ __forceinline__ void and_bcnt4(uint32_t &a0, uint32_t &a1, uint32_t &a2, uint32_t &a3, uint32_t s0,
+                                          uint32_t v0, uint32_t s1, uint32_t v1, uint32_t s2, uint32_t v2,
+                                          uint32_t s3, uint32_t v3)
+{
+    uint32_t t0, t1, t2, t3;
+    asm("v_and_b32 %0, %4, %5\n\t"
+        "v_and_b32 %1, %6, %7\n\t"
+        "v_and_b32 %2, %8, %9\n\t"
+        "v_and_b32 %3, %10, %11"
+        : "=&v"(t0), "=&v"(t1), "=&v"(t2), "=&v"(t3)
+        : "s"(s0), "v"(v0), "s"(s1), "v"(v1), "s"(s2), "v"(v2), "s"(s3), "v"(v3));
+    asm("v_bcnt_u32_b32 %0, %4, %0\n\t"
+        "v_bcnt_u32_b32 %1, %5, %1\n\t"
+        "v_bcnt_u32_b32 %2, %6, %2\n\t"
+        "v_bcnt_u32_b32 %3, %7, %3"
+        : "+v"(a0), "+v"(a1), "+v"(a2), "+v"(a3)
+        : "v"(t0), "v"(t1), "v"(t2), "v"(t3));
+}
+
 // Lexicographic unranking of an m-combination of [0, n) (wave-uniform).
 // rank(c) = C(n,m) - 1 - sum_i C(n-1-c_i, m-i), so the complement is written
 // greedily in the combinatorial number system.
@@ -57,16 +156,19 @@ __device__ inline void unrank(uint64_t r, int n, int m, const uint64_t *binom, i
 template <int CELLS>
 __device__ __forceinline__ float mi_screen(const uint32_t *a0, const uint32_t *a1, float hy, float inv_n)
 {
+    // hardware v_rcp_f32 / v_log_f32 (log2) on normal arguments: every ratio
+    // is clamped to >= 2^-100, and a zero count multiplies its log by zero
     float s = 0.0f;
 #pragma unroll
     for (int x = 0; x < CELLS; ++x) {
         const float fc = (float)a0[x], ft = (float)a1[x];
-        const float r = __builtin_amdgcn_rcpf(fc + ft);
-        const float lc = __logf(fmaxf(fc * r, 1e-30f));
-        const float lt = __logf(fmaxf(ft * r, 1e-30f));
-        s += fc * lc + ft * lt;
+        const float r = __builtin_amdgcn_rcpf(fmaxf(fc + ft, 1.0f));
+        const float lc = __builtin_amdgcn_logf(fmaxf(fc * r, 7.9e-31f));
+        const float lt = __builtin_amdgcn_logf(fmaxf(ft * r, 7.9e-31f));
+        s = fmaf(fc, lc, s);
+        s = fmaf(ft, lt, s);
     }
-    return hy + s * inv_n;
+    return fmaf(s, inv_n * 0.69314718f, hy);
 }
 
 // p*ln(p) in FP64 for p > 0, else 0 (kept out of line: it is only reached for
@@ -147,49 +249,110 @@ __device__ __forceinline__ void score_and_push(bool valid, const uint32_t *a0, c
 // ---------------------------------------------------------------------------
 // Direct scan: all 3^k cells by AND + popcount (the reference's algorithm).
 // ---------------------------------------------------------------------------
+// One 32-bit word of every cell: cell (g_0 .. g_{k-1}) gets
+// popcount(row g_0 of the first SNP & ... & row g_{k-1} of the lane's SNP).
+// Orders 2 and 3 take one v_and_b32 / v_bitop3_b32 per cell; order 4 first
+// forms the 9 pair words of the first two SNPs (9 VALU per word for 81 cells).
+template <int K>
+__device__ __forceinline__ void count_word(uint32_t *acc, const uint32_t (&p)[K - 1][3], const uint32_t (&x)[3])
+{
+    constexpr int CELLS = Pow3<K>::v;
+    uint32_t o0[CELLS], o1[CELLS], o2[CELLS];
+    if constexpr (K == 2) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int z = 0; z < 3; ++z) o0[a * 3 + z] = p[0][a], o1[a * 3 + z] = x[z];
+    } else if constexpr (K == 3) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 3; ++b)
+#pragma unroll
+                for (int z = 0; z < 3; ++z)
+                    o0[(a * 3 + b) * 3 + z] = p[0][a], o1[(a * 3 + b) * 3 + z] = p[1][b],
+                                         o2[(a * 3 + b) * 3 + z] = x[z];
+    } else {
+        uint32_t ab[9];
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 3; ++b) ab[a * 3 + b] = p[0][a] & p[1][b];
+#pragma unroll
+        for (int r = 0; r < 9; ++r)
+#pragma unroll
+            for (int e = 0; e < 3; ++e)
+#pragma unroll
+                for (int z = 0; z < 3; ++z)
+                    o0[(r * 3 + e) * 3 + z] = ab[r], o1[(r * 3 + e) * 3 + z] = p[2][e], o2[(r * 3 + e) * 3 + z] = x[z];
+    }
+    constexpr int FULL = CELLS / 4 * 4;
+#pragma unroll
+    for (int i = 0; i < FULL; i += 4) {
+        if constexpr (K == 2)
+            and2_bcnt4(acc[i], acc[i + 1], acc[i + 2], acc[i + 3], o0[i], o1[i], o0[i + 1], o1[i + 1], o0[i + 2],
+                       o1[i + 2], o0[i + 3], o1[i + 3]);
+        else
+            and3_bcnt4(acc[i], acc[i + 1], acc[i + 2], acc[i + 3], o0[i], o1[i], o2[i], o0[i + 1], o1[i + 1],
+                       o2[i + 1], o0[i + 2], o1[i + 2], o2[i + 2], o0[i + 3], o1[i + 3], o2[i + 3]);
+    }
+#pragma unroll
+    for (int i = FULL; i < CELLS; ++i) {
+        if constexpr (K == 2) acc[i] += __builtin_popcount(o0[i] & o1[i]);
+        else acc[i] += __builtin_popcount(o0[i] & o1[i] & o2[i]);
+    }
+}
+
+// All 3^k cells of one class for the lanes' SNPs, both the lanes' rows and
+// the prefix rows of the next quad in flight while the current quad counts.
 template <int K>
 __device__ __forceinline__ void accumulate_direct(const uint4 *__restrict__ base, int np, int nq,
-                                                  const int *pre, int c, uint32_t *acc)
+                                                  const int *pre, int voff, uint32_t *acc)
 {
-    constexpr int M = K - 1, R = Pow3<M>::v;
+    constexpr int M = K - 1, CELLS = Pow3<K>::v;
 #pragma unroll
-    for (int i = 0; i < R * 3; ++i) acc[i] = 0;
+    for (int i = 0; i < CELLS; ++i) acc[i] = 0;
+    if (nq <= 0) return;
+    const uint4 *row = base;
+    __amdgpu_buffer_rsrc_t r = quad_rsrc(row, np);
+    uint4 cz[3], pz[M][3];
+#pragma unroll
+    for (int z = 0; z < 3; ++z) cz[z] = ld_lane(r, np, z, voff);
+#pragma unroll
+    for (int i = 0; i < M; ++i)
+#pragma unroll
+        for (int z = 0; z < 3; ++z) pz[i][z] = ld_uniform(r, np, z, pre[i]);
     for (int q = 0; q < nq; ++q) {
-        const uint4 *row = base + (size_t)q * 3 * np;
-        const uint4 c0 = row[c], c1 = row[np + c], c2 = row[2 * np + c];
-        uint4 pz[M][3];
+        const uint4 *nrow = q + 1 < nq ? row + (size_t)3 * np : row;
+        const __amdgpu_buffer_rsrc_t nr = quad_rsrc(nrow, np);
+        uint4 ncz[3], npz[M][3];
+#pragma unroll
+        for (int z = 0; z < 3; ++z) ncz[z] = ld_lane(nr, np, z, voff);
 #pragma unroll
         for (int i = 0; i < M; ++i)
 #pragma unroll
-            for (int z = 0; z < 3; ++z) pz[i][z] = row[z * np + pre[i]];
+            for (int z = 0; z < 3; ++z) npz[i][z] = ld_uniform(nr, np, z, pre[i]);
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int w = 0; w < 4; ++w) {
-            uint32_t pr[R];
-            pr[0] = comp(pz[0][0], w);
-            pr[1] = comp(pz[0][1], w);
-            pr[2] = comp(pz[0][2], w);
-            int size = 3;
+            uint32_t p[M][3], x[3];
 #pragma unroll
-            for (int l = 1; l < M; ++l) {
+            for (int i = 0; i < M; ++i)
 #pragma unroll
-                for (int r = Pow3<M>::v / 3 - 1; r >= 0; --r) {
-                    if (r < size) {
-                        const uint32_t v = pr[r];
-                        pr[r * 3 + 2] = v & comp(pz[l][2], w);
-                        pr[r * 3 + 1] = v & comp(pz[l][1], w);
-                        pr[r * 3 + 0] = v & comp(pz[l][0], w);
-                    }
-                }
-                size *= 3;
-            }
-            const uint32_t x0 = comp(c0, w), x1 = comp(c1, w), x2 = comp(c2, w);
+                for (int z = 0; z < 3; ++z) p[i][z] = comp(pz[i][z], w);
 #pragma unroll
-            for (int r = 0; r < R; ++r) {
-                acc[r * 3 + 0] += __builtin_popcount(pr[r] & x0);
-                acc[r * 3 + 1] += __builtin_popcount(pr[r] & x1);
-                acc[r * 3 + 2] += __builtin_popcount(pr[r] & x2);
-            }
+            for (int z = 0; z < 3; ++z) x[z] = comp(cz[z], w);
+            count_word<K>(acc, p, x);
         }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int z = 0; z < 3; ++z) cz[z] = ncz[z];
+#pragma unroll
+        for (int i = 0; i < M; ++i)
+#pragma unroll
+            for (int z = 0; z < 3; ++z) pz[i][z] = npz[i][z];
+        r = nr;
+        row = nrow;
     }
 }
 
@@ -218,9 +381,339 @@ k_scan_direct(const uint4 *__restrict__ bits, const DevData D, const ScanParams 
     for (int t = (last + 1) >> 6; t < D.n_pad >> 6; ++t) {
         const int c = (t << 6) | lane;
         uint32_t a0[CELLS], a1[CELLS];
-        accumulate_direct<K>(b0, D.n_pad, D.quads[0], pre, c, a0);
-        accumulate_direct<K>(b1, D.n_pad, D.quads[1], pre, c, a1);
+        accumulate_direct<K>(b0, D.n_pad, D.quads[0], pre, c * 16, a0);
+        accumulate_direct<K>(b1, D.n_pad, D.quads[1], pre, c * 16, a1);
         score_and_push<CELLS>(c > last && c < D.n, a0, a1, pre_packed | (uint64_t)c, tau, tau_combo, P, st);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Marginal scans: only the 2^k cells whose genotypes are all 0 or 1 are
+// counted by AND + popcount; every other cell follows exactly, by
+// inclusion-exclusion, from counts of fewer SNPs (single-row popcounts, the
+// pair table, the class total).  Valid when every SNP's three rows are
+// disjoint and cover the same samples (checked at upload).
+//
+// A wavefront takes PW consecutive prefixes of the distribution; the 64
+// lanes' last-SNP words are loaded once per quad and shared by all PW
+// prefixes, whose rows are wave-uniform scalar loads.
+// ---------------------------------------------------------------------------
+
+// In-place transform of one class's 3^3 table from the marginal basis
+// (index 2 = "any genotype") to genotype 2, dimension by dimension.
+__device__ __forceinline__ void mobius3(uint32_t *m)
+{
+#pragma unroll
+    for (int ij = 0; ij < 9; ++ij) m[ij * 3 + 2] -= m[ij * 3 + 0] + m[ij * 3 + 1];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) m[i * 9 + 6 + k] -= m[i * 9 + k] + m[i * 9 + 3 + k];
+#pragma unroll
+    for (int jk = 0; jk < 9; ++jk) m[18 + jk] -= m[jk] + m[9 + jk];
+}
+
+__device__ __forceinline__ void mobius2(uint32_t *m)
+{
+#pragma unroll
+    for (int i = 0; i < 3; ++i) m[i * 3 + 2] -= m[i * 3 + 0] + m[i * 3 + 1];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) m[6 + k] -= m[k] + m[3 + k];
+}
+
+// Uniform lookup of a prefix: rank -> SNP indices, or an empty marker.
+template <int M>
+__device__ __forceinline__ bool prefix_at(const ScanParams &P, int n, int64_t j, int *pre)
+{
+    if (j >= P.j1) return false;
+    const uint64_t rank = (uint64_t)P.offset + (uint64_t)j * (uint64_t)P.step;
+    if (rank >= P.total_prefixes) return false;
+    unrank(rank, n, M, P.binom, pre);
+    return true;
+}
+
+// Order 2.  TABLE = true writes the 8 pair counts (f(x,y), x,y in {0,1}, per
+// class) of every pair instead of scoring it.
+template <int PW, bool TABLE>
+__global__ void __launch_bounds__(kBlock)
+k_scan2m(const uint4 *__restrict__ bits, const DevData D, const ScanParams P, TopState *st, uint32_t *pair_out)
+{
+    const int lane = threadIdx.x & 63;
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t g0 = P.j0 + ((int64_t)blockIdx.x * (kBlock / kWave) + wid) * PW;
+    int pa[PW];
+    bool pv[PW];
+    int amin = D.n_pad;
+#pragma unroll
+    for (int i = 0; i < PW; ++i) {
+        int tmp[1];
+        pv[i] = prefix_at<1>(P, D.n, g0 + i, tmp);
+        pa[i] = pv[i] ? tmp[0] : D.n_pad;
+        amin = min(amin, pa[i]);
+    }
+    if (!pv[0]) return;  // slots fill in order: slot 0 empty means all are
+    // an empty slot counts a copy of prefix 0 (discarded) so loads stay in range
+#pragma unroll
+    for (int i = 1; i < PW; ++i)
+        if (!pv[i]) pa[i] = pa[0];
+    const float tau = TABLE ? 0.0f : st->tau;
+    const uint64_t tau_combo = TABLE ? 0 : st->tau_combo;
+    const int np = D.n_pad;
+    for (int t = (amin + 1) >> 6; t < np >> 6; ++t) {
+        const int c = (t << 6) | lane;
+        const int tile_last = (t << 6) | 63;
+        uint32_t f[PW][2][4];
+        bool live[PW];
+#pragma unroll
+        for (int i = 0; i < PW; ++i) {
+            live[i] = pv[i] && tile_last > pa[i];
+#pragma unroll
+            for (int x = 0; x < 8; ++x) f[i][x >> 2][x & 3] = 0;
+        }
+#pragma unroll
+        for (int cls = 0; cls < 2; ++cls) {
+            const int nq = D.quads[cls];
+            if (nq <= 0) continue;
+            const uint4 *row = bits + D.class_off[cls];
+            const int voff = c * 16;
+            // same issue order as the order-3 scan: prefix i+1's rows (two
+            // register sets, alternately) before prefix i is counted, the
+            // lanes' next quad before the last two prefixes of this one
+            __amdgpu_buffer_rsrc_t r = quad_rsrc(row, np);
+            uint4 c0 = ld_lane(r, np, 0, voff), c1 = ld_lane(r, np, 1, voff);
+            uint4 X[2], Y[2];
+            X[0] = ld_uniform(r, np, 0, pa[0]);
+            X[1] = ld_uniform(r, np, 1, pa[0]);
+            for (int q = 0; q < nq; ++q) {
+                const uint4 *nrow = q + 1 < nq ? row + (size_t)3 * np : row;
+                const __amdgpu_buffer_rsrc_t nr = quad_rsrc(nrow, np);
+                uint4 n0, n1;
+#pragma unroll
+                for (int i = 0; i < PW; ++i) {
+                    if (i == PW - 2 || (PW == 1 && i == 0)) {
+                        n0 = ld_lane(nr, np, 0, voff);
+                        n1 = ld_lane(nr, np, 1, voff);
+                    }
+                    const __amdgpu_buffer_rsrc_t r2 = i + 1 < PW ? r : nr;
+                    const int ia = pa[(i + 1) % PW];
+                    uint4 *nxt = (i & 1) ? X : Y;
+                    nxt[0] = ld_uniform(r2, np, 0, ia);
+                    nxt[1] = ld_uniform(r2, np, 1, ia);
+                    __builtin_amdgcn_sched_barrier(0);
+                    const uint4 *cur = (i & 1) ? Y : X;
+                    if (live[i]) {  // uniform
+#pragma unroll
+                        for (int w = 0; w < 4; ++w) {
+                            const uint32_t x0 = comp(c0, w), x1 = comp(c1, w);
+                            const uint32_t p0 = comp(cur[0], w), p1 = comp(cur[1], w);
+                            and2_bcnt4(f[i][cls][0], f[i][cls][1], f[i][cls][2], f[i][cls][3], p0, x0, p0, x1, p1,
+                                       x0, p1, x1);
+                        }
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+                c0 = n0;
+                c1 = n1;
+                row = nrow;
+                r = nr;
+            }
+        }
+        const bool in_range = c < D.n;
+        uint32_t sc[2][2] = {{0, 0}, {0, 0}};
+        if (!TABLE && in_range) {
+            const uint32_t *sp = D.singles + (size_t)c * 6;
+            sc[0][0] = sp[0], sc[0][1] = sp[1], sc[1][0] = sp[3], sc[1][1] = sp[4];
+        }
+#pragma unroll
+        for (int i = 0; i < PW; ++i) {
+            if (!live[i]) continue;  // uniform
+            const bool valid = in_range && c > pa[i];
+            if (TABLE) {
+                if (valid) {
+                    uint4 *o = (uint4 *)(pair_out + pair_index(pa[i], c, D.n) * 8);
+                    o[0] = make_uint4(f[i][0][0], f[i][0][1], f[i][0][2], f[i][0][3]);
+                    o[1] = make_uint4(f[i][1][0], f[i][1][1], f[i][1][2], f[i][1][3]);
+                }
+                continue;
+            }
+            uint32_t m0[9], m1[9];
+            const uint32_t *sa = D.singles + (size_t)pa[i] * 6;
+#pragma unroll
+            for (int cls = 0; cls < 2; ++cls) {
+                uint32_t *m = cls ? m1 : m0;
+                m[0] = f[i][cls][0];
+                m[1] = f[i][cls][1];
+                m[3] = f[i][cls][2];
+                m[4] = f[i][cls][3];
+                m[2] = sa[cls * 3 + 0];
+                m[5] = sa[cls * 3 + 1];
+                m[6] = sc[cls][0];
+                m[7] = sc[cls][1];
+                m[8] = D.total[cls];
+                mobius2(m);
+            }
+            score_and_push<9>(valid, m0, m1, ((uint64_t)pa[i] << P.combo_bits) | (uint64_t)c, tau, tau_combo, P, st);
+        }
+    }
+}
+
+// Order 3.  Per quad, the lanes' last-SNP rows for the next quad are in
+// flight while this quad is counted.  The prefix rows are pipelined one
+// prefix ahead: the pair rows P_xy = a_x & b_y of the prefix about to be
+// counted are formed on the scalar unit first (this waits for its loads),
+// then the loads of the next prefix are issued, then the 64 VALU
+// instructions run -- so every scalar load has one prefix of counting to
+// arrive (scalar loads complete out of order; a wait covers all of them).
+template <int PW>
+__global__ void __launch_bounds__(kBlock)
+k_scan3m(const uint4 *__restrict__ bits, const DevData D, const ScanParams P, TopState *st)
+{
+    const int lane = threadIdx.x & 63;
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t g0 = P.j0 + ((int64_t)blockIdx.x * (kBlock / kWave) + wid) * PW;
+    int pa[PW], pb[PW];
+    bool pv[PW];
+    int bmin = D.n_pad;
+#pragma unroll
+    for (int i = 0; i < PW; ++i) {
+        int tmp[2];
+        pv[i] = prefix_at<2>(P, D.n, g0 + i, tmp);
+        pa[i] = pv[i] ? tmp[0] : 0;
+        pb[i] = pv[i] ? tmp[1] : D.n_pad;
+        bmin = min(bmin, pb[i]);
+    }
+    if (!pv[0]) return;  // slots fill in order: slot 0 empty means all are
+    // an empty slot counts a copy of prefix 0 (discarded) so loads stay in range
+#pragma unroll
+    for (int i = 1; i < PW; ++i)
+        if (!pv[i]) pa[i] = pa[0], pb[i] = pb[0];
+    const float tau = st->tau;
+    const uint64_t tau_combo = st->tau_combo;
+    const int np = D.n_pad, n = D.n;
+    const uint32_t *pair = P.pair;
+    for (int t = (bmin + 1) >> 6; t < np >> 6; ++t) {
+        const int c = (t << 6) | lane;
+        const int tile_last = (t << 6) | 63;
+        uint32_t f[PW][2][8];
+        bool live[PW];
+#pragma unroll
+        for (int i = 0; i < PW; ++i) {
+            live[i] = pv[i] && tile_last > pb[i];
+#pragma unroll
+            for (int x = 0; x < 16; ++x) f[i][x >> 3][x & 7] = 0;
+        }
+#pragma unroll
+        for (int cls = 0; cls < 2; ++cls) {
+            const int nq = D.quads[cls];
+            if (nq <= 0) continue;
+            const uint4 *row = bits + D.class_off[cls];
+            const int voff = c * 16;
+            __amdgpu_buffer_rsrc_t r = quad_rsrc(row, np);
+            // Loads complete in order, so the issue order decides what each
+            // wait covers: the rows of prefix i+1 (two register sets used
+            // alternately) are issued before prefix i is counted, and the
+            // lanes' next quad before the last two prefixes of this one.
+            uint4 c0 = ld_lane(r, np, 0, voff), c1 = ld_lane(r, np, 1, voff);
+            uint4 X[4], Y[4];
+            X[0] = ld_uniform(r, np, 0, pa[0]);
+            X[1] = ld_uniform(r, np, 1, pa[0]);
+            X[2] = ld_uniform(r, np, 0, pb[0]);
+            X[3] = ld_uniform(r, np, 1, pb[0]);
+            for (int q = 0; q < nq; ++q) {
+                const uint4 *nrow = q + 1 < nq ? row + (size_t)3 * np : row;
+                const __amdgpu_buffer_rsrc_t nr = quad_rsrc(nrow, np);
+                uint4 n0, n1;
+#pragma unroll
+                for (int i = 0; i < PW; ++i) {
+                    if (i == PW - 2 || (PW == 1 && i == 0)) {
+                        n0 = ld_lane(nr, np, 0, voff);
+                        n1 = ld_lane(nr, np, 1, voff);
+                    }
+                    const __amdgpu_buffer_rsrc_t r2 = i + 1 < PW ? r : nr;
+                    const int ia = pa[(i + 1) % PW], ib = pb[(i + 1) % PW];
+                    uint4 *nxt = (i & 1) ? X : Y;
+                    nxt[0] = ld_uniform(r2, np, 0, ia);
+                    nxt[1] = ld_uniform(r2, np, 1, ia);
+                    nxt[2] = ld_uniform(r2, np, 0, ib);
+                    nxt[3] = ld_uniform(r2, np, 1, ib);
+                    __builtin_amdgcn_sched_barrier(0);
+                    const uint4 *cur = (i & 1) ? Y : X;
+                    if (live[i]) {  // uniform: the tile holds candidates for prefix i
+#pragma unroll
+                        for (int w = 0; w < 4; ++w) {
+                            const uint32_t x0 = comp(c0, w), x1 = comp(c1, w);
+                            const uint32_t u0 = comp(cur[0], w), u1 = comp(cur[1], w);
+                            const uint32_t v0 = comp(cur[2], w), v1 = comp(cur[3], w);
+                            and3_bcnt4(f[i][cls][0], f[i][cls][1], f[i][cls][2], f[i][cls][3], u0, v0, x0, u0, v0,
+                                       x1, u0, v1, x0, u0, v1, x1);
+                            and3_bcnt4(f[i][cls][4], f[i][cls][5], f[i][cls][6], f[i][cls][7], u1, v0, x0, u1, v0,
+                                       x1, u1, v1, x0, u1, v1, x1);
+                        }
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+                c0 = n0;
+                c1 = n1;
+                row = nrow;
+                r = nr;
+            }
+        }
+        const bool in_range = c < n;
+        uint32_t sc[2][2] = {{0, 0}, {0, 0}};
+        if (in_range) {
+            const uint32_t *sp = D.singles + (size_t)c * 6;
+            sc[0][0] = sp[0], sc[0][1] = sp[1], sc[1][0] = sp[3], sc[1][1] = sp[4];
+        }
+#pragma unroll
+        for (int i = 0; i < PW; ++i) {
+            if (!pv[i] || tile_last <= pb[i]) continue;  // uniform
+            const bool valid = in_range && c > pb[i];
+            uint32_t fac[8] = {0, 0, 0, 0, 0, 0, 0, 0}, fbc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            if (valid) {
+                const uint4 *pac = (const uint4 *)(pair + pair_index(pa[i], c, n) * 8);
+                const uint4 *pbc = (const uint4 *)(pair + pair_index(pb[i], c, n) * 8);
+                const uint4 u0 = pac[0], u1 = pac[1], v0 = pbc[0], v1 = pbc[1];
+                fac[0] = u0.x, fac[1] = u0.y, fac[2] = u0.z, fac[3] = u0.w;
+                fac[4] = u1.x, fac[5] = u1.y, fac[6] = u1.z, fac[7] = u1.w;
+                fbc[0] = v0.x, fbc[1] = v0.y, fbc[2] = v0.z, fbc[3] = v0.w;
+                fbc[4] = v1.x, fbc[5] = v1.y, fbc[6] = v1.z, fbc[7] = v1.w;
+            }
+            const uint32_t *pab = pair + pair_index(pa[i], pb[i], n) * 8;
+            const uint32_t *sa = D.singles + (size_t)pa[i] * 6;
+            const uint32_t *sb = D.singles + (size_t)pb[i] * 6;
+            uint32_t m0[27], m1[27];
+#pragma unroll
+            for (int cls = 0; cls < 2; ++cls) {
+                uint32_t *m = cls ? m1 : m0;
+                // m[x*9 + y*3 + z], x,y,z in {0, 1, 2 = any}
+#pragma unroll
+                for (int x = 0; x < 2; ++x)
+#pragma unroll
+                    for (int y = 0; y < 2; ++y) {
+                        m[x * 9 + y * 3 + 0] = f[i][cls][(x * 2 + y) * 2 + 0];
+                        m[x * 9 + y * 3 + 1] = f[i][cls][(x * 2 + y) * 2 + 1];
+                        m[x * 9 + y * 3 + 2] = pab[cls * 4 + x * 2 + y];
+                    }
+#pragma unroll
+                for (int x = 0; x < 2; ++x)
+#pragma unroll
+                    for (int z = 0; z < 2; ++z) m[x * 9 + 6 + z] = fac[cls * 4 + x * 2 + z];
+#pragma unroll
+                for (int y = 0; y < 2; ++y)
+#pragma unroll
+                    for (int z = 0; z < 2; ++z) m[18 + y * 3 + z] = fbc[cls * 4 + y * 2 + z];
+                m[0 * 9 + 8] = sa[cls * 3 + 0];
+                m[1 * 9 + 8] = sa[cls * 3 + 1];
+                m[18 + 0 * 3 + 2] = sb[cls * 3 + 0];
+                m[18 + 1 * 3 + 2] = sb[cls * 3 + 1];
+                m[18 + 6 + 0] = sc[cls][0];
+                m[18 + 6 + 1] = sc[cls][1];
+                m[26] = D.total[cls];
+                mobius3(m);
+            }
+            const uint64_t combo = ((((uint64_t)pa[i] << P.combo_bits) | (uint64_t)pb[i]) << P.combo_bits) | (uint64_t)c;
+            score_and_push<27>(valid, m0, m1, combo, tau, tau_combo, P, st);
+        }
     }
 }
 
@@ -465,6 +958,34 @@ hipError_t launch_scan_direct(int order, const uint4 *bits, const DevData &D, co
     case 4: hipLaunchKernelGGL(k_scan_direct<4>, grid, block, 0, s, bits, D, P, st); break;
     default: return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_scan_marginal(int order, const uint4 *bits, const DevData &D, const ScanParams &P, TopState *st,
+                                hipStream_t s)
+{
+    const int64_t prefixes = P.j1 - P.j0;
+    if (prefixes <= 0) return hipSuccess;
+    const int64_t per_block = (kBlock / kWave) * kMarginalPW;
+    const dim3 grid((unsigned)((prefixes + per_block - 1) / per_block)), block(kBlock);
+    switch (order) {
+    case 2:
+        hipLaunchKernelGGL((k_scan2m<kMarginalPW, false>), grid, block, 0, s, bits, D, P, st, (uint32_t *)nullptr);
+        break;
+    case 3: hipLaunchKernelGGL(k_scan3m<kMarginalPW>, grid, block, 0, s, bits, D, P, st); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_pair_table(const uint4 *bits, const DevData &D, const ScanParams &P, uint32_t *pair_out,
+                             hipStream_t s)
+{
+    const int64_t prefixes = P.j1 - P.j0;
+    if (prefixes <= 0) return hipSuccess;
+    const int64_t per_block = (kBlock / kWave) * kMarginalPW;
+    const dim3 grid((unsigned)((prefixes + per_block - 1) / per_block)), block(kBlock);
+    hipLaunchKernelGGL((k_scan2m<kMarginalPW, true>), grid, block, 0, s, bits, D, P, (TopState *)nullptr, pair_out);
     return hipGetLastError();
 }
 

@@ -210,11 +210,18 @@ def test_search_vs_oracle_fp64(snps, cases, ctrls, order, outputs, seed):
     # ranking by FP64 MI rounded to float, ties by combination
     print(f"case: synthetic {snps}x{cases}+{ctrls} k={order}", flush=True)
     d, od = synth(snps, cases, ctrls, seed)
-    for algorithm in (fa.ALGO_AUTO, fa.ALGO_DIRECT):
-        combos, mi = fa.GpuSearch(algorithm=algorithm).run_arrays(d, order, fa.Distribution(snps, order - 1), outputs)
-        oc, om, _ = O.search(od, order, outputs, threads=4, mi_mode=2)
-        assert combos.tolist() == oc.tolist()
-        assert np.array_equal(mi, om)
+    oc, om, _ = O.search(od, order, outputs, threads=4, mi_mode=2)
+    for algorithm in (fa.ALGO_AUTO, fa.ALGO_DIRECT, fa.ALGO_MARGINAL):
+        s = fa.GpuSearch(algorithm=algorithm)
+        combos, mi = s.run_arrays(d, order, fa.Distribution(snps, order - 1), outputs)
+        used = s.last_stats.algorithm
+        if algorithm == fa.ALGO_DIRECT or order > 3:
+            assert used == fa.ALGO_DIRECT
+        else:
+            assert used == fa.ALGO_MARGINAL
+        assert combos.tolist() == oc.tolist(), (algorithm, used)
+        assert np.array_equal(mi, om), (algorithm, used)
+        assert s.last_stats.combinations == math.comb(snps, order)
 
 
 def test_distribution_partition():
