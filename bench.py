@@ -116,9 +116,24 @@ def main():
     ap.add_argument("--steps", type=int, default=16)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--algorithm", default="auto", choices=["auto", "direct", "marginal"])
-    ap.add_argument("--cpu-sample-step", type=int, default=512)
+    ap.add_argument("--cpu-sample-step", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only run one device-to-device copy of BYTES (a plain copy kernel of known "
+                         "size, used to scale the FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        n = args.copy_probe // 4
+        src = torch.ones(n, dtype=torch.int32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        print(json.dumps({"copy_bytes": 4 * n, "seconds": dt, "GBps": 4 * n / dt / 1e9}))
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: every machine of this build has a single MI355X, "
               "so --gpus above 1 is not supported (see DESIGN.md for how the scan shards).", file=sys.stderr)

@@ -64,6 +64,20 @@ def synth_spec(snps, cases, ctrls, seed=1, maf=(0.05, 0.5), low_frac=0.0, low=(0
     return sp
 
 
+def read_files(inputs: Sequence[str]):
+    """Host-only ``Dataset<uint64_t>::read``: (words, snps, cases, ctrls,
+    cases_words, ctrls_words) in the reference host layout, no device."""
+    arr = (C.c_char_p * len(inputs))(*[p.encode() for p in inputs])
+    snps, cw, tw = C.c_size_t(), C.c_size_t(), C.c_size_t()
+    cases, ctrls = C.c_uint32(), C.c_uint32()
+    check(lib().fiu_read_files(arr, len(inputs), None, 0, C.byref(snps), C.byref(cases), C.byref(ctrls),
+                               C.byref(cw), C.byref(tw)))
+    words = np.zeros(max(snps.value * 3 * (cw.value + tw.value), 1), np.uint64)
+    check(lib().fiu_read_files(arr, len(inputs), words.ctypes.data, words.size, None, None, None, None, None))
+    n = snps.value * 3 * (cw.value + tw.value)
+    return words[:n], snps.value, cases.value, ctrls.value, cw.value, tw.value
+
+
 def synth_generate(spec, words=True, geno=False):
     """Host arrays (words in the reference layout, int8 genotypes)."""
     cw, tw = C.c_size_t(), C.c_size_t()
@@ -182,10 +196,11 @@ class Result:
 # ---------------------------------------------------------------------------
 # Search
 # ---------------------------------------------------------------------------
-def Config(algorithm=ALGO_AUTO, chunk_combos=0, candidate_cap=0, stream=None):
+def Config(algorithm=ALGO_AUTO, chunk_combos=0, candidate_cap=0, stream=None, sample_threshold=True):
     cfg = _lib.Config()
     lib().fiu_config_init(C.byref(cfg))
     cfg.algorithm, cfg.chunk_combos, cfg.candidate_cap = algorithm, chunk_combos, candidate_cap
+    cfg.flags = 0 if sample_threshold else _lib.FLAG_NO_THRESHOLD_SAMPLE
     cfg.stream = stream
     return cfg
 
@@ -193,8 +208,8 @@ def Config(algorithm=ALGO_AUTO, chunk_combos=0, candidate_cap=0, stream=None):
 class GpuSearch:
     """Search on one MI355X (drop-in for ThreadedSearch, ThreadedSearch.h:28)."""
 
-    def __init__(self, algorithm=ALGO_AUTO, chunk_combos=0, candidate_cap=0):
-        self.config = Config(algorithm, chunk_combos, candidate_cap)
+    def __init__(self, algorithm=ALGO_AUTO, chunk_combos=0, candidate_cap=0, sample_threshold=True):
+        self.config = Config(algorithm, chunk_combos, candidate_cap, sample_threshold=sample_threshold)
         self.last_stats = None
 
     def run_arrays(self, dataset: Dataset, order: int, distribution: Distribution, outputs: int):

@@ -27,7 +27,7 @@ ALGO_AUTO, ALGO_DIRECT, ALGO_MARGINAL = 0, 1, 2
 # Every symbol include/fiuncho_amd.h declares (checked by the CPU tests).
 EXPORTS = (
     "fiu_last_error", "fiu_version", "fiu_set_device",
-    "fiu_dataset_create", "fiu_dataset_read", "fiu_dataset_info", "fiu_dataset_words",
+    "fiu_dataset_create", "fiu_dataset_read", "fiu_read_files", "fiu_dataset_info", "fiu_dataset_words",
     "fiu_dataset_destroy", "fiu_config_init", "fiu_search", "fiu_ct_size",
     "fiu_combine_batch", "fiu_contingency_batch", "fiu_mi_batch",
     "fiu_synth_spec_init", "fiu_synth_generate", "fiu_dataset_synth",
@@ -43,7 +43,10 @@ class FiuError(RuntimeError):
 class Config(C.Structure):
     _fields_ = [("struct_size", C.c_uint32), ("algorithm", C.c_int32),
                 ("chunk_combos", C.c_uint64), ("candidate_cap", C.c_uint32),
-                ("reserved0", C.c_uint32), ("stream", C.c_void_p)]
+                ("flags", C.c_uint32), ("stream", C.c_void_p)]
+
+
+FLAG_NO_THRESHOLD_SAMPLE = 1
 
 
 class SearchStats(C.Structure):
@@ -79,6 +82,8 @@ def lib():
     L.fiu_set_device.argtypes = [C.c_int]
     L.fiu_dataset_create.argtypes = [vp, sz, sz, sz, u32, u32, C.POINTER(vp)]
     L.fiu_dataset_read.argtypes = [C.POINTER(C.c_char_p), sz, C.POINTER(vp)]
+    L.fiu_read_files.argtypes = [C.POINTER(C.c_char_p), sz, vp, sz, C.POINTER(sz), C.POINTER(u32),
+                                 C.POINTER(u32), C.POINTER(sz), C.POINTER(sz)]
     L.fiu_dataset_info.argtypes = [vp, C.POINTER(sz), C.POINTER(u32), C.POINTER(u32),
                                    C.POINTER(sz), C.POINTER(sz), C.POINTER(sz)]
     L.fiu_dataset_words.argtypes = [vp, vp]

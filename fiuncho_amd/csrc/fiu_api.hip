@@ -123,6 +123,7 @@ struct fiu_dataset {
     DBuf<TopState> state;
     DBuf<uint32_t> redo;
     DBuf<uint32_t> pair;        // marginal pair table (order-3 marginal scans)
+    DBuf<uint32_t> hist;        // screening-MI histogram of the threshold sampling pass
     bool pair_ready = false;
     std::vector<hipEvent_t> events;
 
@@ -336,23 +337,27 @@ void search(fiu_dataset *d, int order, int32_t step, int32_t offset, uint32_t ou
     d->cand.alloc(cap);
     d->top.alloc(kMaxOutputs);
     d->state.alloc(1);
-    // chunk plan: a small first chunk sets the threshold, then even chunks
-    const uint64_t per_max = (uint64_t)(n - M);  // combinations of the lightest-loaded prefix bound
+    // chunk plan: even chunks of about `target` combinations; before the
+    // first, a sampling pass over its first prefixes (about 2^24
+    // combinations) histograms the screening MI and sets a starting
+    // threshold, so that the first chunks push few candidates
     const double per_avg = (double)(n - M) / (double)(M + 1);
-    const uint64_t warm = std::min<uint64_t>(cap / 2, std::max<uint64_t>(16ull * outputs, 1ull << 16));
     const uint64_t target = cfg && cfg->chunk_combos ? cfg->chunk_combos : (1ull << 28);
     std::vector<Chunk> chunks;
     {
         int64_t j = 0;
-        const int64_t first = (int64_t)std::max<uint64_t>(1, warm / std::max<uint64_t>(per_max, 1));
-        const int64_t later = (int64_t)std::max<double>(1.0, (double)target / std::max(per_avg, 1.0));
+        const int64_t len = (int64_t)std::max<double>(1.0, (double)target / std::max(per_avg, 1.0));
         while ((uint64_t)j < n_pref) {
-            const int64_t len = chunks.empty() ? first : later;
             const int64_t e = std::min<int64_t>((int64_t)n_pref, j + len);
             chunks.push_back({j, e});
             j = e;
         }
     }
+    const bool sample = !(cfg && (cfg->flags & FIU_FLAG_NO_THRESHOLD_SAMPLE));
+    const int64_t sample_prefixes =
+        chunks.empty() || !sample ? 0
+                       : std::min<int64_t>(chunks[0].j1,
+                                           (int64_t)std::max<double>(64.0, (double)(1ull << 24) / std::max(per_avg, 1.0)));
     d->redo.alloc(std::max<size_t>(chunks.size(), 1));
     TopState init{};
     init.tau = -INFINITY;
@@ -390,6 +395,19 @@ void search(fiu_dataset *d, int order, int32_t step, int32_t offset, uint32_t ou
         check(launch_merge(d->state.p, d->top.p, d->cand.p, outputs, cap, (uint32_t)ci, d->redo.p, s), "merge launch");
         ++st.launches;
     };
+    if (sample_prefixes > 0) {
+        d->hist.alloc(kHistBins);
+        check(hipMemsetAsync(d->hist.p, 0, kHistBins * sizeof(uint32_t), s), "hist clear");
+        ScanParams H = P;
+        H.j0 = 0;
+        H.j1 = sample_prefixes;
+        H.hist = d->hist.p;
+        if (algo == FIU_ALGO_MARGINAL)
+            check(launch_scan_marginal(order, d->bits.p, d->dev, H, d->state.p, s), "sample launch");
+        else
+            check(launch_scan_direct(order, d->bits.p, d->dev, H, d->state.p, s), "sample launch");
+        check(launch_tau_from_hist(d->hist.p, outputs, P.delta, d->state.p, s), "threshold launch");
+    }
     for (size_t ci = 0; ci < chunks.size(); ++ci) run_chunk(ci, true);
     // chunks whose candidates overflowed the buffer are scanned again with
     // the (higher) final threshold until none overflows
@@ -478,6 +496,27 @@ fiu_status fiu_dataset_read(const char *const *paths, size_t npaths, fiu_dataset
         read_files(p, pk);
         *out = create(pk.words.data(), pk.snps, pk.cases_words, pk.ctrls_words, (uint32_t)pk.cases,
                       (uint32_t)pk.ctrls);
+    });
+}
+
+fiu_status fiu_read_files(const char *const *paths, size_t npaths, uint64_t *words, size_t capacity, size_t *snps,
+                          uint32_t *cases, uint32_t *ctrls, size_t *cases_words, size_t *ctrls_words)
+{
+    return guard([&] {
+        if (!paths && npaths) fail(FIU_ERR_INVALID, "paths is NULL");
+        std::vector<std::string> p;
+        for (size_t i = 0; i < npaths; ++i) p.emplace_back(paths[i] ? paths[i] : "");
+        PackedDataset pk;
+        read_files(p, pk);
+        if (snps) *snps = pk.snps;
+        if (cases) *cases = (uint32_t)pk.cases;
+        if (ctrls) *ctrls = (uint32_t)pk.ctrls;
+        if (cases_words) *cases_words = pk.cases_words;
+        if (ctrls_words) *ctrls_words = pk.ctrls_words;
+        if (words) {
+            if (capacity < pk.words.size()) fail(FIU_ERR_INVALID, "words buffer too small");
+            std::memcpy(words, pk.words.data(), pk.words.size() * 8);
+        }
     });
 }
 

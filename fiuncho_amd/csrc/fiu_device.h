@@ -70,7 +70,23 @@ struct ScanParams {
     uint32_t tag;
     double hy64, n64;        // H(Y) and cases+ctrls in FP64
     const uint32_t *pair;    // marginal pair table (order 3), or null
+    uint32_t *hist;          // threshold sampling pass: histogram of the
+                             // screening MI (kHistBins bins), else null
 };
+
+// Histogram of screening MI values by the top 12 bits of their float image
+// (sign, exponent, 3 mantissa bits): monotone for non-negative values, so the
+// N-th best bin bounds the N-th best value from below.
+constexpr int kHistBins = 4096;
+__host__ __device__ inline uint32_t hist_bin(float v)
+{
+    union {
+        float f;
+        uint32_t u;
+    } x;
+    x.f = v > 0.0f ? v : 0.0f;
+    return x.u >> 20;
+}
 
 __host__ __device__ inline uint64_t pair_index(int64_t b, int64_t c, int64_t n)
 {

@@ -9,6 +9,7 @@ hipError_t launch_scan_direct(int order, const uint4 *bits, const DevData &D, co
                               hipStream_t s);
 hipError_t launch_scan_marginal(int order, const uint4 *bits, const DevData &D, const ScanParams &P, TopState *st,
                                 hipStream_t s);
+hipError_t launch_tau_from_hist(const uint32_t *hist, uint32_t N, float delta, TopState *st, hipStream_t s);
 hipError_t launch_pair_table(const uint4 *bits, const DevData &D, const ScanParams &P, uint32_t *pair_out,
                              hipStream_t s);
 hipError_t launch_merge(TopState *st, Cand *top, const Cand *cand, uint32_t N, uint32_t cap, uint32_t chunk_id,

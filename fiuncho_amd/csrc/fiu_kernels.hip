@@ -246,6 +246,73 @@ __device__ __forceinline__ void score_and_push(bool valid, const uint32_t *a0, c
     push_candidates(want, mf, combo, P, st);
 }
 
+// Uniform lookup of a prefix: rank -> SNP indices, or an empty marker.
+template <int M>
+__device__ __forceinline__ bool prefix_at(const ScanParams &P, int n, int64_t j, int *pre)
+{
+    if (j >= P.j1) return false;
+    const uint64_t rank = (uint64_t)P.offset + (uint64_t)j * (uint64_t)P.step;
+    if (rank >= P.total_prefixes) return false;
+    unrank(rank, n, M, P.binom, pre);
+    return true;
+}
+
+// Either the normal epilogue, or (threshold sampling pass) one count in the
+// workgroup's LDS histogram of screening MI values.
+template <bool HIST, int CELLS>
+__device__ __forceinline__ void score_or_sample(bool valid, const uint32_t *a0, const uint32_t *a1, uint64_t combo,
+                                                float tau, uint64_t tau_combo, const ScanParams &P, TopState *st,
+                                                uint32_t *lds_hist)
+{
+    if constexpr (HIST) {
+        if (valid) atomicAdd(&lds_hist[hist_bin(mi_screen<CELLS>(a0, a1, P.hy, P.inv_n))], 1u);
+    } else {
+        score_and_push<CELLS>(valid, a0, a1, combo, tau, tau_combo, P, st);
+    }
+}
+
+template <bool HIST>
+__device__ __forceinline__ void hist_begin(uint32_t *lds_hist)
+{
+    if constexpr (HIST) {
+        for (int b = threadIdx.x; b < kHistBins; b += blockDim.x) lds_hist[b] = 0;
+        __syncthreads();
+    }
+}
+
+template <bool HIST>
+__device__ __forceinline__ void hist_end(uint32_t *lds_hist, uint32_t *hist)
+{
+    if constexpr (HIST) {
+        __syncthreads();
+        for (int b = threadIdx.x; b < kHistBins; b += blockDim.x)
+            if (lds_hist[b]) atomicAdd(&hist[b], lds_hist[b]);
+    }
+}
+
+// Starting threshold from the sampling histogram: the lower edge of the bin
+// holding the N-th best sampled screening value, less the screening margin.
+// The sampled combinations are scanned again afterwards, and at least N of
+// them have an exact value above it, so the final N-th best is never below.
+__global__ void k_tau_from_hist(const uint32_t *hist, uint32_t N, float delta, TopState *st)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    uint64_t cum = 0;
+    for (int b = kHistBins - 1; b >= 0; --b) {
+        cum += hist[b];
+        if (cum >= N && N > 0) {
+            union {
+                uint32_t u;
+                float f;
+            } x;
+            x.u = (uint32_t)b << 20;
+            st->tau = x.f - delta;
+            st->tau_combo = ~0ull;
+            return;
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Direct scan: all 3^k cells by AND + popcount (the reference's algorithm).
 // ---------------------------------------------------------------------------
@@ -356,19 +423,22 @@ __device__ __forceinline__ void accumulate_direct(const uint4 *__restrict__ base
     }
 }
 
-template <int K>
+template <int K, bool HIST>
 __global__ void __launch_bounds__(kBlock)
 k_scan_direct(const uint4 *__restrict__ bits, const DevData D, const ScanParams P, TopState *st)
 {
     constexpr int M = K - 1, CELLS = Pow3<K>::v;
+    __shared__ uint32_t lds_hist[HIST ? kHistBins : 1];
+    hist_begin<HIST>(lds_hist);
     const int lane = threadIdx.x & 63;
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int64_t j = P.j0 + (int64_t)blockIdx.x * (kBlock / kWave) + wid;
-    if (j >= P.j1) return;
-    const uint64_t rank = (uint64_t)P.offset + (uint64_t)j * (uint64_t)P.step;
-    if (rank >= P.total_prefixes) return;
     int pre[M];
-    unrank(rank, D.n, M, P.binom, pre);
+    const bool ok = prefix_at<M>(P, D.n, j, pre);
+    if (!ok) {
+        hist_end<HIST>(lds_hist, P.hist);  // every wave of the block reaches the barrier
+        return;
+    }
     uint64_t pre_packed = 0;
 #pragma unroll
     for (int i = 0; i < M; ++i) pre_packed = (pre_packed << P.combo_bits) | (uint64_t)pre[i];
@@ -378,13 +448,17 @@ k_scan_direct(const uint4 *__restrict__ bits, const DevData D, const ScanParams 
     const uint64_t tau_combo = st->tau_combo;
     const uint4 *b0 = bits + D.class_off[0];
     const uint4 *b1 = bits + D.class_off[1];
-    for (int t = (last + 1) >> 6; t < D.n_pad >> 6; ++t) {
+    // tiles are walked from the last one down, so that wavefronts resident at
+    // the same time read the same tile at about the same time (shared in L2)
+    for (int t = (D.n_pad >> 6) - 1; t >= (last + 1) >> 6; --t) {
         const int c = (t << 6) | lane;
         uint32_t a0[CELLS], a1[CELLS];
         accumulate_direct<K>(b0, D.n_pad, D.quads[0], pre, c * 16, a0);
         accumulate_direct<K>(b1, D.n_pad, D.quads[1], pre, c * 16, a1);
-        score_and_push<CELLS>(c > last && c < D.n, a0, a1, pre_packed | (uint64_t)c, tau, tau_combo, P, st);
+        score_or_sample<HIST, CELLS>(c > last && c < D.n, a0, a1, pre_packed | (uint64_t)c, tau, tau_combo, P, st,
+                                     lds_hist);
     }
+    hist_end<HIST>(lds_hist, P.hist);
 }
 
 // ---------------------------------------------------------------------------
@@ -421,23 +495,14 @@ __device__ __forceinline__ void mobius2(uint32_t *m)
     for (int k = 0; k < 3; ++k) m[6 + k] -= m[k] + m[3 + k];
 }
 
-// Uniform lookup of a prefix: rank -> SNP indices, or an empty marker.
-template <int M>
-__device__ __forceinline__ bool prefix_at(const ScanParams &P, int n, int64_t j, int *pre)
-{
-    if (j >= P.j1) return false;
-    const uint64_t rank = (uint64_t)P.offset + (uint64_t)j * (uint64_t)P.step;
-    if (rank >= P.total_prefixes) return false;
-    unrank(rank, n, M, P.binom, pre);
-    return true;
-}
-
 // Order 2.  TABLE = true writes the 8 pair counts (f(x,y), x,y in {0,1}, per
 // class) of every pair instead of scoring it.
-template <int PW, bool TABLE>
+template <int PW, bool TABLE, bool HIST>
 __global__ void __launch_bounds__(kBlock)
 k_scan2m(const uint4 *__restrict__ bits, const DevData D, const ScanParams P, TopState *st, uint32_t *pair_out)
 {
+    __shared__ uint32_t lds_hist[HIST ? kHistBins : 1];
+    hist_begin<HIST>(lds_hist);
     const int lane = threadIdx.x & 63;
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int64_t g0 = P.j0 + ((int64_t)blockIdx.x * (kBlock / kWave) + wid) * PW;
@@ -451,7 +516,10 @@ k_scan2m(const uint4 *__restrict__ bits, const DevData D, const ScanParams P, To
         pa[i] = pv[i] ? tmp[0] : D.n_pad;
         amin = min(amin, pa[i]);
     }
-    if (!pv[0]) return;  // slots fill in order: slot 0 empty means all are
+    if (!pv[0]) {  // slots fill in order: slot 0 empty means all are
+        hist_end<HIST>(lds_hist, P.hist);
+        return;
+    }
     // an empty slot counts a copy of prefix 0 (discarded) so loads stay in range
 #pragma unroll
     for (int i = 1; i < PW; ++i)
@@ -459,7 +527,7 @@ k_scan2m(const uint4 *__restrict__ bits, const DevData D, const ScanParams P, To
     const float tau = TABLE ? 0.0f : st->tau;
     const uint64_t tau_combo = TABLE ? 0 : st->tau_combo;
     const int np = D.n_pad;
-    for (int t = (amin + 1) >> 6; t < np >> 6; ++t) {
+    for (int t = (np >> 6) - 1; t >= (amin + 1) >> 6; --t) {  // last tile first (shared in L2)
         const int c = (t << 6) | lane;
         const int tile_last = (t << 6) | 63;
         uint32_t f[PW][2][4];
@@ -552,22 +620,25 @@ k_scan2m(const uint4 *__restrict__ bits, const DevData D, const ScanParams P, To
                 m[8] = D.total[cls];
                 mobius2(m);
             }
-            score_and_push<9>(valid, m0, m1, ((uint64_t)pa[i] << P.combo_bits) | (uint64_t)c, tau, tau_combo, P, st);
+            score_or_sample<HIST, 9>(valid, m0, m1, ((uint64_t)pa[i] << P.combo_bits) | (uint64_t)c, tau, tau_combo,
+                                     P, st, lds_hist);
         }
     }
+    hist_end<HIST>(lds_hist, P.hist);
 }
 
-// Order 3.  Per quad, the lanes' last-SNP rows for the next quad are in
-// flight while this quad is counted.  The prefix rows are pipelined one
-// prefix ahead: the pair rows P_xy = a_x & b_y of the prefix about to be
-// counted are formed on the scalar unit first (this waits for its loads),
-// then the loads of the next prefix are issued, then the 64 VALU
-// instructions run -- so every scalar load has one prefix of counting to
-// arrive (scalar loads complete out of order; a wait covers all of them).
-template <int PW>
+// Order 3.  A wavefront counts PW prefixes (a, b) against the 64 SNPs of a
+// tile.  Every cell word is one v_bitop3_b32 (a_x & b_y & c_z) and one
+// accumulating v_bcnt_u32_b32.  All rows arrive by buffer loads, which
+// complete in order: the rows of prefix i+1 are issued before prefix i is
+// counted, and the lanes' next quad before the last two prefixes, so every
+// wait covers data that had a whole prefix of counting to arrive.
+template <int PW, bool HIST>
 __global__ void __launch_bounds__(kBlock)
 k_scan3m(const uint4 *__restrict__ bits, const DevData D, const ScanParams P, TopState *st)
 {
+    __shared__ uint32_t lds_hist[HIST ? kHistBins : 1];
+    hist_begin<HIST>(lds_hist);
     const int lane = threadIdx.x & 63;
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int64_t g0 = P.j0 + ((int64_t)blockIdx.x * (kBlock / kWave) + wid) * PW;
@@ -582,7 +653,10 @@ k_scan3m(const uint4 *__restrict__ bits, const DevData D, const ScanParams P, To
         pb[i] = pv[i] ? tmp[1] : D.n_pad;
         bmin = min(bmin, pb[i]);
     }
-    if (!pv[0]) return;  // slots fill in order: slot 0 empty means all are
+    if (!pv[0]) {  // slots fill in order: slot 0 empty means all are
+        hist_end<HIST>(lds_hist, P.hist);
+        return;
+    }
     // an empty slot counts a copy of prefix 0 (discarded) so loads stay in range
 #pragma unroll
     for (int i = 1; i < PW; ++i)
@@ -591,7 +665,7 @@ k_scan3m(const uint4 *__restrict__ bits, const DevData D, const ScanParams P, To
     const uint64_t tau_combo = st->tau_combo;
     const int np = D.n_pad, n = D.n;
     const uint32_t *pair = P.pair;
-    for (int t = (bmin + 1) >> 6; t < np >> 6; ++t) {
+    for (int t = (np >> 6) - 1; t >= (bmin + 1) >> 6; --t) {  // last tile first (shared in L2)
         const int c = (t << 6) | lane;
         const int tile_last = (t << 6) | 63;
         uint32_t f[PW][2][8];
@@ -712,9 +786,10 @@ k_scan3m(const uint4 *__restrict__ bits, const DevData D, const ScanParams P, To
                 mobius3(m);
             }
             const uint64_t combo = ((((uint64_t)pa[i] << P.combo_bits) | (uint64_t)pb[i]) << P.combo_bits) | (uint64_t)c;
-            score_and_push<27>(valid, m0, m1, combo, tau, tau_combo, P, st);
+            score_or_sample<HIST, 27>(valid, m0, m1, combo, tau, tau_combo, P, st, lds_hist);
         }
     }
+    hist_end<HIST>(lds_hist, P.hist);
 }
 
 // ---------------------------------------------------------------------------
@@ -792,12 +867,11 @@ k_merge(TopState *st, Cand *top, const Cand *cand, uint32_t N, uint32_t cap, uin
     __syncthreads();
     if (tid == 0) {
         st->n_top = n_top;
+        // with fewer than N entries the threshold stays as it was (-inf, or
+        // the sampled starting threshold, which the final N-th best exceeds)
         if (n_top >= N && N > 0) {
             st->tau = top[N - 1].mi;
             st->tau_combo = top[N - 1].combo;
-        } else {
-            st->tau = -INFINITY;
-            st->tau_combo = ~0ull;
         }
         st->cand_count = 0;
         if (st->overflow) {
@@ -946,22 +1020,30 @@ __global__ void k_mi_batch(const uint32_t *tables, uint64_t n, uint64_t ct_size,
 // ---------------------------------------------------------------------------
 // Host launchers.
 // ---------------------------------------------------------------------------
-hipError_t launch_scan_direct(int order, const uint4 *bits, const DevData &D, const ScanParams &P, TopState *st,
+template <bool HIST>
+static hipError_t scan_direct(int order, const uint4 *bits, const DevData &D, const ScanParams &P, TopState *st,
                               hipStream_t s)
 {
     const int64_t waves = P.j1 - P.j0;
     if (waves <= 0) return hipSuccess;
     const dim3 grid((unsigned)((waves + 3) / 4)), block(kBlock);
     switch (order) {
-    case 2: hipLaunchKernelGGL(k_scan_direct<2>, grid, block, 0, s, bits, D, P, st); break;
-    case 3: hipLaunchKernelGGL(k_scan_direct<3>, grid, block, 0, s, bits, D, P, st); break;
-    case 4: hipLaunchKernelGGL(k_scan_direct<4>, grid, block, 0, s, bits, D, P, st); break;
+    case 2: hipLaunchKernelGGL((k_scan_direct<2, HIST>), grid, block, 0, s, bits, D, P, st); break;
+    case 3: hipLaunchKernelGGL((k_scan_direct<3, HIST>), grid, block, 0, s, bits, D, P, st); break;
+    case 4: hipLaunchKernelGGL((k_scan_direct<4, HIST>), grid, block, 0, s, bits, D, P, st); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
-hipError_t launch_scan_marginal(int order, const uint4 *bits, const DevData &D, const ScanParams &P, TopState *st,
+hipError_t launch_scan_direct(int order, const uint4 *bits, const DevData &D, const ScanParams &P, TopState *st,
+                              hipStream_t s)
+{
+    return P.hist ? scan_direct<true>(order, bits, D, P, st, s) : scan_direct<false>(order, bits, D, P, st, s);
+}
+
+template <bool HIST>
+static hipError_t scan_marginal(int order, const uint4 *bits, const DevData &D, const ScanParams &P, TopState *st,
                                 hipStream_t s)
 {
     const int64_t prefixes = P.j1 - P.j0;
@@ -970,11 +1052,24 @@ hipError_t launch_scan_marginal(int order, const uint4 *bits, const DevData &D, 
     const dim3 grid((unsigned)((prefixes + per_block - 1) / per_block)), block(kBlock);
     switch (order) {
     case 2:
-        hipLaunchKernelGGL((k_scan2m<kMarginalPW, false>), grid, block, 0, s, bits, D, P, st, (uint32_t *)nullptr);
+        hipLaunchKernelGGL((k_scan2m<kMarginalPW, false, HIST>), grid, block, 0, s, bits, D, P, st,
+                           (uint32_t *)nullptr);
         break;
-    case 3: hipLaunchKernelGGL(k_scan3m<kMarginalPW>, grid, block, 0, s, bits, D, P, st); break;
+    case 3: hipLaunchKernelGGL((k_scan3m<kMarginalPW, HIST>), grid, block, 0, s, bits, D, P, st); break;
     default: return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_scan_marginal(int order, const uint4 *bits, const DevData &D, const ScanParams &P, TopState *st,
+                                hipStream_t s)
+{
+    return P.hist ? scan_marginal<true>(order, bits, D, P, st, s) : scan_marginal<false>(order, bits, D, P, st, s);
+}
+
+hipError_t launch_tau_from_hist(const uint32_t *hist, uint32_t N, float delta, TopState *st, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_tau_from_hist, dim3(1), dim3(64), 0, s, hist, N, delta, st);
     return hipGetLastError();
 }
 
@@ -985,7 +1080,8 @@ hipError_t launch_pair_table(const uint4 *bits, const DevData &D, const ScanPara
     if (prefixes <= 0) return hipSuccess;
     const int64_t per_block = (kBlock / kWave) * kMarginalPW;
     const dim3 grid((unsigned)((prefixes + per_block - 1) / per_block)), block(kBlock);
-    hipLaunchKernelGGL((k_scan2m<kMarginalPW, true>), grid, block, 0, s, bits, D, P, (TopState *)nullptr, pair_out);
+    hipLaunchKernelGGL((k_scan2m<kMarginalPW, true, false>), grid, block, 0, s, bits, D, P, (TopState *)nullptr,
+                       pair_out);
     return hipGetLastError();
 }
 

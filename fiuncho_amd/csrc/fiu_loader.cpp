@@ -98,6 +98,28 @@ void pack_genotypes(const std::vector<int8_t> &geno, const std::vector<uint8_t> 
     }
 }
 
+// The reference's fill loops (RAWFile.hpp:226-259, TPEDFile.hpp:213-245) test
+// the "buffer full" condition of both classes after every sample, so once a
+// class has 64k samples every following sample of the other class writes the
+// cleared buffer into word k-1 again: that word (all three rows) is zero when
+// the sample right after the 64k-th one belongs to the other class.  Files
+// are read exactly as the reference reads them, so this is reproduced.
+static void file_flush_quirk(const std::vector<uint8_t> &is_ctrl, PackedDataset &d)
+{
+    size_t cnt[2] = {0, 0};
+    const size_t per = 3 * (d.cases_words + d.ctrls_words);
+    for (size_t j = 0; j < is_ctrl.size(); ++j) {
+        const int c = is_ctrl[j] != 0;
+        ++cnt[c];
+        if (cnt[c] % 64 == 0 && j + 1 < is_ctrl.size() && (is_ctrl[j + 1] != 0) != c) {
+            const size_t W = c ? d.ctrls_words : d.cases_words;
+            const size_t off = c ? 3 * d.cases_words : 0, w = cnt[c] / 64 - 1;
+            for (size_t s = 0; s < d.snps; ++s)
+                for (int k = 0; k < 3; ++k) d.words[s * per + off + k * W + w] = 0;
+        }
+    }
+}
+
 void read_raw(const std::string &path, PackedDataset &out)
 {
     const std::string text = read_all(path);
@@ -184,6 +206,7 @@ void read_raw(const std::string &path, PackedDataset &out)
     for (size_t j = 0; j < samples; ++j)
         for (size_t v = 0; v < nvar; ++v) geno[v * samples + j] = by_sample[j * nvar + v];
     pack_genotypes(geno, is_ctrl, nvar, samples, out);
+    file_flush_quirk(is_ctrl, out);
 }
 
 void read_tped(const std::string &tped, const std::string &tfam, PackedDataset &out)
@@ -245,6 +268,7 @@ void read_tped(const std::string &tped, const std::string &tfam, PackedDataset &
         ++snps;
     }
     pack_genotypes(geno, is_ctrl, snps, samples, out);
+    file_flush_quirk(is_ctrl, out);
 }
 
 void read_files(const std::vector<std::string> &paths, PackedDataset &out)

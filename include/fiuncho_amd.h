@@ -70,6 +70,16 @@ fiu_status fiu_dataset_create(const uint64_t *words, size_t snps,
 fiu_status fiu_dataset_read(const char *const *paths, size_t npaths,
                             fiu_dataset **out);
 
+/* Host-only read of the same files into the reference host layout, without a
+ * device: call with words = NULL to learn the shape, then with a buffer of
+ * at least snps*3*(cases_words+ctrls_words) words (`capacity`).
+ * Replaces: the tables Dataset<uint64_t>::read builds (Dataset.h:48-94,
+ * RAWFile.hpp:292-313, TPEDFile.hpp:271-296). */
+fiu_status fiu_read_files(const char *const *paths, size_t npaths,
+                          uint64_t *words, size_t capacity, size_t *snps,
+                          uint32_t *cases, uint32_t *ctrls,
+                          size_t *cases_words, size_t *ctrls_words);
+
 /* Dataset shape (any out pointer may be NULL). */
 fiu_status fiu_dataset_info(const fiu_dataset *ds, size_t *snps,
                             uint32_t *cases, uint32_t *ctrls,
@@ -93,12 +103,16 @@ typedef enum {
                               same samples (checked at upload) */
 } fiu_algorithm;
 
+/* fiu_config.flags */
+#define FIU_FLAG_NO_THRESHOLD_SAMPLE 1u  /* skip the pass that seeds the top-N
+                                            threshold from a sample */
+
 typedef struct {
     uint32_t struct_size;    /* sizeof(fiu_config) */
     int32_t algorithm;       /* fiu_algorithm */
     uint64_t chunk_combos;   /* combinations per scan launch, 0 = auto */
     uint32_t candidate_cap;  /* device candidate buffer entries, 0 = auto */
-    uint32_t reserved0;
+    uint32_t flags;          /* FIU_FLAG_* */
     void *stream;            /* hipStream_t to run on, NULL = library stream */
 } fiu_config;
 

@@ -79,6 +79,29 @@ static void pack_class_rows(const int8_t *g, const uint8_t *is_ctrl,
         for (int k = 0; k < 3; ++k) rows[k * words + cnt / 64] = buf[k];
 }
 
+/* The fill loops of RAWFile.hpp:226-259 / TPEDFile.hpp:213-245 test the
+ * "buffer full" condition of BOTH classes after every sample, so after the
+ * 64k-th sample of a class every following sample of the other class flushes
+ * the (already cleared) buffer again into word k-1: that word ends up zero
+ * when the sample right after the 64k-th one belongs to the other class.
+ * Applied to datasets read from files, as the reference reads them. */
+static void file_flush_quirk(const uint8_t *is_ctrl, size_t samples,
+                             orc_dataset *d)
+{
+    size_t cnt[2] = {0, 0};
+    const size_t per = 3 * (d->cases_words + d->ctrls_words);
+    for (size_t j = 0; j < samples; ++j) {
+        const int c = is_ctrl[j] != 0;
+        ++cnt[c];
+        if (cnt[c] % 64 == 0 && j + 1 < samples && (is_ctrl[j + 1] != 0) != c) {
+            const size_t W = c ? d->ctrls_words : d->cases_words;
+            const size_t off = c ? 3 * d->cases_words : 0, w = cnt[c] / 64 - 1;
+            for (size_t s = 0; s < d->snps; ++s)
+                for (int k = 0; k < 3; ++k) d->words[s * per + off + k * W + w] = 0;
+        }
+    }
+}
+
 int orc_pack(const int8_t *geno, const uint8_t *is_ctrl, size_t snps,
              size_t samples, int align, orc_dataset *out)
 {
@@ -277,6 +300,7 @@ int orc_read_raw(const char *path, int align, orc_dataset *out, char *err,
     free(ph);
     free(g);
     int rc = orc_pack(gm, ctrl, nvar, ns, align, out);
+    if (rc == 0) file_flush_quirk(ctrl, ns, out);
     free(gm);
     free(ctrl);
     return rc;
@@ -370,6 +394,7 @@ int orc_read_tped(const char *tped, const char *tfam, int align,
     free(buf);
     free(al);
     int rc = orc_pack(gm, ctrl, nv, ns, align, out);
+    if (rc == 0) file_flush_quirk(ctrl, ns, out);
     free(gm);
     free(ctrl);
     return rc;

@@ -11,6 +11,9 @@ outputs only:
   testdata_search.json  ThreadedSearch(1).run top-90 at orders 2, 3, 4, and the
                         SHA-256 of the bit tables the reference loader builds
                         from the .tped/.tfam, .raw and GAMETES .raw files
+  loader_quirk.npz      a small .raw and .tped/.tfam (their text is stored) whose
+                        sample order triggers the fill loop's re-flush of a full
+                        word, and the bit tables the reference loader builds
   synth_search.json     ThreadedSearch(8).run on small synthetic datasets made
                         by fiuncho_amd's seeded generator (written as PLINK
                         .raw files so that the reference's own loader reads
@@ -99,6 +102,36 @@ def main():
         search["top"][str(k)] = {"combos": combos.tolist(), "mi": [float(x) for x in mi]}
     with open(os.path.join(HERE, "testdata_search.json"), "w") as f:
         json.dump(search, f, indent=1)
+
+    # loader fixture whose sample order makes the reference's fill loop
+    # re-flush a full word (64 cases then a control, 64 controls then a case)
+    rng = np.random.default_rng(77)
+    order = [0] * 64 + [1] + [0] * 67 + [1] * 63 + [0] + [1] * 70 + [0] * 60
+    geno = rng.integers(0, 3, size=(5, len(order))).astype(np.int8)
+    quirk = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        raw = os.path.join(tmp, "q.raw")
+        with open(raw, "w") as f:
+            f.write("FID IID PAT MAT SEX PHENOTYPE " + " ".join(f"Q{i}_T" for i in range(5)) + "\n")
+            for j, cl in enumerate(order):
+                f.write(f"f{j} i{j} 0 0 1 {1 if cl else 2} " + " ".join(str(int(v)) for v in geno[:, j]) + "\n")
+        tped, tfam = os.path.join(tmp, "q.tped"), os.path.join(tmp, "q.tfam")
+        with open(tfam, "w") as f:
+            for j, cl in enumerate(order):
+                f.write(f"f{j} i{j} 0 0 1 {1 if cl else 2}\n")
+        pairs = {0: "G G", 1: "A G", 2: "A A"}
+        with open(tped, "w") as f:
+            for s in range(5):
+                f.write(f"1 q{s} 0 {100 + s} " + " ".join(pairs[int(v)] for v in geno[s]) + "\n")
+        for name, files in (("raw", (raw,)), ("tped", (tped, tfam))):
+            d = ref.load(*files)
+            quirk[f"{name}_words"] = d.words()
+            quirk[f"{name}_shape"] = np.array([d.snps, d.cases, d.ctrls, d.cases_words, d.ctrls_words])
+            with open(files[0]) as fh:
+                quirk[f"{name}_text"] = np.frombuffer(fh.read().encode(), np.uint8)
+        with open(tfam) as fh:
+            quirk["tfam_text"] = np.frombuffer(fh.read().encode(), np.uint8)
+    np.savez_compressed(os.path.join(HERE, "loader_quirk.npz"), **quirk)
 
     synth = {}
     with tempfile.TemporaryDirectory() as tmp:

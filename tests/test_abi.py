@@ -56,3 +56,46 @@ def test_synth_hwe_frequencies():
     assert (p > 0.03).all() and (p < 0.53).all()
     het = (g == 1).mean(axis=1)
     assert np.abs(het - 2 * p * (1 - p)).max() < 0.04
+
+
+def test_product_loader_matches_reference_tables(tmp_path):
+    import hashlib
+    import json
+    from .conftest import DATA, GOLDEN
+    with open(os.path.join(GOLDEN, "testdata_search.json")) as f:
+        sha = json.load(f)["sha256"]
+    for files in ([os.path.join(DATA, "test.tped"), os.path.join(DATA, "test.tfam")],
+                  [os.path.join(DATA, "test.tfam"), os.path.join(DATA, "test.tped")],
+                  [os.path.join(DATA, "test.raw")], [os.path.join(DATA, "test.gametes.raw")]):
+        words, snps, cases, ctrls, cw, tw = fa.read_files(files)
+        assert (snps, cases, ctrls, cw, tw) == (10, 9000, 9800, 144, 160)
+        assert hashlib.sha256(words.tobytes()).hexdigest() == sha["tped"]
+    q = np.load(os.path.join(GOLDEN, "loader_quirk.npz"))
+    for key, name in (("raw_text", "q.raw"), ("tped_text", "q.tped"), ("tfam_text", "q.tfam")):
+        (tmp_path / name).write_bytes(q[key].tobytes())
+    assert np.array_equal(fa.read_files([str(tmp_path / "q.raw")])[0], q["raw_words"])
+    assert np.array_equal(fa.read_files([str(tmp_path / "q.tped"), str(tmp_path / "q.tfam")])[0], q["tped_words"])
+
+
+def test_product_loader_errors(tmp_path):
+    import pytest
+    from .conftest import DATA
+    bad = tmp_path / "x.txt"
+    bad.write_text("1 2 3\n")
+    for files in ([str(bad)], [os.path.join(DATA, "test.tped")], [os.path.join(DATA, "test.tped")] * 2,
+                  [str(tmp_path / "missing.raw")], []):
+        with pytest.raises(fa.FiuError):
+            fa.read_files(files)
+    tfam = tmp_path / "b.tfam"
+    tfam.write_text("f1 i1 0 0 1 3\n")          # phenotype must be 1 or 2
+    tped = tmp_path / "b.tped"
+    tped.write_text("1 s 0 1 A A\n")
+    with pytest.raises(fa.FiuError):
+        fa.read_files([str(tped), str(tfam)])
+    tfam.write_text("f1 i1 0 0 1 2\n")
+    tped.write_text("1 s 0 1 A X\n")            # invalid nucleotide
+    with pytest.raises(fa.FiuError):
+        fa.read_files([str(tped), str(tfam)])
+    tped.write_text("1 s 0 1 A A C C\n")        # 2 samples' alleles for 1 sample
+    with pytest.raises(fa.FiuError):
+        fa.read_files([str(tped), str(tfam)])

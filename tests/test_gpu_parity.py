@@ -296,7 +296,30 @@ def test_small_chunks_and_candidate_overflow():
     print("case: overflow / redo", flush=True)
     d, od = synth(90, 256, 256, 71)
     ref_c, ref_m = fa.GpuSearch().run_arrays(d, 2, fa.Distribution(90, 1), 64)
-    s = fa.GpuSearch(chunk_combos=300, candidate_cap=128)
+    oc, om, _ = O.search(od, 2, 64, threads=2, mi_mode=2)
+    assert ref_c.tolist() == oc.tolist() and np.array_equal(ref_m, om)
+    # without the sampled starting threshold the first chunks overflow
+    s = fa.GpuSearch(chunk_combos=300, candidate_cap=128, sample_threshold=False)
     c, m = s.run_arrays(d, 2, fa.Distribution(90, 1), 64)
     assert c.tolist() == ref_c.tolist() and np.array_equal(m, ref_m)
     assert s.last_stats.redo_chunks > 0
+    # with it, the same tiny buffer suffices
+    s = fa.GpuSearch(chunk_combos=300, candidate_cap=128)
+    c, m = s.run_arrays(d, 2, fa.Distribution(90, 1), 64)
+    assert c.tolist() == ref_c.tolist() and np.array_equal(m, ref_m)
+
+
+@pytest.mark.parametrize("order", [2, 3])
+def test_sampled_threshold_with_late_signal(order):
+    # the strongest combinations sit in the last prefixes (far from the
+    # sampled ones): the starting threshold must not cut any of them
+    print(f"case: late signal k={order}", flush=True)
+    n = 40
+    spec = fa.synth_spec(n, 300, 300, seed=81, planted=tuple(range(n - order, n)))
+    words, geno, cw, tw = fa.synth_generate(spec, words=True, geno=True)
+    d = fa.Dataset.from_words(words, n, cw, tw, 300, 300)
+    od = O.pack(geno, np.r_[np.zeros(300, np.uint8), np.ones(300, np.uint8)])
+    oc, om, _ = O.search(od, order, 30, threads=2, mi_mode=2)
+    for chunk in (0, 50):
+        c, m = fa.GpuSearch(chunk_combos=chunk).run_arrays(d, order, fa.Distribution(n, order - 1), 30)
+        assert c.tolist() == oc.tolist() and np.array_equal(m, om)

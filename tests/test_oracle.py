@@ -273,3 +273,30 @@ def test_search_synthetic_matches_reference(golden_dir):
         combos, mi, _ = O.search(d, g["order"], g["outputs"], threads=4, mi_mode=0)
         assert {tuple(c) for c in combos} == {tuple(c) for c in g["combos"]}, name
         assert sorted(mi.tolist()) == sorted(np.float32(g["mi"]).tolist()), name
+
+
+def _write_quirk_files(tmp):
+    q = np.load(os.path.join(GOLDEN, "loader_quirk.npz"))
+    paths = {}
+    for key, name in (("raw_text", "q.raw"), ("tped_text", "q.tped"), ("tfam_text", "q.tfam")):
+        paths[name] = os.path.join(tmp, name)
+        with open(paths[name], "wb") as f:
+            f.write(q[key].tobytes())
+    return q, paths
+
+
+def test_loader_flush_quirk_matches_reference(tmp_path):
+    # RAWFile.hpp:245-258 / TPEDFile.hpp:229-244: a full word flushed at the
+    # 64k-th sample of a class is re-flushed as zero by the next sample of the
+    # other class; the reference build's tables for such a file are stored
+    q, p = _write_quirk_files(str(tmp_path))
+    r = O.read_raw(p["q.raw"])
+    t = O.read_tped(p["q.tped"], p["q.tfam"])
+    assert [r.snps, r.cases, r.ctrls, r.cases_words, r.ctrls_words] == q["raw_shape"].tolist()
+    assert np.array_equal(r.words, q["raw_words"])
+    assert np.array_equal(t.words, q["tped_words"])
+    # and the effect is real: the re-flushed words are zero in every row
+    cw = r.cases_words
+    per = 3 * (cw + r.ctrls_words)
+    for s in range(r.snps):
+        assert all(int(r.words[s * per + k * cw + 0]) == 0 for k in range(3))
