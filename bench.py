@@ -37,6 +37,11 @@ N_SNPS, N_CASES, N_CTRLS = 10_000, 8_000, 8_000
 PLANTED = (17, 4242, 9001)
 ORDER, OUTPUTS, SLICES = 3, 100, 64
 VALU_PEAK = 256 * 64 * 2.4e9          # int32 VALU lane-ops per second (MI355X)
+# Bytes fetched beyond L2 per triple by the order-3 marginal scan kernel:
+# rocprofv3 FETCH_SIZE per dispatch x 2.0 (the factor a plain 1 GiB copy
+# shows) + WRITE_SIZE x 1.0, divided by the triples of those dispatches
+# (profiles/k_scan3m_order3_10000x16000.md).
+TRAFFIC_BYTES_PER_TRIPLE = {"marginal": 1563.0}
 
 
 def alg_ops_per_combo(order, cases, ctrls):
@@ -84,7 +89,8 @@ def cpu_baseline(words, geno, cw, tw, threads, sample_step):
         kind = "port"
     return {"value": combos * (N_CASES + N_CTRLS) / sec, "unit": "combo*samples/s", "cores": threads,
             "kind": kind,
-            "sample": f"Distribution({N_SNPS}, {ORDER - 1}, {sample_step}, 0) of the same dataset: "
+            "sample": f"Distribution({N_SNPS}, {ORDER - 1}, {sample_step}, 0) of the same dataset "
+                      f"(read from a PLINK .raw by the reference loader): "
                       f"{combos} of {math.comb(N_SNPS, ORDER)} triples, {sec:.1f} s, "
                       f"ThreadedSearch({threads}) default AVX-512 build" if kind == "reference" else
                       f"Distribution({N_SNPS}, {ORDER - 1}, {sample_step}, 0): {combos} triples, {sec:.1f} s, "
@@ -135,8 +141,9 @@ def main():
         print(json.dumps({"copy_bytes": 4 * n, "seconds": dt, "GBps": 4 * n / dt / 1e9}))
         return
     if args.gpus != 1:
-        print("bench.py measures one GPU: every machine of this build has a single MI355X, "
-              "so --gpus above 1 is not supported (see DESIGN.md for how the scan shards).", file=sys.stderr)
+        print("bench.py measures one GPU (a single MI355X); --gpus above 1 is not supported. "
+              "The scan shards by Distribution layers, one search per GPU, with a host merge "
+              "of the per-GPU top-N lists.", file=sys.stderr)
         sys.exit(2)
     if args.steps < 1 or args.steps + args.warmup > SLICES:
         sys.exit(f"--steps + --warmup must be within 1..{SLICES}")
@@ -176,6 +183,13 @@ def main():
     value = combos * unit_samples / elapsed
     ops = combos * alg_ops_per_combo(ORDER, N_CASES, N_CTRLS)
     achieved = ops / (scan_ms / 1e3)
+    algo_name = {1: "direct", 2: "marginal"}.get(used_algo, str(used_algo))
+    # operations the kernel executes per triple: AND + popcount of the 8
+    # (marginal) or 27 (direct) cells over every stored 32-bit word
+    words32 = 2 * (cw + tw)
+    executed = combos * 2 * (8 if algo_name == "marginal" else 27) * words32
+    per_triple = TRAFFIC_BYTES_PER_TRIPLE.get(algo_name)
+    traffic = per_triple * combos / max(launches, 1) if per_triple else None
     line = {
         "metric": "genotype-combos*samples/s at order 3",
         "value": value,
@@ -192,10 +206,12 @@ def main():
         "config": {"workload": "order-3 exhaustive scan, 10,000 SNPs x 16,000 samples (8,000/8,000), top-100",
                    "slices": f"{args.steps} of {SLICES} Distribution(10000, 2, {SLICES}, i) slices",
                    "combinations": combos,
-                   "algorithm": {1: "direct", 2: "marginal"}.get(used_algo, str(used_algo))},
+                   "algorithm": algo_name},
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": VALU_PEAK / 1e12, "unit": "TOP/s",
-                     "frac": achieved / VALU_PEAK, "traffic": None,
-                     "kernel": "k_scan (scan launches only, HIP events)",
+                     "frac": achieved / VALU_PEAK, "traffic": traffic,
+                     "kernel": "k_scan3m" if algo_name == "marginal" else "k_scan_direct<3>",
+                     "ops_per_combo_sample": alg_ops_per_combo(ORDER, N_CASES, N_CTRLS) / unit_samples,
+                     "executed_frac": executed / (scan_ms / 1e3) / VALU_PEAK,
                      "launches": launches, "avg_launch_ms": scan_ms / max(launches, 1)},
         "top1": {"combination": list(top[0][0]), "mi": top[0][1]} if top else None,
     }
