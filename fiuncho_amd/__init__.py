@@ -32,6 +32,7 @@ from ._lib import ALGO_AUTO, ALGO_DIRECT, ALGO_MARGINAL, FiuError, check, lib
 
 __all__ = ["Dataset", "Distribution", "Result", "GpuSearch", "Config", "combine_batch",
            "contingency_batch", "mi_batch", "ct_size", "synth_spec", "synth_generate",
+           "search_batch", "batch_config", "Server",
            "FiuError", "ALGO_AUTO", "ALGO_DIRECT", "ALGO_MARGINAL", "version"]
 
 
@@ -230,6 +231,117 @@ class GpuSearch:
         """``Search::run``: results sorted by MI descending (ties by combination)."""
         combos, mi = self.run_arrays(dataset, order, distribution, outputs)
         return [Result([int(x) for x in c], float(v)) for c, v in zip(combos, mi)]
+
+
+# ---------------------------------------------------------------------------
+# Batched searches and the scan server
+# ---------------------------------------------------------------------------
+def batch_config(max_active=0, memory_budget=0, step_ms=0.0, candidate_cap=0):
+    cfg = _lib.BatchConfig()
+    lib().fiu_batch_config_init(C.byref(cfg))
+    if max_active:
+        cfg.max_active = max_active
+    if memory_budget:
+        cfg.memory_budget = memory_budget
+    if step_ms:
+        cfg.step_ms = step_ms
+    if candidate_cap:
+        cfg.candidate_cap = candidate_cap
+    return cfg
+
+
+def search_batch(scans, config=None):
+    """Run many searches together (``fiu_search_batch``).
+
+    ``scans``: sequence of (dataset, order, distribution, outputs).  Returns
+    (list of (combos, mi) per search, batch stats)."""
+    n = len(scans)
+    arr = (_lib.Scan * max(n, 1))()
+    bufs = []
+    for i, (ds, order, dist, outputs) in enumerate(scans):
+        if dist.k != order - 1:
+            raise ValueError("distribution.k must be order - 1")
+        combos = np.zeros(max(outputs, 1) * order, np.int32)
+        mi = np.zeros(max(outputs, 1), np.float32)
+        bufs.append((combos, mi, order))
+        s = arr[i]
+        s.dataset, s.order, s.dist_step, s.dist_offset, s.outputs = (
+            ds.handle.value, order, dist.step, dist.offset, outputs)
+        s.out_combos, s.out_mi = combos.ctypes.data, mi.ctypes.data
+    stats = _lib.BatchStats()
+    cfg = config if config is not None else batch_config()
+    check(lib().fiu_search_batch(arr, n, C.byref(cfg), C.byref(stats)))
+    out = []
+    for i, (combos, mi, order) in enumerate(bufs):
+        c = arr[i].out_count
+        out.append((combos[:c * order].reshape(c, order), mi[:c]))
+    return out, stats
+
+
+class Server:
+    """Scan server (``fiu_server_*``): continuous batching of scan requests."""
+
+    def __init__(self, config=None):
+        cfg = config if config is not None else batch_config()
+        h = C.c_void_p()
+        check(lib().fiu_server_create(C.byref(cfg), C.byref(h)))
+        self._h = h
+        self._pending = {}
+
+    def submit(self, order, outputs, dataset=None, words=None, snps=0, cases_words=0, ctrls_words=0,
+               cases=0, ctrls=0, distribution=None):
+        """Submit a request on a resident ``dataset`` or on host ``words``
+        (reference layout; copied before this returns).  Returns its id."""
+        r = _lib.Request()
+        r.struct_size = C.sizeof(_lib.Request)
+        r.order, r.outputs = order, outputs
+        dist = distribution or Distribution(0, order - 1)
+        r.dist_step, r.dist_offset = dist.step, dist.offset
+        if dataset is not None:
+            r.dataset = dataset.handle.value
+        else:
+            w = np.ascontiguousarray(words, dtype=np.uint64)
+            if w.size != snps * 3 * (cases_words + ctrls_words):
+                raise ValueError("words has the wrong length for the given shape")
+            r.words = w.ctypes.data
+            r.snps, r.cases_words, r.ctrls_words, r.cases, r.ctrls = snps, cases_words, ctrls_words, cases, ctrls
+        rid = C.c_uint64()
+        check(lib().fiu_server_submit(self._h, C.byref(r), C.byref(rid)))
+        self._pending[rid.value] = (order, outputs)
+        return rid.value
+
+    def wait(self, rid, timeout_ms=-1.0):
+        """(combos, mi, stats) of request ``rid``; None when still running
+        after ``timeout_ms`` (>= 0)."""
+        order, outputs = self._pending[rid]
+        combos = np.zeros(max(outputs, 1) * order, np.int32)
+        mi = np.zeros(max(outputs, 1), np.float32)
+        cnt = C.c_uint32()
+        st = _lib.RequestStats()
+        s = lib().fiu_server_wait(self._h, rid, timeout_ms, combos.ctypes.data, mi.ctypes.data, C.byref(cnt),
+                                  C.byref(st))
+        if s == _lib.FIU_ERR_PENDING:
+            return None
+        del self._pending[rid]
+        check(s)
+        n = cnt.value
+        return combos[:n * order].reshape(n, order), mi[:n], st
+
+    def info(self):
+        i = _lib.ServerInfo()
+        check(lib().fiu_server_query(self._h, C.byref(i)))
+        return i
+
+    def close(self):
+        if self._h and self._h.value:
+            check(lib().fiu_server_destroy(self._h))
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # ---------------------------------------------------------------------------

@@ -30,6 +30,8 @@ EXPORTS = (
     "fiu_dataset_create", "fiu_dataset_read", "fiu_read_files", "fiu_dataset_info", "fiu_dataset_words",
     "fiu_dataset_destroy", "fiu_config_init", "fiu_search", "fiu_ct_size",
     "fiu_combine_batch", "fiu_contingency_batch", "fiu_mi_batch",
+    "fiu_batch_config_init", "fiu_search_batch", "fiu_server_create", "fiu_server_submit",
+    "fiu_server_wait", "fiu_server_query", "fiu_server_destroy",
     "fiu_synth_spec_init", "fiu_synth_generate", "fiu_dataset_synth",
 )
 
@@ -53,6 +55,44 @@ class SearchStats(C.Structure):
     _fields_ = [("combinations", C.c_uint64), ("launches", C.c_uint64),
                 ("redo_chunks", C.c_uint64), ("scan_ms", C.c_double),
                 ("total_ms", C.c_double), ("algorithm", C.c_int32), ("reserved", C.c_int32)]
+
+
+class BatchConfig(C.Structure):
+    _fields_ = [("struct_size", C.c_uint32), ("max_active", C.c_uint32), ("memory_budget", C.c_uint64),
+                ("step_ms", C.c_double), ("candidate_cap", C.c_uint32), ("flags", C.c_uint32)]
+
+
+class Scan(C.Structure):
+    _fields_ = [("dataset", C.c_void_p), ("order", C.c_uint16), ("reserved", C.c_uint16),
+                ("dist_step", C.c_int32), ("dist_offset", C.c_int32), ("outputs", C.c_uint32),
+                ("out_combos", C.c_void_p), ("out_mi", C.c_void_p), ("out_count", C.c_uint32),
+                ("status", C.c_int32)]
+
+
+class BatchStats(C.Structure):
+    _fields_ = [("steps", C.c_uint64), ("units", C.c_uint64), ("redo_ranges", C.c_uint64),
+                ("device_ms", C.c_double), ("total_ms", C.c_double)]
+
+
+class Request(C.Structure):
+    _fields_ = [("struct_size", C.c_uint32), ("order", C.c_uint16), ("reserved", C.c_uint16),
+                ("dataset", C.c_void_p), ("words", C.c_void_p),
+                ("snps", C.c_uint64), ("cases_words", C.c_uint64), ("ctrls_words", C.c_uint64),
+                ("cases", C.c_uint32), ("ctrls", C.c_uint32),
+                ("dist_step", C.c_int32), ("dist_offset", C.c_int32),
+                ("outputs", C.c_uint32), ("reserved2", C.c_uint32)]
+
+
+class RequestStats(C.Structure):
+    _fields_ = [("submit_ms", C.c_double), ("admit_ms", C.c_double), ("start_ms", C.c_double),
+                ("done_ms", C.c_double), ("latency_ms", C.c_double), ("combinations", C.c_uint64),
+                ("steps", C.c_uint64), ("algorithm", C.c_int32), ("reserved", C.c_int32)]
+
+
+class ServerInfo(C.Structure):
+    _fields_ = [("queued", C.c_uint64), ("active", C.c_uint64), ("completed", C.c_uint64),
+                ("memory_budget", C.c_uint64), ("memory_used", C.c_uint64), ("steps", C.c_uint64),
+                ("device_ms", C.c_double)]
 
 
 class SynthSpec(C.Structure):
@@ -97,6 +137,14 @@ def lib():
     L.fiu_combine_batch.argtypes = [vp, vp, u16, sz, vp]
     L.fiu_contingency_batch.argtypes = [vp, vp, u16, sz, vp, vp]
     L.fiu_mi_batch.argtypes = [vp, sz, u16, u32, u32, vp]
+    L.fiu_batch_config_init.argtypes = [C.POINTER(BatchConfig)]
+    L.fiu_batch_config_init.restype = None
+    L.fiu_search_batch.argtypes = [C.POINTER(Scan), sz, C.POINTER(BatchConfig), C.POINTER(BatchStats)]
+    L.fiu_server_create.argtypes = [C.POINTER(BatchConfig), C.POINTER(vp)]
+    L.fiu_server_submit.argtypes = [vp, C.POINTER(Request), C.POINTER(C.c_uint64)]
+    L.fiu_server_wait.argtypes = [vp, C.c_uint64, C.c_double, vp, vp, C.POINTER(u32), C.POINTER(RequestStats)]
+    L.fiu_server_query.argtypes = [vp, C.POINTER(ServerInfo)]
+    L.fiu_server_destroy.argtypes = [vp]
     L.fiu_synth_spec_init.argtypes = [C.POINTER(SynthSpec)]
     L.fiu_synth_spec_init.restype = None
     L.fiu_synth_generate.argtypes = [C.POINTER(SynthSpec), vp, vp, C.POINTER(sz), C.POINTER(sz)]

@@ -11,79 +11,29 @@
 #include <functional>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
+#include "fiu_internal.h"
 #include "fiu_kernels.h"
 #include "fiu_loader.h"
 #include "fiu_synth.h"
-#include <thread>
 
 using namespace fiu;
 
-namespace {
+namespace fiu {
 
-thread_local std::string g_error;
-thread_local int g_device = 0;
-
-struct Failure {
-    fiu_status status;
-    std::string msg;
-};
-
-[[noreturn]] void fail(fiu_status s, const std::string &m) { throw Failure{s, m}; }
-
-void check(hipError_t e, const char *what)
+std::string &error_slot()
 {
-    if (e != hipSuccess) fail(FIU_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+    thread_local std::string s;
+    return s;
 }
 
-template <class F> fiu_status guard(F &&f)
+int &current_device()
 {
-    try {
-        f();
-        g_error.clear();
-        return FIU_OK;
-    } catch (const Failure &e) {
-        g_error = e.msg;
-        return e.status;
-    } catch (const LoadError &e) {
-        g_error = e.what();
-        return FIU_ERR_IO;
-    } catch (const std::bad_alloc &) {
-        g_error = "host allocation failed";
-        return FIU_ERR_NOMEM;
-    } catch (const std::exception &e) {
-        g_error = e.what();
-        return FIU_ERR_INVALID;
-    }
+    thread_local int d = 0;
+    return d;
 }
-
-// Device buffer with RAII.
-template <class T> struct DBuf {
-    T *p = nullptr;
-    size_t n = 0;
-    DBuf() = default;
-    DBuf(const DBuf &) = delete;
-    DBuf &operator=(const DBuf &) = delete;
-    ~DBuf() { release(); }
-    void release()
-    {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-    }
-    void alloc(size_t count)
-    {
-        if (count <= n && p) return;
-        release();
-        if (hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)) != hipSuccess) {
-            p = nullptr;
-            (void)hipGetLastError();
-            fail(FIU_ERR_NOMEM, "device allocation of " + std::to_string(count * sizeof(T)) + " bytes failed");
-        }
-        n = count;
-    }
-};
 
 uint64_t binom_sat(uint64_t n, uint64_t k)
 {
@@ -104,64 +54,69 @@ int bit_width(uint64_t x)
     return std::max(b, 1);
 }
 
-}  // namespace
+}  // namespace fiu
 
-struct fiu_dataset {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    size_t snps = 0, cw = 0, tw = 0;
-    uint32_t cases = 0, ctrls = 0;
-    std::vector<uint64_t> host_words;
-    DBuf<uint4> bits;
-    DBuf<uint32_t> singles;
-    DBuf<uint32_t> flags;
-    DevData dev{};
-    // search scratch (grown on demand, kept for later calls)
-    DBuf<uint64_t> binom;
-    int binom_m = -1;
-    DBuf<Cand> cand, top;
-    DBuf<TopState> state;
-    DBuf<uint32_t> redo;
-    DBuf<uint32_t> pair;        // marginal pair table (order-3 marginal scans)
-    DBuf<uint32_t> hist;        // screening-MI histogram of the threshold sampling pass
-    bool pair_ready = false;
-    std::vector<hipEvent_t> events;
+fiu_dataset::~fiu_dataset()
+{
+    for (auto e : events) (void)hipEventDestroy(e);
+    if (stream && own_stream) (void)hipStreamDestroy(stream);
+}
 
-    ~fiu_dataset()
-    {
-        for (auto e : events) (void)hipEventDestroy(e);
-        if (stream) (void)hipStreamDestroy(stream);
-    }
-};
+namespace fiu {
 
 namespace {
 
-void upload(fiu_dataset *d, const uint64_t *words)
+struct Shape {
+    int n, n_pad, q0, q1;
+    uint64_t off1, total;   // uint4 units
+};
+
+Shape shape_of(size_t snps, size_t cw, size_t tw)
+{
+    Shape s;
+    s.n = (int)snps;
+    s.n_pad = (s.n + 63) / 64 * 64;
+    s.q0 = (int)((cw + 1) / 2);
+    s.q1 = (int)((tw + 1) / 2);
+    s.off1 = (uint64_t)s.q0 * 3 * s.n_pad;
+    s.total = s.off1 + (uint64_t)s.q1 * 3 * s.n_pad;
+    return s;
+}
+
+void upload(fiu_dataset *d, const uint64_t *words, hipStream_t stream)
 {
     if (d->snps > (size_t)INT32_MAX - 64) fail(FIU_ERR_UNSUPPORTED, "more SNPs than INT_MAX");
     check(hipSetDevice(d->device), "hipSetDevice");
-    check(hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking), "hipStreamCreate");
-    const int n = (int)d->snps, n_pad = (n + 63) / 64 * 64;
-    const int q0 = (int)((d->cw + 1) / 2), q1 = (int)((d->tw + 1) / 2);
-    const uint64_t off1 = (uint64_t)q0 * 3 * n_pad;
-    const uint64_t total = off1 + (uint64_t)q1 * 3 * n_pad;
-    d->bits.alloc(total);
+    if (stream) {
+        d->stream = stream;
+        d->own_stream = false;
+    } else {
+        check(hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking), "hipStreamCreate");
+        d->own_stream = true;
+    }
+    const Shape sh = shape_of(d->snps, d->cw, d->tw);
+    const int n = sh.n, n_pad = sh.n_pad;
+    d->bits.alloc(sh.total);
     const size_t nwords = d->snps * 3 * (d->cw + d->tw);
     {
         DBuf<uint64_t> staging;
+        staging.arena = d->arena;
         staging.alloc(nwords);
         if (nwords)
             check(hipMemcpyAsync(staging.p, words, nwords * 8, hipMemcpyHostToDevice, d->stream), "upload");
-        check(launch_transpose(staging.p, d->bits.p, n, n_pad, 0, d->cw, d->tw, q0, 0, d->stream), "transpose");
-        check(launch_transpose(staging.p, d->bits.p, n, n_pad, 1, d->cw, d->tw, q1, off1, d->stream), "transpose");
+        check(launch_transpose(staging.p, d->bits.p, n, n_pad, 0, d->cw, d->tw, sh.q0, 0, d->stream), "transpose");
+        check(launch_transpose(staging.p, d->bits.p, n, n_pad, 1, d->cw, d->tw, sh.q1, sh.off1, d->stream),
+              "transpose");
+        // the staging buffer is released (and may be reused) once the
+        // transposes have read it
         check(hipStreamSynchronize(d->stream), "transpose sync");
     }
     DevData &D = d->dev;
     D.bits = d->bits.p;
     D.class_off[0] = 0;
-    D.class_off[1] = off1;
-    D.quads[0] = q0;
-    D.quads[1] = q1;
+    D.class_off[1] = sh.off1;
+    D.quads[0] = sh.q0;
+    D.quads[1] = sh.q1;
     D.n = n;
     D.n_pad = n_pad;
     D.cases = d->cases;
@@ -180,27 +135,42 @@ void upload(fiu_dataset *d, const uint64_t *words)
     D.total[1] = fl[2];
 }
 
-fiu_dataset *create(const uint64_t *words, size_t snps, size_t cw, size_t tw, uint32_t cases, uint32_t ctrls)
+}  // namespace
+
+size_t dataset_bytes(size_t snps, size_t cw, size_t tw)
+{
+    const Shape sh = shape_of(snps, cw, tw);
+    auto up = [](size_t b) { return (b + Arena::kAlign - 1) / Arena::kAlign * Arena::kAlign; };
+    return up(std::max<uint64_t>(sh.total, 1) * 16) + up((size_t)sh.n_pad * 24) + up(16);
+}
+
+size_t staging_bytes(size_t snps, size_t cw, size_t tw)
+{
+    return (std::max<size_t>(snps * 3 * (cw + tw), 1) * 8 + Arena::kAlign - 1) / Arena::kAlign * Arena::kAlign;
+}
+
+fiu_dataset *create_dataset(const uint64_t *words, size_t snps, size_t cw, size_t tw, uint32_t cases,
+                            uint32_t ctrls, Arena *arena, hipStream_t stream, bool keep_host_words)
 {
     if (!words && snps) fail(FIU_ERR_INVALID, "words is NULL");
     if (snps && (cw == 0 || tw == 0) && (cases || ctrls)) fail(FIU_ERR_INVALID, "zero words per row");
     if ((uint64_t)cases > cw * 64 || (uint64_t)ctrls > tw * 64)
         fail(FIU_ERR_INVALID, "more samples than bits per row");
     auto d = std::make_unique<fiu_dataset>();
-    d->device = g_device;
+    d->device = current_device();
+    d->use_arena(arena);
     d->snps = snps;
     d->cw = cw;
     d->tw = tw;
     d->cases = cases;
     d->ctrls = ctrls;
-    d->host_words.assign(words, words + snps * 3 * (cw + tw));
-    upload(d.get(), d->host_words.data());
+    if (keep_host_words) {
+        d->host_words.assign(words, words + snps * 3 * (cw + tw));
+        upload(d.get(), d->host_words.data(), stream);
+    } else {
+        upload(d.get(), words, stream);
+    }
     return d.release();
-}
-
-hipStream_t stream_of(const fiu_dataset *d, const fiu_config *cfg)
-{
-    return cfg && cfg->stream ? (hipStream_t)cfg->stream : d->stream;
 }
 
 void mi_constants(uint32_t cases, uint32_t ctrls, ScanParams &P)
@@ -219,6 +189,7 @@ void mi_constants(uint32_t cases, uint32_t ctrls, ScanParams &P)
 
 void ensure_binom(fiu_dataset *d, int m, hipStream_t s)
 {
+    std::lock_guard<std::mutex> lk(d->prep_mu);
     if (d->binom_m >= m) return;
     const uint64_t n = d->snps, stride = n + 1;
     std::vector<uint64_t> h((uint64_t)(m + 1) * stride);
@@ -230,21 +201,19 @@ void ensure_binom(fiu_dataset *d, int m, hipStream_t s)
     d->binom_m = m;
 }
 
-struct Chunk {
-    int64_t j0, j1;
-};
-
-// Pair counts f(x, y), x, y in {0, 1}, of every SNP pair, both classes
-// (32 bytes per pair).  Returns false when the table does not fit in the
-// free device memory.
 bool ensure_pair_table(fiu_dataset *d, hipStream_t s)
 {
+    ensure_binom(d, 1, s);
+    std::lock_guard<std::mutex> lk(d->prep_mu);
     if (d->pair_ready) return true;
     const uint64_t n = d->snps, pairs = n * (n - 1) / 2;
-    size_t free_b = 0, total_b = 0;
-    check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
-    if (pairs * 32 + (64ull << 20) > free_b) return false;
-    ensure_binom(d, 1, s);
+    if (d->arena) {
+        if (pairs * 32 > d->arena->largest_free()) return false;
+    } else {
+        size_t free_b = 0, total_b = 0;
+        check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+        if (pairs * 32 + (64ull << 20) > free_b) return false;
+    }
     d->pair.alloc(pairs * 8);
     ScanParams P{};
     P.j0 = 0;
@@ -253,16 +222,14 @@ bool ensure_pair_table(fiu_dataset *d, hipStream_t s)
     P.offset = 0;
     P.total_prefixes = n;
     P.binom = d->binom.p;
-    check(launch_pair_table(d->bits.p, d->dev, P, d->pair.p, s), "pair table launch");
+    check(launch_pair_table(d->dev, P, d->pair.p, s), "pair table launch");
     check(hipStreamSynchronize(s), "pair table sync");
     d->pair_ready = true;
     return true;
 }
 
-// Exact number of combinations whose (m)-prefix has lexicographic rank
-// = offset (mod step): sum over prefixes of n - 1 - last.  The prefixes that
-// share their first m-1 indices form a run of consecutive ranks whose last
-// index increases by one; each run is summed in closed form.
+// The prefixes that share their first m-1 indices form a run of consecutive
+// ranks whose last index increases by one; each run is summed in closed form.
 uint64_t count_combinations(uint64_t n, int m, uint64_t step, uint64_t offset)
 {
     uint64_t total = 0, rank = 0;
@@ -292,6 +259,19 @@ uint64_t count_combinations(uint64_t n, int m, uint64_t step, uint64_t offset)
     if (m >= 1) rec(0, 0);
     return total;
 }
+
+}  // namespace fiu
+
+namespace {
+
+hipStream_t stream_of(const fiu_dataset *d, const fiu_config *cfg)
+{
+    return cfg && cfg->stream ? (hipStream_t)cfg->stream : d->stream;
+}
+
+struct Chunk {
+    int64_t j0, j1;
+};
 
 void search(fiu_dataset *d, int order, int32_t step, int32_t offset, uint32_t outputs, const fiu_config *cfg,
             int32_t *out_combos, float *out_mi, uint32_t *out_count, fiu_search_stats *stats)
@@ -387,9 +367,9 @@ void search(fiu_dataset *d, int order, int32_t step, int32_t offset, uint32_t ou
         P.j1 = chunks[ci].j1;
         if (timed) check(hipEventRecord(d->events[ev], s), "event");
         if (algo == FIU_ALGO_MARGINAL)
-            check(launch_scan_marginal(order, d->bits.p, d->dev, P, d->state.p, s), "scan launch");
+            check(launch_scan_marginal(order, d->dev, P, d->state.p, s), "scan launch");
         else
-            check(launch_scan_direct(order, d->bits.p, d->dev, P, d->state.p, s), "scan launch");
+            check(launch_scan_direct(order, d->dev, P, d->state.p, s), "scan launch");
         if (timed) check(hipEventRecord(d->events[ev + 1], s), "event");
         if (timed) ev += 2;
         check(launch_merge(d->state.p, d->top.p, d->cand.p, outputs, cap, (uint32_t)ci, d->redo.p, s), "merge launch");
@@ -403,9 +383,9 @@ void search(fiu_dataset *d, int order, int32_t step, int32_t offset, uint32_t ou
         H.j1 = sample_prefixes;
         H.hist = d->hist.p;
         if (algo == FIU_ALGO_MARGINAL)
-            check(launch_scan_marginal(order, d->bits.p, d->dev, H, d->state.p, s), "sample launch");
+            check(launch_scan_marginal(order, d->dev, H, d->state.p, s), "sample launch");
         else
-            check(launch_scan_direct(order, d->bits.p, d->dev, H, d->state.p, s), "sample launch");
+            check(launch_scan_direct(order, d->dev, H, d->state.p, s), "sample launch");
         check(launch_tau_from_hist(d->hist.p, outputs, P.delta, d->state.p, s), "threshold launch");
     }
     for (size_t ci = 0; ci < chunks.size(); ++ci) run_chunk(ci, true);
@@ -435,12 +415,7 @@ void search(fiu_dataset *d, int order, int32_t step, int32_t offset, uint32_t ou
     check(hipMemcpy(&hs, d->state.p, sizeof(hs), hipMemcpyDeviceToHost), "state read");
     std::vector<Cand> top(hs.n_top);
     if (hs.n_top) check(hipMemcpy(top.data(), d->top.p, hs.n_top * sizeof(Cand), hipMemcpyDeviceToHost), "top read");
-    const uint64_t mask = bits == 64 ? ~0ull : ((1ull << bits) - 1);
-    for (uint32_t i = 0; i < hs.n_top; ++i) {
-        for (int q = 0; q < order; ++q)
-            out_combos[(size_t)i * order + q] = (int32_t)((top[i].combo >> (bits * (order - 1 - q))) & mask);
-        out_mi[i] = top[i].mi;
-    }
+    unpack_results(top.data(), hs.n_top, order, bits, out_combos, out_mi);
     *out_count = hs.n_top;
     // combinations evaluated: closed form for the whole space, else a walk
     // over the runs of prefixes (O(C(n, order-2)))
@@ -453,11 +428,25 @@ void search(fiu_dataset *d, int order, int32_t step, int32_t offset, uint32_t ou
 
 }  // namespace
 
+namespace fiu {
+
+void unpack_results(const Cand *top, uint32_t count, int order, int bits, int32_t *out_combos, float *out_mi)
+{
+    const uint64_t mask = bits == 64 ? ~0ull : ((1ull << bits) - 1);
+    for (uint32_t i = 0; i < count; ++i) {
+        for (int q = 0; q < order; ++q)
+            out_combos[(size_t)i * order + q] = (int32_t)((top[i].combo >> (bits * (order - 1 - q))) & mask);
+        out_mi[i] = top[i].mi;
+    }
+}
+
+}  // namespace fiu
+
 extern "C" {
 
-const char *fiu_last_error(void) { return g_error.c_str(); }
+const char *fiu_last_error(void) { return error_slot().c_str(); }
 
-const char *fiu_version(void) { return "fiuncho_amd 0.1 (gfx950)"; }
+const char *fiu_version(void) { return "fiuncho_amd 0.2 (gfx950)"; }
 
 fiu_status fiu_set_device(int device)
 {
@@ -465,7 +454,7 @@ fiu_status fiu_set_device(int device)
         int count = 0;
         check(hipGetDeviceCount(&count), "hipGetDeviceCount");
         if (device < 0 || device >= count) fail(FIU_ERR_INVALID, "no such device");
-        g_device = device;
+        current_device() = device;
     });
 }
 
@@ -482,7 +471,7 @@ fiu_status fiu_dataset_create(const uint64_t *words, size_t snps, size_t cases_w
 {
     return guard([&] {
         if (!out) fail(FIU_ERR_INVALID, "out is NULL");
-        *out = create(words, snps, cases_words, ctrls_words, cases, ctrls);
+        *out = create_dataset(words, snps, cases_words, ctrls_words, cases, ctrls);
     });
 }
 
@@ -494,8 +483,8 @@ fiu_status fiu_dataset_read(const char *const *paths, size_t npaths, fiu_dataset
         for (size_t i = 0; i < npaths; ++i) p.emplace_back(paths[i] ? paths[i] : "");
         PackedDataset pk;
         read_files(p, pk);
-        *out = create(pk.words.data(), pk.snps, pk.cases_words, pk.ctrls_words, (uint32_t)pk.cases,
-                      (uint32_t)pk.ctrls);
+        *out = create_dataset(pk.words.data(), pk.snps, pk.cases_words, pk.ctrls_words, (uint32_t)pk.cases,
+                              (uint32_t)pk.ctrls);
     });
 }
 
@@ -599,7 +588,7 @@ fiu_status fiu_combine_batch(const fiu_dataset *ds, const int32_t *combos, uint1
         dc.alloc(n * order);
         dout.alloc(n * per);
         check(hipMemcpyAsync(dc.p, combos, n * order * 4, hipMemcpyHostToDevice, ds->stream), "copy");
-        check(launch_combine_batch(ds->bits.p, ds->dev, dc.p, order, n, ds->cw, ds->tw, dout.p, ds->stream), "combine");
+        check(launch_combine_batch(ds->dev, dc.p, order, n, ds->cw, ds->tw, dout.p, ds->stream), "combine");
         check(hipMemcpyAsync(out, dout.p, n * per * 8, hipMemcpyDeviceToHost, ds->stream), "copy back");
         check(hipStreamSynchronize(ds->stream), "sync");
     });
@@ -621,7 +610,7 @@ fiu_status fiu_contingency_batch(const fiu_dataset *ds, const int32_t *combos, u
         dc.alloc(n * order);
         dout.alloc(n * 2 * cts);
         check(hipMemcpyAsync(dc.p, combos, n * order * 4, hipMemcpyHostToDevice, ds->stream), "copy");
-        check(launch_contingency_batch(ds->bits.p, ds->dev, dc.p, order, n, cts, dout.p, ds->stream), "contingency");
+        check(launch_contingency_batch(ds->dev, dc.p, order, n, cts, dout.p, ds->stream), "contingency");
         if (out_mi) {
             ScanParams P{};
             mi_constants(ds->cases, ds->ctrls, P);
@@ -641,7 +630,7 @@ fiu_status fiu_mi_batch(const uint32_t *tables, size_t n, uint16_t order, uint32
         if ((!tables || !out_mi) && n) fail(FIU_ERR_INVALID, "NULL argument");
         if (order < 1 || order > 12) fail(FIU_ERR_UNSUPPORTED, "order outside 1..12");
         if (!n) return;
-        check(hipSetDevice(g_device), "hipSetDevice");
+        check(hipSetDevice(current_device()), "hipSetDevice");
         const size_t cts = fiu_ct_size(order);
         hipStream_t s;
         check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "stream");
@@ -725,7 +714,7 @@ fiu_status fiu_dataset_synth(const fiu_synth_spec *spec, fiu_dataset **out)
         const size_t cw = synth_words(s.cases), tw = synth_words(s.ctrls);
         std::vector<uint64_t> words(s.snps * 3 * (cw + tw));
         synth_generate(s, words.data(), nullptr, synth_threads(spec));
-        *out = create(words.data(), s.snps, cw, tw, s.cases, s.ctrls);
+        *out = create_dataset(words.data(), s.snps, cw, tw, s.cases, s.ctrls);
     });
 }
 

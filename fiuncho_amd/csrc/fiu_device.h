@@ -12,7 +12,6 @@ constexpr int kBlock = 256;          // threads per scan workgroup (4 waves)
 constexpr int kMaxOrder = 4;         // orders with a scan kernel
 constexpr int kMaxPrefix = 7;        // longest prefix the unranker handles
 constexpr int kMarginalPW = 4;       // prefixes per wavefront in marginal scans
-constexpr int kWindowQuads = 8;      // quads per LDS-staged window of a tile
 constexpr int kMergeSlots = 8192;    // LDS entries of the top-N merge
 constexpr int kMaxOutputs = kMergeSlots / 2;
 
@@ -53,7 +52,7 @@ struct TopState {
     uint32_t cand_count;     // candidates pushed since the last merge
     uint32_t overflow;       // a push did not fit in the buffer
     uint32_t redo_count;     // chunks that must be scanned again
-    uint32_t pad_;
+    uint32_t step_overflow;  // batched scans: the last merge saw an overflow
 };
 
 struct ScanParams {
@@ -87,6 +86,51 @@ __host__ __device__ inline uint32_t hist_bin(float v)
     x.f = v > 0.0f ? v : 0.0f;
     return x.u >> 20;
 }
+
+// ---------------------------------------------------------------------------
+// Batched scans: one persistent launch works through the units of many scans
+// (different datasets, sizes and orders).  A unit is one wavefront's work: a
+// group of prefixes (kMarginalPW for marginal kinds, 1 for direct kinds)
+// against a chunk of `tiles` consecutive tiles of 64 last-SNP candidates.
+// ---------------------------------------------------------------------------
+enum BatchKind : int32_t {
+    kKindScan2m = 0,   // order 2, marginal
+    kKindScan3m = 1,   // order 3, marginal (pair table in P.pair)
+    kKindDirect2 = 2,  // order 2, all cells
+    kKindDirect3 = 3,  // order 3, all cells
+    kKindDirect4 = 4,  // order 4, all cells (its own launch: register budget)
+};
+
+__host__ __device__ inline int batch_group(int32_t kind) { return kind <= kKindScan3m ? kMarginalPW : 1; }
+
+struct BatchJob {
+    DevData D;
+    ScanParams P;            // distribution, MI constants, candidate buffer
+    TopState *st;
+    Cand *top;
+    uint32_t outputs;
+    int32_t kind;            // BatchKind
+    int32_t chunks;          // tile chunks per prefix group
+    int32_t tiles;           // tiles per chunk
+};
+
+// A contiguous prefix range [j0, j1) of one job in one launch; its units are
+// numbered from `start` (prefix group major, tile chunk minor).
+struct BatchSeg {
+    uint64_t start;
+    int64_t j0, j1;
+    int32_t job;
+    int32_t pad_;
+};
+
+// Per-launch header in device memory (the launch itself is a fixed graph).
+struct BatchHeader {
+    uint32_t nseg[2];        // segments of the main launch, of the order-4 launch
+    uint32_t units[2];       // units of each launch
+    uint32_t nmerge;         // jobs to merge after the launches
+    uint32_t head[2];        // work counters (cleared by the graph)
+    uint32_t pad_;
+};
 
 __host__ __device__ inline uint64_t pair_index(int64_t b, int64_t c, int64_t n)
 {

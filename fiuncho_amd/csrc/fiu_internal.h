@@ -1,0 +1,191 @@
+// Host-side internals shared by the C ABI (fiu_api.hip) and the batched scan
+// engine / scan server (fiu_engine.hip): error plumbing, device buffers, the
+// dataset handle and the per-dataset preparation steps.
+#pragma once
+
+#include "../../include/fiuncho_amd.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "fiu_kernels.h"
+#include "fiu_loader.h"
+
+namespace fiu {
+
+struct Failure {
+    fiu_status status;
+    std::string msg;
+};
+
+[[noreturn]] inline void fail(fiu_status s, const std::string &m) { throw Failure{s, m}; }
+
+inline void check(hipError_t e, const char *what)
+{
+    if (e != hipSuccess) fail(FIU_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// Thread-local message slot behind fiu_last_error(), and the device selected
+// by fiu_set_device() on this thread.
+std::string &error_slot();
+int &current_device();
+
+template <class F> fiu_status guard(F &&f)
+{
+    try {
+        f();
+        error_slot().clear();
+        return FIU_OK;
+    } catch (const Failure &e) {
+        error_slot() = e.msg;
+        return e.status;
+    } catch (const LoadError &e) {
+        error_slot() = e.what();
+        return FIU_ERR_IO;
+    } catch (const std::bad_alloc &) {
+        error_slot() = "host allocation failed";
+        return FIU_ERR_NOMEM;
+    } catch (const std::exception &e) {
+        error_slot() = e.what();
+        return FIU_ERR_INVALID;
+    }
+}
+
+// First-fit sub-allocator over one device allocation: the scan server takes
+// its whole memory budget once and carves tables and buffers out of it, so
+// that admitting and retiring scans never calls hipMalloc / hipFree (which
+// can wait for the device).  Thread-safe.
+class Arena {
+  public:
+    static constexpr size_t kAlign = 256;
+    Arena(size_t bytes, int device);
+    ~Arena();
+    Arena(const Arena &) = delete;
+    Arena &operator=(const Arena &) = delete;
+    void *alloc(size_t bytes);   // nullptr when no free range fits
+    void release(void *p);
+    size_t capacity() const { return size_; }
+    size_t used() const;
+    size_t largest_free() const;
+
+  private:
+    char *base_ = nullptr;
+    size_t size_ = 0;
+    size_t used_ = 0;
+    std::map<size_t, size_t> free_;   // offset -> length
+    std::map<size_t, size_t> taken_;  // offset -> length
+    mutable std::mutex mu_;
+};
+
+// Device buffer with RAII, from hipMalloc or from an Arena.
+template <class T> struct DBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    Arena *arena = nullptr;
+    DBuf() = default;
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    ~DBuf() { release(); }
+    void release()
+    {
+        if (p) {
+            if (arena) arena->release(p);
+            else (void)hipFree(p);
+        }
+        p = nullptr;
+        n = 0;
+    }
+    void alloc(size_t count)
+    {
+        if (count <= n && p) return;
+        release();
+        const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+        if (arena) {
+            p = (T *)arena->alloc(bytes);
+            if (!p) fail(FIU_ERR_NOMEM, "scan server memory budget exhausted (" + std::to_string(bytes) + " bytes)");
+        } else if (hipMalloc(&p, bytes) != hipSuccess) {
+            p = nullptr;
+            (void)hipGetLastError();
+            fail(FIU_ERR_NOMEM, "device allocation of " + std::to_string(bytes) + " bytes failed");
+        }
+        n = count;
+    }
+};
+
+uint64_t binom_sat(uint64_t n, uint64_t k);
+int bit_width(uint64_t x);
+
+// Exact number of combinations whose m-prefix has lexicographic rank
+// = offset (mod step).
+uint64_t count_combinations(uint64_t n, int m, uint64_t step, uint64_t offset);
+
+// H(Y), 1/N and the screening margin of a dataset's class sizes.
+void mi_constants(uint32_t cases, uint32_t ctrls, ScanParams &P);
+
+}  // namespace fiu
+
+struct fiu_dataset {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = true;
+    fiu::Arena *arena = nullptr;   // set: every device buffer comes from it
+    size_t snps = 0, cw = 0, tw = 0;
+    uint32_t cases = 0, ctrls = 0;
+    std::vector<uint64_t> host_words;
+    fiu::DBuf<uint4> bits;
+    fiu::DBuf<uint32_t> singles;
+    fiu::DBuf<uint32_t> flags;
+    fiu::DevData dev{};
+    // search scratch (grown on demand, kept for later calls)
+    fiu::DBuf<uint64_t> binom;
+    int binom_m = -1;
+    fiu::DBuf<fiu::Cand> cand, top;
+    fiu::DBuf<fiu::TopState> state;
+    fiu::DBuf<uint32_t> redo;
+    fiu::DBuf<uint32_t> pair;        // marginal pair table (order-3 marginal scans)
+    fiu::DBuf<uint32_t> hist;        // screening-MI histogram of the threshold sampling pass
+    bool pair_ready = false;
+    std::vector<hipEvent_t> events;
+    std::mutex prep_mu;              // binom / pair table preparation
+
+    void use_arena(fiu::Arena *a)
+    {
+        arena = a;
+        bits.arena = singles.arena = flags.arena = binom.arena = redo.arena = pair.arena = hist.arena = a;
+        cand.arena = top.arena = a;
+        state.arena = a;
+    }
+    ~fiu_dataset();
+};
+
+namespace fiu {
+
+// Copy host-layout words to the device image of a dataset (transpose to the
+// SNP-minor layout, row popcounts, coverage flags).  `stream` null: the
+// dataset gets a stream of its own.  Memory from `arena` when given.
+fiu_dataset *create_dataset(const uint64_t *words, size_t snps, size_t cw, size_t tw, uint32_t cases,
+                            uint32_t ctrls, Arena *arena = nullptr, hipStream_t stream = nullptr,
+                            bool keep_host_words = true);
+
+// Device bytes a dataset of this shape takes (bits, row counts, flags) and
+// the transient staging its upload needs.
+size_t dataset_bytes(size_t snps, size_t cw, size_t tw);
+size_t staging_bytes(size_t snps, size_t cw, size_t tw);
+
+// binom[j*(n+1)+x] = C(x, j) for j <= m, on the device.
+void ensure_binom(fiu_dataset *d, int m, hipStream_t s);
+// Pair counts f(x, y), x, y in {0, 1}, of every SNP pair (32 bytes per
+// pair).  Returns false when the table does not fit in free device memory
+// (hipMalloc datasets) or in the arena.
+bool ensure_pair_table(fiu_dataset *d, hipStream_t s);
+inline size_t pair_table_bytes(size_t snps) { return snps < 2 ? 0 : snps * (snps - 1) / 2 * 32; }
+
+// Top-N entries -> ascending SNP indices (order per entry) and float MI.
+void unpack_results(const Cand *top, uint32_t count, int order, int bits, int32_t *out_combos, float *out_mi);
+
+}  // namespace fiu

@@ -5,23 +5,27 @@
 
 namespace fiu {
 
-hipError_t launch_scan_direct(int order, const uint4 *bits, const DevData &D, const ScanParams &P, TopState *st,
-                              hipStream_t s);
-hipError_t launch_scan_marginal(int order, const uint4 *bits, const DevData &D, const ScanParams &P, TopState *st,
-                                hipStream_t s);
+hipError_t launch_scan_direct(int order, const DevData &D, const ScanParams &P, TopState *st, hipStream_t s);
+hipError_t launch_scan_marginal(int order, const DevData &D, const ScanParams &P, TopState *st, hipStream_t s);
 hipError_t launch_tau_from_hist(const uint32_t *hist, uint32_t N, float delta, TopState *st, hipStream_t s);
-hipError_t launch_pair_table(const uint4 *bits, const DevData &D, const ScanParams &P, uint32_t *pair_out,
-                             hipStream_t s);
+hipError_t launch_pair_table(const DevData &D, const ScanParams &P, uint32_t *pair_out, hipStream_t s);
 hipError_t launch_merge(TopState *st, Cand *top, const Cand *cand, uint32_t N, uint32_t cap, uint32_t chunk_id,
                         uint32_t *redo, hipStream_t s);
 hipError_t launch_transpose(const uint64_t *src, uint4 *dst, int n, int n_pad, int cls, uint64_t cw, uint64_t tw,
                             int quads, uint64_t class_off, hipStream_t s);
 hipError_t launch_singles(const uint4 *bits, const DevData &D, uint32_t *singles, uint32_t *flags, hipStream_t s);
-hipError_t launch_combine_batch(const uint4 *bits, const DevData &D, const int32_t *combos, int k, uint64_t n,
-                                uint64_t cw, uint64_t tw, uint64_t *out, hipStream_t s);
-hipError_t launch_contingency_batch(const uint4 *bits, const DevData &D, const int32_t *combos, int k, uint64_t n,
-                                    uint64_t ct_size, uint32_t *out, hipStream_t s);
+hipError_t launch_combine_batch(const DevData &D, const int32_t *combos, int k, uint64_t n, uint64_t cw, uint64_t tw,
+                                uint64_t *out, hipStream_t s);
+hipError_t launch_contingency_batch(const DevData &D, const int32_t *combos, int k, uint64_t n, uint64_t ct_size,
+                                    uint32_t *out, hipStream_t s);
 hipError_t launch_mi_batch(const uint32_t *tables, uint64_t n, uint64_t ct_size, double n64, double hy64, float *out,
                            hipStream_t s);
+
+// Batched scans: resident workgroups of the persistent scan kernel of launch
+// 0 (orders 2-3) or 1 (order 4) on the current device, and the whole batched
+// step: counter reset, the two persistent scans, the per-job merges.
+int batch_blocks(int launch);
+hipError_t launch_batch(const BatchJob *jobs, const BatchSeg *segs0, const BatchSeg *segs1, const int32_t *merge_list,
+                        BatchHeader *hdr, int blocks0, int blocks1, int merge_blocks, hipStream_t s);
 
 }  // namespace fiu

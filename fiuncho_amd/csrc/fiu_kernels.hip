@@ -8,12 +8,14 @@
 //   search loop         include/fiuncho/ThreadedSearch.h:142-182
 //   distribution        include/fiuncho/Distribution.h:54-176
 //
-// Scan kernels map one wavefront to one (k-1)-SNP prefix and its 64 lanes to
-// 64 consecutive candidates for the last SNP.  The prefix rows are
-// wave-uniform: they are fetched with scalar loads and ANDed on the scalar
-// unit, so every VALU instruction of the inner loop is either a v_and_b32
-// (SGPR x VGPR) or an accumulating v_bcnt_u32_b32.  Counts stay in VGPRs;
-// MI and the top-N test run in the same kernel (no count table is written).
+// Scan kernels map one wavefront to one or a few (k-1)-SNP prefixes and its
+// 64 lanes to 64 consecutive candidates for the last SNP (a tile).  The
+// prefix rows are wave-uniform 16-byte buffer loads, the lanes' rows one
+// coalesced 1 KiB line per row and quad; every VALU instruction of the inner
+// loop is a v_and_b32 / v_bitop3_b32 or an accumulating v_bcnt_u32_b32.
+// Counts stay in VGPRs; MI and the top-N test run in the same kernel (no
+// count table is written).  The scans are written as per-wavefront units so
+// that one-scan launches and the persistent batched launch share them.
 #include "fiu_kernels.h"
 
 #include <math.h>
@@ -26,17 +28,6 @@ template <> struct Pow3<0> { static constexpr int v = 1; };
 __device__ __forceinline__ uint32_t comp(const uint4 &v, int w)
 {
     return w == 0 ? v.x : w == 1 ? v.y : w == 2 ? v.z : v.w;
-}
-
-// acc += popcount(s & v) as exactly two VALU instructions: v_and_b32 with the
-// wave-uniform operand in an SGPR, then the accumulating v_bcnt_u32_b32.
-// (Left to itself the compiler re-associates the sums into
-// v_bcnt(x, 0) + v_add3_u32, one extra instruction per two cells.)
-__device__ __forceinline__ void and_bcnt(uint32_t &acc, uint32_t s, uint32_t v)
-{
-    uint32_t t;
-    asm("v_and_b32 %0, %1, %2" : "=v"(t) : "s"(s), "v"(v));
-    asm("v_bcnt_u32_b32 %0, %1, %0" : "+v"(acc) : "v"(t));
 }
 
 // Buffer resource over the three genotype rows of one quad (all SNPs of one
@@ -98,27 +89,6 @@ __device__ __forceinline__ void and2_bcnt4(uint32_t &a0, uint32_t &a1, uint32_t 
         "v_and_b32 %3, %10, %11"
         : "=&v"(t0), "=&v"(t1), "=&v"(t2), "=&v"(t3)
         : "v"(x0), "v"(y0), "v"(x1), "v"(y1), "v"(x2), "v"(y2), "v"(x3), "v"(y3));
-    asm("v_bcnt_u32_b32 %0, %4, %0\n\t"
-        "v_bcnt_u32_b32 %1, %5, %1\n\t"
-        "v_bcnt_u32_b32 %2, %6, %2\n\t"
-        "v_bcnt_u32_b32 %3, %7, %3"
-        : "+v"(a0), "+v"(a1), "+v"(a2), "+v"(a3)
-        : "v"(t0), "v"(t1), "v"(t2), "v"(t3));
-}
-
-// Four independent cells at once: the four ANDs are issued before the four
-// popcounts, so no popcount waits on the AND just before it.
-__device__ __forceinline__ void and_bcnt4(uint32_t &a0, uint32_t &a1, uint32_t &a2, uint32_t &a3, uint32_t s0,
-                                          uint32_t v0, uint32_t s1, uint32_t v1, uint32_t s2, uint32_t v2,
-                                          uint32_t s3, uint32_t v3)
-{
-    uint32_t t0, t1, t2, t3;
-    asm("v_and_b32 %0, %4, %5\n\t"
-        "v_and_b32 %1, %6, %7\n\t"
-        "v_and_b32 %2, %8, %9\n\t"
-        "v_and_b32 %3, %10, %11"
-        : "=&v"(t0), "=&v"(t1), "=&v"(t2), "=&v"(t3)
-        : "s"(s0), "v"(v0), "s"(s1), "v"(v1), "s"(s2), "v"(v2), "s"(s3), "v"(v3));
     asm("v_bcnt_u32_b32 %0, %4, %0\n\t"
         "v_bcnt_u32_b32 %1, %5, %1\n\t"
         "v_bcnt_u32_b32 %2, %6, %2\n\t"
@@ -246,11 +216,11 @@ __device__ __forceinline__ void score_and_push(bool valid, const uint32_t *a0, c
     push_candidates(want, mf, combo, P, st);
 }
 
-// Uniform lookup of a prefix: rank -> SNP indices, or an empty marker.
+// Uniform lookup of prefix j < j1: rank -> SNP indices, or an empty marker.
 template <int M>
-__device__ __forceinline__ bool prefix_at(const ScanParams &P, int n, int64_t j, int *pre)
+__device__ __forceinline__ bool prefix_at(const ScanParams &P, int n, int64_t j, int64_t j1, int *pre)
 {
-    if (j >= P.j1) return false;
+    if (j >= j1) return false;
     const uint64_t rank = (uint64_t)P.offset + (uint64_t)j * (uint64_t)P.step;
     if (rank >= P.total_prefixes) return false;
     unrank(rank, n, M, P.binom, pre);
@@ -423,22 +393,17 @@ __device__ __forceinline__ void accumulate_direct(const uint4 *__restrict__ base
     }
 }
 
+// One wavefront's unit of a direct scan: prefix j of the distribution against
+// the tiles t_hi, t_hi - 1, ..., down to max(t_lo, first tile past the
+// prefix).  Whole-prefix units pass t_hi = last tile, t_lo = 0.
 template <int K, bool HIST>
-__global__ void __launch_bounds__(kBlock)
-k_scan_direct(const uint4 *__restrict__ bits, const DevData D, const ScanParams P, TopState *st)
+__device__ __forceinline__ void direct_unit(const DevData &D, const ScanParams &P, TopState *st, int64_t j,
+                                            int64_t j1, int t_hi, int t_lo, uint32_t *lds_hist)
 {
     constexpr int M = K - 1, CELLS = Pow3<K>::v;
-    __shared__ uint32_t lds_hist[HIST ? kHistBins : 1];
-    hist_begin<HIST>(lds_hist);
     const int lane = threadIdx.x & 63;
-    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int64_t j = P.j0 + (int64_t)blockIdx.x * (kBlock / kWave) + wid;
     int pre[M];
-    const bool ok = prefix_at<M>(P, D.n, j, pre);
-    if (!ok) {
-        hist_end<HIST>(lds_hist, P.hist);  // every wave of the block reaches the barrier
-        return;
-    }
+    if (!prefix_at<M>(P, D.n, j, j1, pre)) return;
     uint64_t pre_packed = 0;
 #pragma unroll
     for (int i = 0; i < M; ++i) pre_packed = (pre_packed << P.combo_bits) | (uint64_t)pre[i];
@@ -446,11 +411,12 @@ k_scan_direct(const uint4 *__restrict__ bits, const DevData D, const ScanParams 
     const int last = pre[M - 1];
     const float tau = st->tau;
     const uint64_t tau_combo = st->tau_combo;
-    const uint4 *b0 = bits + D.class_off[0];
-    const uint4 *b1 = bits + D.class_off[1];
+    const uint4 *b0 = D.bits + D.class_off[0];
+    const uint4 *b1 = D.bits + D.class_off[1];
     // tiles are walked from the last one down, so that wavefronts resident at
     // the same time read the same tile at about the same time (shared in L2)
-    for (int t = (D.n_pad >> 6) - 1; t >= (last + 1) >> 6; --t) {
+    const int t_end = max(t_lo, (last + 1) >> 6);
+    for (int t = min(t_hi, (D.n_pad >> 6) - 1); t >= t_end; --t) {
         const int c = (t << 6) | lane;
         uint32_t a0[CELLS], a1[CELLS];
         accumulate_direct<K>(b0, D.n_pad, D.quads[0], pre, c * 16, a0);
@@ -458,7 +424,18 @@ k_scan_direct(const uint4 *__restrict__ bits, const DevData D, const ScanParams 
         score_or_sample<HIST, CELLS>(c > last && c < D.n, a0, a1, pre_packed | (uint64_t)c, tau, tau_combo, P, st,
                                      lds_hist);
     }
-    hist_end<HIST>(lds_hist, P.hist);
+}
+
+template <int K, bool HIST>
+__global__ void __launch_bounds__(kBlock)
+k_scan_direct(const DevData D, const ScanParams P, TopState *st)
+{
+    __shared__ uint32_t lds_hist[HIST ? kHistBins : 1];
+    hist_begin<HIST>(lds_hist);
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    direct_unit<K, HIST>(D, P, st, P.j0 + (int64_t)blockIdx.x * (kBlock / kWave) + wid, P.j1, INT32_MAX, 0,
+                         lds_hist);
+    hist_end<HIST>(lds_hist, P.hist);  // every wave of the block reaches the barrier
 }
 
 // ---------------------------------------------------------------------------
@@ -497,29 +474,26 @@ __device__ __forceinline__ void mobius2(uint32_t *m)
 
 // Order 2.  TABLE = true writes the 8 pair counts (f(x,y), x,y in {0,1}, per
 // class) of every pair instead of scoring it.
+// One wavefront's unit: prefixes g0 .. g0+PW-1 against tiles t_hi down to
+// max(t_lo, first tile past the smallest prefix).
 template <int PW, bool TABLE, bool HIST>
-__global__ void __launch_bounds__(kBlock)
-k_scan2m(const uint4 *__restrict__ bits, const DevData D, const ScanParams P, TopState *st, uint32_t *pair_out)
+__device__ __forceinline__ void scan2m_unit(const DevData &D, const ScanParams &P, TopState *st,
+                                            uint32_t *pair_out, int64_t g0, int64_t j1, int t_hi, int t_lo,
+                                            uint32_t *lds_hist)
 {
-    __shared__ uint32_t lds_hist[HIST ? kHistBins : 1];
-    hist_begin<HIST>(lds_hist);
+    const uint4 *__restrict__ bits = D.bits;
     const int lane = threadIdx.x & 63;
-    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int64_t g0 = P.j0 + ((int64_t)blockIdx.x * (kBlock / kWave) + wid) * PW;
     int pa[PW];
     bool pv[PW];
     int amin = D.n_pad;
 #pragma unroll
     for (int i = 0; i < PW; ++i) {
         int tmp[1];
-        pv[i] = prefix_at<1>(P, D.n, g0 + i, tmp);
+        pv[i] = prefix_at<1>(P, D.n, g0 + i, j1, tmp);
         pa[i] = pv[i] ? tmp[0] : D.n_pad;
         amin = min(amin, pa[i]);
     }
-    if (!pv[0]) {  // slots fill in order: slot 0 empty means all are
-        hist_end<HIST>(lds_hist, P.hist);
-        return;
-    }
+    if (!pv[0]) return;  // slots fill in order: slot 0 empty means all are
     // an empty slot counts a copy of prefix 0 (discarded) so loads stay in range
 #pragma unroll
     for (int i = 1; i < PW; ++i)
@@ -527,7 +501,8 @@ k_scan2m(const uint4 *__restrict__ bits, const DevData D, const ScanParams P, To
     const float tau = TABLE ? 0.0f : st->tau;
     const uint64_t tau_combo = TABLE ? 0 : st->tau_combo;
     const int np = D.n_pad;
-    for (int t = (np >> 6) - 1; t >= (amin + 1) >> 6; --t) {  // last tile first (shared in L2)
+    const int t_end = max(t_lo, (amin + 1) >> 6);
+    for (int t = min(t_hi, (np >> 6) - 1); t >= t_end; --t) {  // last tile first (shared in L2)
         const int c = (t << 6) | lane;
         const int tile_last = (t << 6) | 63;
         uint32_t f[PW][2][4];
@@ -624,6 +599,17 @@ k_scan2m(const uint4 *__restrict__ bits, const DevData D, const ScanParams P, To
                                      P, st, lds_hist);
         }
     }
+}
+
+template <int PW, bool TABLE, bool HIST>
+__global__ void __launch_bounds__(kBlock)
+k_scan2m(const DevData D, const ScanParams P, TopState *st, uint32_t *pair_out)
+{
+    __shared__ uint32_t lds_hist[HIST ? kHistBins : 1];
+    hist_begin<HIST>(lds_hist);
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    scan2m_unit<PW, TABLE, HIST>(D, P, st, pair_out, P.j0 + ((int64_t)blockIdx.x * (kBlock / kWave) + wid) * PW,
+                                 P.j1, INT32_MAX, 0, lds_hist);
     hist_end<HIST>(lds_hist, P.hist);
 }
 
@@ -633,30 +619,26 @@ k_scan2m(const uint4 *__restrict__ bits, const DevData D, const ScanParams P, To
 // complete in order: the rows of prefix i+1 are issued before prefix i is
 // counted, and the lanes' next quad before the last two prefixes, so every
 // wait covers data that had a whole prefix of counting to arrive.
+// Unit: prefixes g0 .. g0+PW-1, tiles t_hi down to max(t_lo, first tile past
+// the smallest prefix).
 template <int PW, bool HIST>
-__global__ void __launch_bounds__(kBlock)
-k_scan3m(const uint4 *__restrict__ bits, const DevData D, const ScanParams P, TopState *st)
+__device__ __forceinline__ void scan3m_unit(const DevData &D, const ScanParams &P, TopState *st, int64_t g0,
+                                            int64_t j1, int t_hi, int t_lo, uint32_t *lds_hist)
 {
-    __shared__ uint32_t lds_hist[HIST ? kHistBins : 1];
-    hist_begin<HIST>(lds_hist);
+    const uint4 *__restrict__ bits = D.bits;
     const int lane = threadIdx.x & 63;
-    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int64_t g0 = P.j0 + ((int64_t)blockIdx.x * (kBlock / kWave) + wid) * PW;
     int pa[PW], pb[PW];
     bool pv[PW];
     int bmin = D.n_pad;
 #pragma unroll
     for (int i = 0; i < PW; ++i) {
         int tmp[2];
-        pv[i] = prefix_at<2>(P, D.n, g0 + i, tmp);
+        pv[i] = prefix_at<2>(P, D.n, g0 + i, j1, tmp);
         pa[i] = pv[i] ? tmp[0] : 0;
         pb[i] = pv[i] ? tmp[1] : D.n_pad;
         bmin = min(bmin, pb[i]);
     }
-    if (!pv[0]) {  // slots fill in order: slot 0 empty means all are
-        hist_end<HIST>(lds_hist, P.hist);
-        return;
-    }
+    if (!pv[0]) return;  // slots fill in order: slot 0 empty means all are
     // an empty slot counts a copy of prefix 0 (discarded) so loads stay in range
 #pragma unroll
     for (int i = 1; i < PW; ++i)
@@ -665,7 +647,8 @@ k_scan3m(const uint4 *__restrict__ bits, const DevData D, const ScanParams P, To
     const uint64_t tau_combo = st->tau_combo;
     const int np = D.n_pad, n = D.n;
     const uint32_t *pair = P.pair;
-    for (int t = (np >> 6) - 1; t >= (bmin + 1) >> 6; --t) {  // last tile first (shared in L2)
+    const int t_end = max(t_lo, (bmin + 1) >> 6);
+    for (int t = min(t_hi, (np >> 6) - 1); t >= t_end; --t) {  // last tile first (shared in L2)
         const int c = (t << 6) | lane;
         const int tile_last = (t << 6) | 63;
         uint32_t f[PW][2][8];
@@ -789,18 +772,27 @@ k_scan3m(const uint4 *__restrict__ bits, const DevData D, const ScanParams P, To
             score_or_sample<HIST, 27>(valid, m0, m1, combo, tau, tau_combo, P, st, lds_hist);
         }
     }
+}
+
+template <int PW, bool HIST>
+__global__ void __launch_bounds__(kBlock)
+k_scan3m(const DevData D, const ScanParams P, TopState *st)
+{
+    __shared__ uint32_t lds_hist[HIST ? kHistBins : 1];
+    hist_begin<HIST>(lds_hist);
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    scan3m_unit<PW, HIST>(D, P, st, P.j0 + ((int64_t)blockIdx.x * (kBlock / kWave) + wid) * PW, P.j1, INT32_MAX,
+                          0, lds_hist);
     hist_end<HIST>(lds_hist, P.hist);
 }
 
 // ---------------------------------------------------------------------------
 // Top-N merge (one workgroup): current list + candidates -> best N, unique.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(1024)
-k_merge(TopState *st, Cand *top, const Cand *cand, uint32_t N, uint32_t cap, uint32_t chunk_id,
-        uint32_t *redo)
+// Body shared by the single-scan and the batched merge (1024 threads).
+__device__ __forceinline__ void merge_topn(TopState *st, Cand *top, const Cand *cand, uint32_t N, uint32_t cap,
+                                           Cand *buf, uint32_t *psum)
 {
-    __shared__ Cand buf[kMergeSlots];
-    __shared__ uint32_t psum[1024];
     const int tid = threadIdx.x;
     const uint32_t count = min(st->cand_count, cap);
     uint32_t n_top = st->n_top;
@@ -874,10 +866,106 @@ k_merge(TopState *st, Cand *top, const Cand *cand, uint32_t N, uint32_t cap, uin
             st->tau_combo = top[N - 1].combo;
         }
         st->cand_count = 0;
-        if (st->overflow) {
-            redo[st->redo_count++] = chunk_id;
-            st->overflow = 0;
+    }
+}
+
+__global__ void __launch_bounds__(1024)
+k_merge(TopState *st, Cand *top, const Cand *cand, uint32_t N, uint32_t cap, uint32_t chunk_id,
+        uint32_t *redo)
+{
+    __shared__ Cand buf[kMergeSlots];
+    __shared__ uint32_t psum[1024];
+    merge_topn(st, top, cand, N, cap, buf, psum);
+    if (threadIdx.x == 0 && st->overflow) {
+        redo[st->redo_count++] = chunk_id;
+        st->overflow = 0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Batched scans.
+// ---------------------------------------------------------------------------
+// Persistent scan over the segments of one launch: every wavefront takes the
+// next unit from a counter until the units run out (every wave reaches the
+// exit, so the grid drains).  LAUNCH 0 runs the order-2/3 kinds in one
+// kernel, LAUNCH 1 the order-4 kind, whose register budget would otherwise
+// set the occupancy of the others.
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef const __attribute__((address_space(4))) BatchJob ConstJob;
+#else
+typedef const BatchJob ConstJob;  // host pass: kernel bodies are only parsed
+#endif
+
+template <int LAUNCH>
+__global__ void __launch_bounds__(kBlock)
+k_batch_scan(const BatchJob *__restrict__ jobs, const BatchSeg *__restrict__ segs, BatchHeader *hdr)
+{
+    const int lane = threadIdx.x & 63;
+    const uint32_t nseg = hdr->nseg[LAUNCH], units = hdr->units[LAUNCH];
+    for (;;) {
+        uint32_t u = 0;
+        if (lane == 0) u = atomicAdd(&hdr->head[LAUNCH], 1u);
+        u = __builtin_amdgcn_readfirstlane(u);
+        if (u >= units) return;
+        // segment holding unit u: the last one whose start is <= u
+        uint32_t lo = 0, hi = nseg - 1;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi + 1) >> 1;
+            if (segs[mid].start <= u) lo = mid;
+            else hi = mid - 1;
         }
+        const int64_t j0 = segs[lo].j0, j1 = segs[lo].j1;
+        // The job table is read-only while the launch runs: reading it through
+        // the constant address space lets the compiler treat every field as
+        // invariant (scalar loads it may repeat instead of keeping registers).
+        const ConstJob &J = ((const ConstJob *)jobs)[segs[lo].job];
+        const DevData D = J.D;
+        const ScanParams P = J.P;
+        TopState *st = J.st;
+        const uint32_t chunks = (uint32_t)J.chunks, local = u - (uint32_t)segs[lo].start;
+        const int64_t g = (int64_t)(local / chunks);
+        const int c = (int)(local % chunks);
+        const int t_hi = (D.n_pad >> 6) - 1 - c * J.tiles, t_lo = t_hi - J.tiles + 1;
+        if constexpr (LAUNCH == 0) {
+            switch (J.kind) {
+            case kKindScan2m:
+                scan2m_unit<kMarginalPW, false, false>(D, P, st, nullptr, j0 + g * kMarginalPW, j1, t_hi, t_lo,
+                                                       nullptr);
+                break;
+            case kKindScan3m:
+                scan3m_unit<kMarginalPW, false>(D, P, st, j0 + g * kMarginalPW, j1, t_hi, t_lo, nullptr);
+                break;
+            case kKindDirect2: direct_unit<2, false>(D, P, st, j0 + g, j1, t_hi, t_lo, nullptr); break;
+            case kKindDirect3: direct_unit<3, false>(D, P, st, j0 + g, j1, t_hi, t_lo, nullptr); break;
+            default: break;
+            }
+        } else {
+            direct_unit<4, false>(D, P, st, j0 + g, j1, t_hi, t_lo, nullptr);
+        }
+    }
+}
+
+// One workgroup per job of the launch: merge its candidates into its top-N
+// and record whether its candidate buffer overflowed in this launch.
+__global__ void __launch_bounds__(1024)
+k_batch_merge(const BatchJob *__restrict__ jobs, const int32_t *__restrict__ list, const BatchHeader *hdr)
+{
+    __shared__ Cand buf[kMergeSlots];
+    __shared__ uint32_t psum[1024];
+    if (blockIdx.x >= hdr->nmerge) return;
+    const BatchJob &J = jobs[list[blockIdx.x]];
+    merge_topn(J.st, J.top, J.P.cand, J.outputs, J.P.cand_cap, buf, psum);
+    if (threadIdx.x == 0) {
+        J.st->step_overflow = J.st->overflow;
+        J.st->overflow = 0;
+    }
+}
+
+__global__ void k_batch_reset(BatchHeader *hdr)
+{
+    if (threadIdx.x == 0) {
+        hdr->head[0] = 0;
+        hdr->head[1] = 0;
     }
 }
 
@@ -1021,30 +1109,27 @@ __global__ void k_mi_batch(const uint32_t *tables, uint64_t n, uint64_t ct_size,
 // Host launchers.
 // ---------------------------------------------------------------------------
 template <bool HIST>
-static hipError_t scan_direct(int order, const uint4 *bits, const DevData &D, const ScanParams &P, TopState *st,
-                              hipStream_t s)
+static hipError_t scan_direct(int order, const DevData &D, const ScanParams &P, TopState *st, hipStream_t s)
 {
     const int64_t waves = P.j1 - P.j0;
     if (waves <= 0) return hipSuccess;
     const dim3 grid((unsigned)((waves + 3) / 4)), block(kBlock);
     switch (order) {
-    case 2: hipLaunchKernelGGL((k_scan_direct<2, HIST>), grid, block, 0, s, bits, D, P, st); break;
-    case 3: hipLaunchKernelGGL((k_scan_direct<3, HIST>), grid, block, 0, s, bits, D, P, st); break;
-    case 4: hipLaunchKernelGGL((k_scan_direct<4, HIST>), grid, block, 0, s, bits, D, P, st); break;
+    case 2: hipLaunchKernelGGL((k_scan_direct<2, HIST>), grid, block, 0, s, D, P, st); break;
+    case 3: hipLaunchKernelGGL((k_scan_direct<3, HIST>), grid, block, 0, s, D, P, st); break;
+    case 4: hipLaunchKernelGGL((k_scan_direct<4, HIST>), grid, block, 0, s, D, P, st); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
-hipError_t launch_scan_direct(int order, const uint4 *bits, const DevData &D, const ScanParams &P, TopState *st,
-                              hipStream_t s)
+hipError_t launch_scan_direct(int order, const DevData &D, const ScanParams &P, TopState *st, hipStream_t s)
 {
-    return P.hist ? scan_direct<true>(order, bits, D, P, st, s) : scan_direct<false>(order, bits, D, P, st, s);
+    return P.hist ? scan_direct<true>(order, D, P, st, s) : scan_direct<false>(order, D, P, st, s);
 }
 
 template <bool HIST>
-static hipError_t scan_marginal(int order, const uint4 *bits, const DevData &D, const ScanParams &P, TopState *st,
-                                hipStream_t s)
+static hipError_t scan_marginal(int order, const DevData &D, const ScanParams &P, TopState *st, hipStream_t s)
 {
     const int64_t prefixes = P.j1 - P.j0;
     if (prefixes <= 0) return hipSuccess;
@@ -1052,19 +1137,39 @@ static hipError_t scan_marginal(int order, const uint4 *bits, const DevData &D, 
     const dim3 grid((unsigned)((prefixes + per_block - 1) / per_block)), block(kBlock);
     switch (order) {
     case 2:
-        hipLaunchKernelGGL((k_scan2m<kMarginalPW, false, HIST>), grid, block, 0, s, bits, D, P, st,
-                           (uint32_t *)nullptr);
+        hipLaunchKernelGGL((k_scan2m<kMarginalPW, false, HIST>), grid, block, 0, s, D, P, st, (uint32_t *)nullptr);
         break;
-    case 3: hipLaunchKernelGGL((k_scan3m<kMarginalPW, HIST>), grid, block, 0, s, bits, D, P, st); break;
+    case 3: hipLaunchKernelGGL((k_scan3m<kMarginalPW, HIST>), grid, block, 0, s, D, P, st); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
-hipError_t launch_scan_marginal(int order, const uint4 *bits, const DevData &D, const ScanParams &P, TopState *st,
-                                hipStream_t s)
+hipError_t launch_scan_marginal(int order, const DevData &D, const ScanParams &P, TopState *st, hipStream_t s)
 {
-    return P.hist ? scan_marginal<true>(order, bits, D, P, st, s) : scan_marginal<false>(order, bits, D, P, st, s);
+    return P.hist ? scan_marginal<true>(order, D, P, st, s) : scan_marginal<false>(order, D, P, st, s);
+}
+
+int batch_blocks(int launch)
+{
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    const hipError_t e = launch == 0
+                             ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_scan<0>, kBlock, 0)
+                             : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_scan<1>, kBlock, 0);
+    if (e != hipSuccess) return 0;
+    return std::max(1, per_cu) * cus;
+}
+
+hipError_t launch_batch(const BatchJob *jobs, const BatchSeg *segs0, const BatchSeg *segs1, const int32_t *merge_list,
+                        BatchHeader *hdr, int blocks0, int blocks1, int merge_blocks, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, hdr);
+    hipLaunchKernelGGL((k_batch_scan<0>), dim3(blocks0), dim3(kBlock), 0, s, jobs, segs0, hdr);
+    if (blocks1 > 0) hipLaunchKernelGGL((k_batch_scan<1>), dim3(blocks1), dim3(kBlock), 0, s, jobs, segs1, hdr);
+    hipLaunchKernelGGL(k_batch_merge, dim3(merge_blocks), dim3(1024), 0, s, jobs, merge_list, hdr);
+    return hipGetLastError();
 }
 
 hipError_t launch_tau_from_hist(const uint32_t *hist, uint32_t N, float delta, TopState *st, hipStream_t s)
@@ -1073,15 +1178,13 @@ hipError_t launch_tau_from_hist(const uint32_t *hist, uint32_t N, float delta, T
     return hipGetLastError();
 }
 
-hipError_t launch_pair_table(const uint4 *bits, const DevData &D, const ScanParams &P, uint32_t *pair_out,
-                             hipStream_t s)
+hipError_t launch_pair_table(const DevData &D, const ScanParams &P, uint32_t *pair_out, hipStream_t s)
 {
     const int64_t prefixes = P.j1 - P.j0;
     if (prefixes <= 0) return hipSuccess;
     const int64_t per_block = (kBlock / kWave) * kMarginalPW;
     const dim3 grid((unsigned)((prefixes + per_block - 1) / per_block)), block(kBlock);
-    hipLaunchKernelGGL((k_scan2m<kMarginalPW, true, false>), grid, block, 0, s, bits, D, P, (TopState *)nullptr,
-                       pair_out);
+    hipLaunchKernelGGL((k_scan2m<kMarginalPW, true, false>), grid, block, 0, s, D, P, (TopState *)nullptr, pair_out);
     return hipGetLastError();
 }
 
@@ -1109,23 +1212,23 @@ hipError_t launch_singles(const uint4 *bits, const DevData &D, uint32_t *singles
     return hipGetLastError();
 }
 
-hipError_t launch_combine_batch(const uint4 *bits, const DevData &D, const int32_t *combos, int k, uint64_t n,
-                                uint64_t cw, uint64_t tw, uint64_t *out, hipStream_t s)
+hipError_t launch_combine_batch(const DevData &D, const int32_t *combos, int k, uint64_t n, uint64_t cw, uint64_t tw,
+                                uint64_t *out, hipStream_t s)
 {
     uint64_t R = 1;
     for (int i = 0; i < k; ++i) R *= 3;
     const uint64_t total = n * R * (cw + tw);
     if (!total) return hipSuccess;
-    hipLaunchKernelGGL(k_combine_batch, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, bits, D, combos, k,
+    hipLaunchKernelGGL(k_combine_batch, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, D.bits, D, combos, k,
                        n, cw, tw, out);
     return hipGetLastError();
 }
 
-hipError_t launch_contingency_batch(const uint4 *bits, const DevData &D, const int32_t *combos, int k, uint64_t n,
-                                    uint64_t ct_size, uint32_t *out, hipStream_t s)
+hipError_t launch_contingency_batch(const DevData &D, const int32_t *combos, int k, uint64_t n, uint64_t ct_size,
+                                    uint32_t *out, hipStream_t s)
 {
     if (!n) return hipSuccess;
-    hipLaunchKernelGGL(k_contingency_batch, dim3((unsigned)n), dim3(64), 0, s, bits, D, combos, k, ct_size, out);
+    hipLaunchKernelGGL(k_contingency_batch, dim3((unsigned)n), dim3(64), 0, s, D.bits, D, combos, k, ct_size, out);
     return hipGetLastError();
 }
 

@@ -174,6 +174,106 @@ fiu_status fiu_contingency_batch(const fiu_dataset *ds, const int32_t *combos,
 fiu_status fiu_mi_batch(const uint32_t *tables, size_t n, uint16_t order,
                         uint32_t cases, uint32_t ctrls, float *out_mi);
 
+/* ------------------------------------------------------------------------ */
+/* Batched scans and the scan server                                         */
+/* ------------------------------------------------------------------------ */
+/* No counterpart in the reference, which runs one search per process
+ * (MPIEngine.h:135-137).  Many searches over different datasets, sizes and
+ * orders advance together through batched device steps: one persistent
+ * launch over a slice of every active search, then one top-N merge per
+ * search, replayed from a HIP graph.  Each search returns exactly what
+ * fiu_search returns for it alone. */
+
+typedef struct {
+    uint32_t struct_size;    /* sizeof(fiu_batch_config) */
+    uint32_t max_active;     /* searches on the device at once (<= 1024) */
+    uint64_t memory_budget;  /* server: device bytes for the tables it
+                                uploads, 0 = a quarter of the free
+                                memory, at most 8 GiB */
+    double step_ms;          /* target device time of one batched step */
+    uint32_t candidate_cap;  /* candidate buffer entries per search */
+    uint32_t flags;          /* reserved, 0 */
+} fiu_batch_config;
+
+void fiu_batch_config_init(fiu_batch_config *cfg);
+
+typedef struct {
+    const fiu_dataset *dataset;
+    uint16_t order;
+    uint16_t reserved;
+    int32_t dist_step, dist_offset;   /* Distribution (step, offset) */
+    uint32_t outputs;
+    int32_t *out_combos;              /* outputs*order ints */
+    float *out_mi;                    /* outputs floats */
+    uint32_t out_count;               /* filled: results */
+    int32_t status;                   /* filled: fiu_status of this search */
+} fiu_scan;
+
+typedef struct {
+    uint64_t steps;          /* batched steps launched */
+    uint64_t units;          /* wavefront work units executed */
+    uint64_t redo_ranges;    /* prefix ranges scanned again (candidate overflow) */
+    double device_ms;        /* device time of the steps (HIP events) */
+    double total_ms;         /* wall time of the call */
+} fiu_batch_stats;
+
+/* Run n searches together; returns when all are done.  Each fiu_scan gets
+ * its own results and status; the call returns the first failing status
+ * (FIU_OK when every search succeeded).  `cfg` and `stats` may be NULL. */
+fiu_status fiu_search_batch(fiu_scan *scans, size_t n, const fiu_batch_config *cfg,
+                            fiu_batch_stats *stats);
+
+/* Scan server: a scheduler thread keeps stepping the searches on the device
+ * while new requests are admitted.  A request names a resident dataset, or
+ * carries host tables (reference layout, copied at submit) that the server
+ * uploads into its memory budget when the request's footprint (tables, pair
+ * table, binomials) fits, and frees when the request completes.  Requests
+ * that can never fit are refused with FIU_ERR_REJECTED. */
+typedef struct fiu_server fiu_server;
+
+typedef struct {
+    uint32_t struct_size;             /* sizeof(fiu_request) */
+    uint16_t order;
+    uint16_t reserved;
+    const fiu_dataset *dataset;       /* resident dataset, or NULL and: */
+    const uint64_t *words;            /* host tables, as fiu_dataset_create */
+    uint64_t snps, cases_words, ctrls_words;
+    uint32_t cases, ctrls;
+    int32_t dist_step, dist_offset;
+    uint32_t outputs;
+    uint32_t reserved2;
+} fiu_request;
+
+typedef struct {
+    double submit_ms, admit_ms, start_ms, done_ms;   /* since server creation */
+    double latency_ms;       /* done_ms - submit_ms */
+    uint64_t combinations;   /* combinations evaluated */
+    uint64_t steps;          /* batched steps the request took part in */
+    int32_t algorithm;       /* fiu_algorithm used */
+    int32_t reserved;
+} fiu_request_stats;
+
+typedef struct {
+    uint64_t queued;         /* submitted, not yet on the device */
+    uint64_t active;         /* on the device */
+    uint64_t completed;
+    uint64_t memory_budget, memory_used;
+    uint64_t steps;
+    double device_ms;
+} fiu_server_info;
+
+fiu_status fiu_server_create(const fiu_batch_config *cfg, fiu_server **out);
+fiu_status fiu_server_submit(fiu_server *sv, const fiu_request *req, uint64_t *id);
+/* Wait for request `id` (timeout_ms < 0: no limit; 0: poll) and collect its
+ * results (out buffers of outputs*order ints and outputs floats).  Returns
+ * FIU_ERR_PENDING on timeout; a collected request is forgotten. */
+fiu_status fiu_server_wait(fiu_server *sv, uint64_t id, double timeout_ms,
+                           int32_t *out_combos, float *out_mi, uint32_t *out_count,
+                           fiu_request_stats *stats);
+fiu_status fiu_server_query(fiu_server *sv, fiu_server_info *out);
+/* Finishes the requests already on the device; queued ones are dropped. */
+fiu_status fiu_server_destroy(fiu_server *sv);
+
 
 /* ------------------------------------------------------------------------ */
 /* Synthetic data (no counterpart in the reference: its tests read files)    */
