@@ -97,6 +97,168 @@ def cpu_baseline(words, geno, cw, tw, threads, sample_step):
                       f"oracle AVX-512 restatement, {threads} threads"}
 
 
+# ---------------------------------------------------------------------------
+# Batched and served forms against the plain one-search-per-call form
+# ---------------------------------------------------------------------------
+def uneven_specs(count, seed0, scale):
+    """Uneven scans: samples ~ U{1,000..50,000}, case fraction ~ U(0.3, 0.7),
+    order 2 or 3 (every 16th dataset order 4), MAF ~ U(0.05, 0.5), one seed per
+    dataset (seed0 + i).  SNP counts per order come from `scale`."""
+    out = []
+    for i in range(count):
+        r = np.random.default_rng(seed0 + i)
+        order = 4 if i % 16 == 15 else int(r.choice([2, 3]))
+        lo, hi, nlo, nhi = scale[order]
+        samples = int(r.integers(nlo, nhi + 1))
+        frac = float(r.uniform(0.3, 0.7))
+        cases = max(1, int(round(samples * frac)))
+        out.append(dict(order=order, snps=int(r.integers(lo, hi + 1)), cases=cases, ctrls=samples - cases,
+                        seed=seed0 + i))
+    return out
+
+
+# SNP and sample ranges per order: the uneven-batch workload keeps the stated
+# order-2 shapes (500..20,000 SNPs, 1,000..50,000 samples) and scales the
+# order-3 / order-4 SNP counts so the whole batch is seconds, not hours.
+BATCH_SCALE = {2: (500, 20000, 1000, 50000), 3: (200, 1500, 1000, 50000), 4: (100, 300, 1000, 5000)}
+# Served requests: small scans, about 1-20 ms each.
+SERVE_SCALE = {2: (500, 4000, 1000, 20000), 3: (100, 600, 1000, 20000), 4: (30, 80, 1000, 5000)}
+BATCH_OUTPUTS = 50
+
+
+def make_words(sp):
+    import fiuncho_amd as fa
+    spec = fa.synth_spec(sp["snps"], sp["cases"], sp["ctrls"], seed=sp["seed"])
+    words, _, cw, tw = fa.synth_generate(spec, words=True)
+    return words, cw, tw
+
+
+def combo_samples(sp):
+    return math.comb(sp["snps"], sp["order"]) * (sp["cases"] + sp["ctrls"])
+
+
+def bench_batch(count=64):
+    """64 uneven datasets (seeds 100..163), top-50 each: every search one call
+    (plain) against all of them in one fiu_search_batch (batched).  Datasets
+    are resident and their tables prepared before either is timed."""
+    import fiuncho_amd as fa
+    specs = uneven_specs(count, 100, BATCH_SCALE)
+    dss = []
+    for sp in specs:
+        words, cw, tw = make_words(sp)
+        d = fa.Dataset.from_words(words, sp["snps"], cw, tw, sp["cases"], sp["ctrls"])
+        del words
+        # binomial and pair tables (a one-prefix search builds them)
+        fa.GpuSearch().run_arrays(d, sp["order"], fa.Distribution(sp["snps"], sp["order"] - 1, 1 << 30, 0), 1)
+        dss.append(d)
+    print(f"[bench] batch: {count} datasets resident", file=sys.stderr, flush=True)
+    scans = [(d, sp["order"], fa.Distribution(sp["snps"], sp["order"] - 1), BATCH_OUTPUTS) for d, sp in zip(dss, specs)]
+    t0 = time.perf_counter()
+    plain = [fa.GpuSearch().run_arrays(*s) for s in scans]
+    t_plain = time.perf_counter() - t0
+    print(f"[bench] batch: plain {t_plain:.2f} s", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    batched, st = fa.search_batch(scans)
+    t_batch = time.perf_counter() - t0
+    print(f"[bench] batch: batched {t_batch:.2f} s", file=sys.stderr, flush=True)
+    same = all(a[0].tolist() == b[0].tolist() and np.array_equal(a[1], b[1]) for a, b in zip(plain, batched))
+    work = sum(combo_samples(sp) for sp in specs)
+    orders = [sp["order"] for sp in specs]
+    for d in dss:
+        d.close()
+    return {"workload": f"{count} uneven datasets (seeds 100..{99 + count}): order 2 with 500..20,000 SNPs, "
+                        "order 3 with 200..1,500 SNPs, order 4 (every 16th) with 100..300 SNPs and "
+                        "1,000..5,000 samples; 1,000..50,000 samples, case fraction 0.3..0.7, top-50",
+            "orders": {str(k): orders.count(k) for k in (2, 3, 4)},
+            "combo_samples": work, "plain_s": t_plain, "batched_s": t_batch,
+            "speedup": t_plain / t_batch, "batched_combo_samples_per_s": work / t_batch,
+            "steps": st.steps, "device_s": st.device_ms / 1e3, "results_identical": same}
+
+
+def percentile(xs, q):
+    return float(np.percentile(np.asarray(xs, float), q)) if xs else None
+
+
+def bench_serve(count=300, load=0.7):
+    """Scan-request stream: `count` requests as above at serving scale (seed
+    1000 + request id), each carrying host tables.  Capacity of each form
+    with every request queued at once, then a Poisson stream at `load` x the
+    served capacity through both forms (plain: one worker, one request at a
+    time, FIFO).  Latency = completion - arrival."""
+    import fiuncho_amd as fa
+    specs = uneven_specs(count, 1000, SERVE_SCALE)
+    reqs = []
+    for sp in specs:
+        words, cw, tw = make_words(sp)
+        reqs.append(dict(order=sp["order"], outputs=BATCH_OUTPUTS, words=words, snps=sp["snps"],
+                         cases_words=cw, ctrls_words=tw, cases=sp["cases"], ctrls=sp["ctrls"]))
+    work = sum(combo_samples(sp) for sp in specs)
+    print(f"[bench] serve: {count} requests generated", file=sys.stderr, flush=True)
+
+    def plain_one(r):
+        d = fa.Dataset.from_words(r["words"], r["snps"], r["cases_words"], r["ctrls_words"], r["cases"], r["ctrls"])
+        out = fa.GpuSearch().run_arrays(d, r["order"], fa.Distribution(r["snps"], r["order"] - 1), r["outputs"])
+        d.close()
+        return out
+
+    # capacity, every request queued at t = 0
+    t0 = time.perf_counter()
+    plain_res = [plain_one(r) for r in reqs]
+    t_plain = time.perf_counter() - t0
+    srv = fa.Server()
+    t0 = time.perf_counter()
+    ids = [srv.submit(**r) for r in reqs]
+    served_res = [srv.wait(i) for i in ids]
+    t_served = time.perf_counter() - t0
+    same = all(a[0].tolist() == b[0].tolist() and np.array_equal(a[1], b[1]) for a, b in zip(plain_res, served_res))
+    print(f"[bench] serve: capacity plain {count / t_plain:.1f} req/s, served {count / t_served:.1f} req/s",
+          file=sys.stderr, flush=True)
+    # Poisson arrivals at `load` x the served capacity
+    rate = load * count / t_served
+    gaps = np.random.default_rng(7).exponential(1.0 / rate, size=count)
+    arrivals = np.cumsum(gaps)
+
+    t0 = time.perf_counter()
+    sub = []
+    for r, a in zip(reqs, arrivals):
+        dt = a - (time.perf_counter() - t0)
+        if dt > 0:
+            time.sleep(dt)
+        sub.append(srv.submit(**r))
+    lat_served = [srv.wait(i)[2].latency_ms for i in sub]
+    t_stream_served = time.perf_counter() - t0
+    srv.close()
+    print(f"[bench] serve: served stream p50 {percentile(lat_served, 50):.1f} ms, "
+          f"p99 {percentile(lat_served, 99):.1f} ms", file=sys.stderr, flush=True)
+
+    t0 = time.perf_counter()
+    lat_plain = []
+    for r, a in zip(reqs, arrivals):
+        dt = a - (time.perf_counter() - t0)
+        if dt > 0:
+            time.sleep(dt)
+        plain_one(r)
+        lat_plain.append((time.perf_counter() - t0 - a) * 1e3)
+    t_stream_plain = time.perf_counter() - t0
+    print(f"[bench] serve: plain stream p50 {percentile(lat_plain, 50):.1f} ms, "
+          f"p99 {percentile(lat_plain, 99):.1f} ms", file=sys.stderr, flush=True)
+    orders = [sp["order"] for sp in specs]
+    return {"workload": f"{count} requests (seeds 1000..{999 + count}): order 2 with 500..4,000 SNPs, order 3 "
+                        "with 100..600 SNPs, order 4 (every 16th) with 30..80 SNPs; 1,000..20,000 samples "
+                        "(order 4: ..5,000), case fraction 0.3..0.7, top-50; host tables uploaded per request",
+            "orders": {str(k): orders.count(k) for k in (2, 3, 4)},
+            "combo_samples": work,
+            "capacity_req_per_s": {"plain": count / t_plain, "served": count / t_served},
+            "throughput_speedup": t_plain / t_served,
+            "served_combo_samples_per_s": work / t_served,
+            "stream": {"arrival": "Poisson", "offered_req_per_s": rate, "load_vs_served_capacity": load,
+                       "served": {"p50_ms": percentile(lat_served, 50), "p99_ms": percentile(lat_served, 99),
+                                  "wall_s": t_stream_served},
+                       "plain": {"p50_ms": percentile(lat_plain, 50), "p99_ms": percentile(lat_plain, 99),
+                                 "wall_s": t_stream_plain}},
+            "results_identical": same}
+
+
 def write_raw(path, geno, cases):
     """PLINK .raw (FID IID PAT MAT SEX PHENOTYPE + one column per SNP)."""
     snps, n = geno.shape
@@ -124,6 +286,12 @@ def main():
     ap.add_argument("--algorithm", default="auto", choices=["auto", "direct", "marginal"])
     ap.add_argument("--cpu-sample-step", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the uneven-batch comparison")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served-stream comparison")
+    ap.add_argument("--batch-datasets", type=int, default=64)
+    ap.add_argument("--serve-requests", type=int, default=300)
+    ap.add_argument("--only-extra", action="store_true",
+                    help="run only the batch / serve comparisons (no headline scan)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run one device-to-device copy of BYTES (a plain copy kernel of known "
                          "size, used to scale the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -150,6 +318,14 @@ def main():
 
     import fiuncho_amd as fa
 
+    if args.only_extra:
+        extra = {}
+        if not args.no_batch:
+            extra["batched"] = bench_batch(args.batch_datasets)
+        if not args.no_serve:
+            extra["serving"] = bench_serve(args.serve_requests)
+        print(json.dumps(extra), flush=True)
+        return
     algo = {"auto": fa.ALGO_AUTO, "direct": fa.ALGO_DIRECT, "marginal": fa.ALGO_MARGINAL}[args.algorithm]
     t0 = time.perf_counter()
     spec = fa.synth_spec(N_SNPS, N_CASES, N_CTRLS, seed=1, planted=PLANTED, penetrance_seed=2)
@@ -215,6 +391,11 @@ def main():
                      "launches": launches, "avg_launch_ms": scan_ms / max(launches, 1)},
         "top1": {"combination": list(top[0][0]), "mi": top[0][1]} if top else None,
     }
+    ds.close()
+    if not args.no_batch:
+        line["batched"] = bench_batch(args.batch_datasets)
+    if not args.no_serve:
+        line["serving"] = bench_serve(args.serve_requests)
     if not args.no_cpu:
         threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
         print(f"[bench] CPU baseline on {threads} threads ...", file=sys.stderr, flush=True)

@@ -173,6 +173,158 @@ fiu_dataset *create_dataset(const uint64_t *words, size_t snps, size_t cw, size_
     return d.release();
 }
 
+namespace {
+
+size_t align_up(size_t b) { return (b + Arena::kAlign - 1) / Arena::kAlign * Arena::kAlign; }
+
+// Row popcounts, disjointness of each SNP's rows and equality of their union
+// with SNP 0's union, per class -- what k_singles computes on the device.
+void host_singles(const uint64_t *words, size_t snps, size_t cw, size_t tw, uint32_t *singles, bool &uniform,
+                  uint32_t total[2])
+{
+    const size_t per = 3 * (cw + tw);
+    bool bad = false;
+    for (int cls = 0; cls < 2; ++cls) {
+        const size_t W = cls ? tw : cw, off = cls ? 3 * cw : 0;
+        std::vector<uint64_t> ref(W, 0);
+        if (snps) {
+            const uint64_t *r = words + off;
+            for (size_t w = 0; w < W; ++w) ref[w] = r[w] | r[W + w] | r[2 * W + w];
+        }
+        uint32_t tot = 0;
+        for (size_t w = 0; w < W; ++w) tot += (uint32_t)__builtin_popcountll(ref[w]);
+        total[cls] = snps ? tot : 0;
+        for (size_t s = 0; s < snps; ++s) {
+            const uint64_t *r0 = words + s * per + off, *r1 = r0 + W, *r2 = r1 + W;
+            uint32_t c0 = 0, c1 = 0, c2 = 0;
+            uint64_t b = 0;
+            for (size_t w = 0; w < W; ++w) {
+                const uint64_t a = r0[w], bb = r1[w], c = r2[w];
+                c0 += (uint32_t)__builtin_popcountll(a);
+                c1 += (uint32_t)__builtin_popcountll(bb);
+                c2 += (uint32_t)__builtin_popcountll(c);
+                b |= (a & bb) | (a & c) | (bb & c) | ((a | bb | c) ^ ref[w]);
+            }
+            singles[(s * 2 + cls) * 3 + 0] = c0;
+            singles[(s * 2 + cls) * 3 + 1] = c1;
+            singles[(s * 2 + cls) * 3 + 2] = c2;
+            bad |= b != 0;
+        }
+    }
+    uniform = snps > 0 && !bad;
+}
+
+}  // namespace
+
+size_t staged_host_bytes(size_t snps, size_t cw, size_t tw, int order)
+{
+    const Shape sh = shape_of(snps, cw, tw);
+    const size_t M = (size_t)std::max(order - 1, 1);
+    return align_up(snps * 3 * (cw + tw) * 8) + align_up((size_t)sh.n_pad * 24) + align_up((M + 1) * (snps + 1) * 8);
+}
+
+fiu_dataset *stage_dataset(const uint64_t *words, size_t snps, size_t cw, size_t tw, uint32_t cases,
+                           uint32_t ctrls, int order, Arena *arena, Arena *pinned, StagedUpload &up)
+{
+    if (!words && snps) fail(FIU_ERR_INVALID, "words is NULL");
+    if (snps && (cw == 0 || tw == 0) && (cases || ctrls)) fail(FIU_ERR_INVALID, "zero words per row");
+    if ((uint64_t)cases > cw * 64 || (uint64_t)ctrls > tw * 64)
+        fail(FIU_ERR_INVALID, "more samples than bits per row");
+    if (snps > (size_t)INT32_MAX - 64) fail(FIU_ERR_UNSUPPORTED, "more SNPs than INT_MAX");
+    const Shape sh = shape_of(snps, cw, tw);
+    const int M = std::max(order - 1, 1);
+    up = StagedUpload();
+    up.n_words = snps * 3 * (cw + tw);
+    up.off_singles = align_up(up.n_words * 8);
+    up.off_binom = up.off_singles + align_up((size_t)sh.n_pad * 24);
+    up.binom_n = (size_t)(M + 1) * (snps + 1);
+    up.host_bytes = up.off_binom + align_up(up.binom_n * 8);
+    up.host = (char *)pinned->alloc(up.host_bytes);
+    if (!up.host) return nullptr;
+    auto d = std::make_unique<fiu_dataset>();
+    d->device = current_device();
+    d->use_arena(arena);
+    d->snps = snps;
+    d->cw = cw;
+    d->tw = tw;
+    d->cases = cases;
+    d->ctrls = ctrls;
+    d->own_stream = false;
+    try {
+        d->bits.alloc(sh.total);
+        d->singles.alloc((size_t)sh.n_pad * 6);
+        d->binom.alloc(up.binom_n);
+        up.staging_d = (uint64_t *)arena->alloc(std::max<size_t>(up.n_words, 1) * 8);
+        if (!up.staging_d) fail(FIU_ERR_NOMEM, "no room for the upload staging");
+    } catch (const Failure &) {
+        pinned->release(up.host);
+        up.host = nullptr;
+        return nullptr;  // the dataset's buffers go back to the arena with it
+    }
+    // host side of the preparation
+    std::memcpy(up.host, words, up.n_words * 8);
+    uint32_t *sg = (uint32_t *)(up.host + up.off_singles);
+    std::memset(sg, 0, (size_t)sh.n_pad * 24);
+    bool uniform = false;
+    uint32_t total[2] = {0, 0};
+    host_singles(words, snps, cw, tw, sg, uniform, total);
+    uint64_t *bn = (uint64_t *)(up.host + up.off_binom);
+    for (int j = 0; j <= M; ++j)
+        for (uint64_t x = 0; x <= snps; ++x) bn[j * (snps + 1) + x] = binom_sat(x, (uint64_t)j);
+    d->binom_m = M;
+    DevData &D = d->dev;
+    D.bits = d->bits.p;
+    D.class_off[0] = 0;
+    D.class_off[1] = sh.off1;
+    D.quads[0] = sh.q0;
+    D.quads[1] = sh.q1;
+    D.n = sh.n;
+    D.n_pad = sh.n_pad;
+    D.cases = cases;
+    D.ctrls = ctrls;
+    D.singles = d->singles.p;
+    D.uniform = uniform ? 1 : 0;
+    D.total[0] = total[0];
+    D.total[1] = total[1];
+    if (order == 3 && uniform) {
+        d->pair.arena = arena;
+        if (pair_table_bytes(snps) <= arena->largest_free()) {
+            try {
+                d->pair.alloc(snps * (snps - 1) / 2 * 8);
+                up.pair = true;
+                d->pair_ready = true;   // in stream order, before any scan of it
+            } catch (const Failure &) {
+                up.pair = false;        // no room: the scans fall back to the direct kind
+            }
+        }
+    }
+    return d.release();
+}
+
+void enqueue_upload(fiu_dataset *d, StagedUpload &up, hipStream_t s)
+{
+    const Shape sh = shape_of(d->snps, d->cw, d->tw);
+    if (up.n_words)
+        check(hipMemcpyAsync(up.staging_d, up.host, up.n_words * 8, hipMemcpyHostToDevice, s), "upload");
+    check(launch_transpose(up.staging_d, d->bits.p, sh.n, sh.n_pad, 0, d->cw, d->tw, sh.q0, 0, s), "transpose");
+    check(launch_transpose(up.staging_d, d->bits.p, sh.n, sh.n_pad, 1, d->cw, d->tw, sh.q1, sh.off1, s), "transpose");
+    check(hipMemcpyAsync(d->singles.p, up.host + up.off_singles, (size_t)sh.n_pad * 24, hipMemcpyHostToDevice, s),
+          "row counts");
+    check(hipMemcpyAsync(d->binom.p, up.host + up.off_binom, up.binom_n * 8, hipMemcpyHostToDevice, s), "binomials");
+    if (up.pair) {
+        ScanParams P{};
+        P.j0 = 0;
+        P.j1 = (int64_t)d->snps;
+        P.step = 1;
+        P.offset = 0;
+        P.total_prefixes = d->snps;
+        P.binom = d->binom.p;
+        check(launch_pair_table(d->dev, P, d->pair.p, s), "pair table launch");
+    }
+    d->arena->release(up.staging_d);
+    up.staging_d = nullptr;
+}
+
 void mi_constants(uint32_t cases, uint32_t ctrls, ScanParams &P)
 {
     const double n = (double)cases + (double)ctrls;

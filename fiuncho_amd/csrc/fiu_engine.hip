@@ -43,14 +43,17 @@ namespace fiu {
 // ---------------------------------------------------------------------------
 // Arena
 // ---------------------------------------------------------------------------
-Arena::Arena(size_t bytes, int device)
+Arena::Arena(size_t bytes, int device, bool pinned_host) : pinned_(pinned_host)
 {
     bytes = (std::max<size_t>(bytes, kAlign) + kAlign - 1) / kAlign * kAlign;
     check(hipSetDevice(device), "hipSetDevice");
-    if (hipMalloc(&base_, bytes) != hipSuccess) {
+    const hipError_t e = pinned_ ? hipHostMalloc((void **)&base_, bytes, hipHostMallocDefault)
+                                 : hipMalloc((void **)&base_, bytes);
+    if (e != hipSuccess) {
         base_ = nullptr;
         (void)hipGetLastError();
-        fail(FIU_ERR_NOMEM, "scan server: cannot allocate a memory budget of " + std::to_string(bytes) + " bytes");
+        fail(FIU_ERR_NOMEM, std::string("scan server: cannot allocate ") + (pinned_ ? "pinned host" : "device") +
+                                " memory of " + std::to_string(bytes) + " bytes");
     }
     size_ = bytes;
     free_[0] = bytes;
@@ -58,7 +61,10 @@ Arena::Arena(size_t bytes, int device)
 
 Arena::~Arena()
 {
-    if (base_) (void)hipFree(base_);
+    if (base_) {
+        if (pinned_) (void)hipHostFree(base_);
+        else (void)hipFree(base_);
+    }
 }
 
 void *Arena::alloc(size_t bytes)
@@ -131,6 +137,8 @@ struct Job {
     std::vector<uint64_t> words;          // server: host tables until upload
     uint64_t snps = 0, cw = 0, tw = 0;
     uint32_t cases = 0, ctrls = 0;
+    bool staged = false;                  // device work of the upload still to enqueue
+    StagedUpload up;
     int order = 0;
     int32_t step = 1, offset = 0;
     uint32_t outputs = 0;
@@ -153,7 +161,7 @@ struct Job {
 };
 
 struct EngineConfig {
-    int max_active = 64;
+    int max_active = 128;
     double step_ms = 6.0;
     uint32_t cand_cap = 1u << 16;
 };
@@ -180,6 +188,13 @@ class Engine {
     bool pump();
     bool idle();
     void set_on_done(std::function<void(Job *)> f) { on_done_ = std::move(f); }
+    // Pinned staging of staged uploads is released here once the step after
+    // its copies has completed; `freed` runs after each such release.
+    void set_pinned(Arena *pinned, std::function<void()> freed)
+    {
+        pinned_ = pinned;
+        on_pinned_freed_ = std::move(freed);
+    }
     fiu_batch_stats stats() const { return stats_; }
     hipStream_t prep_stream() const { return prep_; }
     // Plan constants of a job (kind, bits, prefixes, unit shape); throws on
@@ -225,6 +240,9 @@ class Engine {
     double rate_ = 8.0e8;    // VALU wave-instructions per ms (calibrated)
     fiu_batch_stats stats_{};
     std::function<void(Job *)> on_done_;
+    Arena *pinned_ = nullptr;
+    std::function<void()> on_pinned_freed_;
+    std::vector<std::pair<uint64_t, void *>> deferred_;   // (step index, pinned block)
 };
 
 Engine::Engine(int device, const EngineConfig &cfg, Clock::time_point epoch) : device_(device), cfg_(cfg), epoch_(epoch)
@@ -351,6 +369,15 @@ void Engine::activate(Job *j, int slot)
     j->slot = slot;
     slots_[slot] = j;
     active_.push_back(j);
+    if (j->staged) {
+        // copies, transposes and the pair table go ahead of the job's first
+        // step on the same stream; the pinned source is released once the
+        // next step (launched after them) has completed
+        enqueue_upload(j->ds, j->up, stream_);
+        deferred_.emplace_back(step_no_, j->up.host);
+        j->up.host = nullptr;
+        j->staged = false;
+    }
     BatchJob &B = jobs_h_[slot];
     std::memset(&B, 0, sizeof(B));
     fiu_dataset *d = j->ds;
@@ -537,6 +564,20 @@ void Engine::complete_step(int parity)
     for (Job *j : R.jobs)
         if (j->slot >= 0 && j->inflight == 0 && j->next >= j->n_pref && j->redo.empty()) finish(j, S[j->slot].n_top);
     R.valid = false;
+    // pinned staging whose copies were enqueued before this step is free
+    const uint64_t finished = done_no_;
+    bool freed = false;
+    for (size_t i = 0; i < deferred_.size();) {
+        if (deferred_[i].first <= finished) {
+            pinned_->release(deferred_[i].second);
+            deferred_[i] = deferred_.back();
+            deferred_.pop_back();
+            freed = true;
+        } else {
+            ++i;
+        }
+    }
+    if (freed && on_pinned_freed_) on_pinned_freed_();
     ++done_no_;
 }
 
@@ -586,10 +627,14 @@ class Server {
     void ingest_main();
     void sched_main();
     size_t footprint(const Job *j) const;
+    size_t host_footprint(const Job *j) const;
+    bool fits(const Job *j) const;
+    void release_staging(Job *j);
 
     int device_;
     Clock::time_point epoch_;
     std::unique_ptr<Arena> arena_;
+    std::unique_ptr<Arena> pinned_;   // staging of host tables on their way to the device
     std::unique_ptr<Engine> eng_;
     hipStream_t ingest_stream_ = nullptr;
     std::mutex mu_;
@@ -598,6 +643,7 @@ class Server {
     std::unordered_map<uint64_t, std::unique_ptr<Job>> jobs_;  // every job not yet collected
     uint64_t next_id_ = 1;
     uint64_t uploading_ = 0, on_device_ = 0, completed_ = 0;
+    uint64_t mem_gen_ = 0;   // bumped whenever device or pinned memory is released
     bool stop_ = false;
     std::string fatal_;
     std::thread ingest_, sched_;
@@ -612,11 +658,21 @@ Server::Server(int device, const EngineConfig &ec, size_t budget) : device_(devi
         budget = std::min<size_t>(free_b / 4, 8ull << 30);
     }
     arena_ = std::make_unique<Arena>(budget, device_);
+    pinned_ = std::make_unique<Arena>(std::min<size_t>(budget, 1ull << 30), device_, true);
     eng_ = std::make_unique<Engine>(device_, ec, epoch_);
     check(hipStreamCreateWithFlags(&ingest_stream_, hipStreamNonBlocking), "hipStreamCreate");
+    eng_->set_pinned(pinned_.get(), [this] {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            ++mem_gen_;
+        }
+        cv_ingest_.notify_all();
+    });
     eng_->set_on_done([this](Job *j) {
+        release_staging(j);
         j->owned.reset();  // device tables back to the arena
         std::lock_guard<std::mutex> lk(mu_);
+        ++mem_gen_;
         j->done = true;
         --on_device_;
         ++completed_;
@@ -649,6 +705,28 @@ size_t Server::footprint(const Job *j) const
     b += (size_t)(j->order + 1) * (j->snps + 1) * 8 + Arena::kAlign;   // binomial table
     if (j->order == 3) b += pair_table_bytes(j->snps) + Arena::kAlign;
     return b;
+}
+
+size_t Server::host_footprint(const Job *j) const
+{
+    if (j->ds) return 0;
+    const size_t b = staged_host_bytes(j->snps, j->cw, j->tw, j->order);
+    return b <= pinned_->capacity() ? b : 0;   // larger tables take the synchronous path
+}
+
+bool Server::fits(const Job *j) const
+{
+    return footprint(j) <= arena_->capacity() - arena_->used() &&
+           host_footprint(j) <= pinned_->capacity() - pinned_->used();
+}
+
+void Server::release_staging(Job *j)
+{
+    if (!j->staged) return;
+    if (j->up.host) pinned_->release(j->up.host);
+    if (j->up.staging_d) arena_->release(j->up.staging_d);
+    j->up = StagedUpload();
+    j->staged = false;
 }
 
 uint64_t Server::submit(const fiu_request *req)
@@ -694,6 +772,7 @@ void Server::ingest_main()
 {
     (void)hipSetDevice(device_);
     current_device() = device_;
+    uint64_t blocked_gen = ~0ull;   // memory generation at the last failed allocation
     for (;;) {
         Job *j = nullptr;
         {
@@ -702,27 +781,42 @@ void Server::ingest_main()
             cv_ingest_.wait(lk, [&] {
                 if (stop_) return true;
                 if (queue_.empty()) return false;
-                const size_t need = footprint(queue_.front());
-                return need <= arena_->capacity() - arena_->used() || on_device_ + uploading_ == 0;
+                if (on_device_ + uploading_ == 0) return true;
+                return mem_gen_ != blocked_gen && fits(queue_.front());
             });
             if (stop_) return;
             j = queue_.front();
             queue_.pop_front();
             ++uploading_;
         }
-        bool ok = true;
+        bool ok = true, no_room = false;
         fiu_status st = FIU_OK;
         std::string msg;
         try {
-            if (!j->ds) {
-                j->owned.reset(create_dataset(j->words.data(), j->snps, j->cw, j->tw, j->cases, j->ctrls,
-                                              arena_.get(), ingest_stream_, false));
-                j->ds = j->owned.get();
-                std::vector<uint64_t>().swap(j->words);
+            if (!j->ds && host_footprint(j) > 0) {
+                // staged: host preparation now, device work queued by the scheduler
+                fiu_dataset *d = stage_dataset(j->words.data(), j->snps, j->cw, j->tw, j->cases, j->ctrls, j->order,
+                                               arena_.get(), pinned_.get(), j->up);
+                if (!d) {
+                    no_room = true;
+                } else {
+                    j->owned.reset(d);
+                    j->ds = d;
+                    j->staged = true;
+                    std::vector<uint64_t>().swap(j->words);
+                    Engine::plan_job(j);
+                }
+            } else {
+                if (!j->ds) {
+                    j->owned.reset(create_dataset(j->words.data(), j->snps, j->cw, j->tw, j->cases, j->ctrls,
+                                                  arena_.get(), ingest_stream_, false));
+                    j->ds = j->owned.get();
+                    std::vector<uint64_t>().swap(j->words);
+                }
+                ensure_binom(j->ds, j->order - 1, ingest_stream_);
+                if (j->order == 3 && j->ds->dev.uniform) (void)ensure_pair_table(j->ds, ingest_stream_);
+                Engine::plan_job(j);
             }
-            ensure_binom(j->ds, j->order - 1, ingest_stream_);
-            if (j->order == 3 && j->ds->dev.uniform) (void)ensure_pair_table(j->ds, ingest_stream_);
-            Engine::plan_job(j);
         } catch (const Failure &e) {
             ok = false;
             st = e.status;
@@ -734,14 +828,30 @@ void Server::ingest_main()
         }
         std::unique_lock<std::mutex> lk(mu_);
         --uploading_;
-        if (!ok && st == FIU_ERR_NOMEM && on_device_ > 0) {
-            // fragmented budget: wait for a running scan to retire, then retry
-            j->owned.reset();
-            j->ds = nullptr;
+        if (no_room || (!ok && st == FIU_ERR_NOMEM && on_device_ > 0)) {
+            // fragmented budget: wait until memory is released, then retry
+            release_staging(j);
+            if (!j->words.empty() || j->owned) {
+                j->owned.reset();
+                j->ds = nullptr;
+            }
+            blocked_gen = mem_gen_;
             queue_.push_front(j);
+            if (on_device_ + uploading_ == 0 && no_room) {
+                // nothing will free memory: give up on this request
+                queue_.pop_front();
+                j->status = FIU_ERR_REJECTED;
+                j->error = "request does not fit in the server's memory budget";
+                j->done = true;
+                j->rs.done_ms = ms_since(epoch_);
+                j->rs.latency_ms = j->rs.done_ms - j->rs.submit_ms;
+                ++completed_;
+                cv_done_.notify_all();
+            }
             continue;
         }
         if (!ok) {
+            release_staging(j);
             j->owned.reset();
             j->status = st == FIU_ERR_NOMEM ? FIU_ERR_REJECTED : st;
             j->error = msg;
@@ -852,7 +962,7 @@ void fiu_batch_config_init(fiu_batch_config *cfg)
     if (!cfg) return;
     std::memset(cfg, 0, sizeof(*cfg));
     cfg->struct_size = sizeof(fiu_batch_config);
-    cfg->max_active = 64;
+    cfg->max_active = 128;
     cfg->step_ms = 6.0;
     cfg->candidate_cap = 1u << 16;
 }

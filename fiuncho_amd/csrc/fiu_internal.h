@@ -56,14 +56,14 @@ template <class F> fiu_status guard(F &&f)
     }
 }
 
-// First-fit sub-allocator over one device allocation: the scan server takes
-// its whole memory budget once and carves tables and buffers out of it, so
-// that admitting and retiring scans never calls hipMalloc / hipFree (which
-// can wait for the device).  Thread-safe.
+// First-fit sub-allocator over one device allocation (or one pinned host
+// allocation): the scan server takes its whole memory budget once and carves
+// tables and buffers out of it, so that admitting and retiring scans never
+// calls hipMalloc / hipFree (which can wait for the device).  Thread-safe.
 class Arena {
   public:
     static constexpr size_t kAlign = 256;
-    Arena(size_t bytes, int device);
+    Arena(size_t bytes, int device, bool pinned_host = false);
     ~Arena();
     Arena(const Arena &) = delete;
     Arena &operator=(const Arena &) = delete;
@@ -75,6 +75,7 @@ class Arena {
 
   private:
     char *base_ = nullptr;
+    bool pinned_ = false;
     size_t size_ = 0;
     size_t used_ = 0;
     std::map<size_t, size_t> free_;   // offset -> length
@@ -176,6 +177,32 @@ fiu_dataset *create_dataset(const uint64_t *words, size_t snps, size_t cw, size_
 // the transient staging its upload needs.
 size_t dataset_bytes(size_t snps, size_t cw, size_t tw);
 size_t staging_bytes(size_t snps, size_t cw, size_t tw);
+
+// Staged upload (scan server): the host does every part of the preparation
+// that needs no device (row counts, coverage check, binomials) and copies the
+// tables into pinned memory; enqueue_upload() later puts the copies, the
+// transposes and the pair table on one stream with no host synchronisation,
+// so that preparing a dataset never waits for the work already queued.
+struct StagedUpload {
+    char *host = nullptr;            // pinned: [words][row counts][binomials]
+    size_t host_bytes = 0;
+    size_t n_words = 0, off_singles = 0, off_binom = 0, binom_n = 0;
+    uint64_t *staging_d = nullptr;   // device copy of the words (arena)
+    bool pair = false;               // build the order-3 pair table
+};
+
+// Host bytes (pinned) a staged upload of this shape needs for `order`.
+size_t staged_host_bytes(size_t snps, size_t cw, size_t tw, int order);
+
+// Allocates the dataset's device buffers (and the device staging) from
+// `arena` and its pinned staging from `pinned`; returns null when either has
+// no room.  Throws on invalid shapes.
+fiu_dataset *stage_dataset(const uint64_t *words, size_t snps, size_t cw, size_t tw, uint32_t cases,
+                           uint32_t ctrls, int order, Arena *arena, Arena *pinned, StagedUpload &up);
+// Copies, transposes and builds the pair table on `s`; the device staging is
+// released at once (later users of that memory are ordered after it on `s`),
+// the pinned staging must be kept until the work completes.
+void enqueue_upload(fiu_dataset *d, StagedUpload &up, hipStream_t s);
 
 // binom[j*(n+1)+x] = C(x, j) for j <= m, on the device.
 void ensure_binom(fiu_dataset *d, int m, hipStream_t s);

@@ -186,7 +186,8 @@ fiu_status fiu_mi_batch(const uint32_t *tables, size_t n, uint16_t order,
 
 typedef struct {
     uint32_t struct_size;    /* sizeof(fiu_batch_config) */
-    uint32_t max_active;     /* searches on the device at once (<= 1024) */
+    uint32_t max_active;     /* searches on the device at once (default 128,
+                                at most 1024) */
     uint64_t memory_budget;  /* server: device bytes for the tables it
                                 uploads, 0 = a quarter of the free
                                 memory, at most 8 GiB */
