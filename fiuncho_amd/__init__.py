@@ -28,12 +28,12 @@ from typing import List, Sequence
 import numpy as np
 
 from . import _lib
-from ._lib import ALGO_AUTO, ALGO_DIRECT, ALGO_MARGINAL, FiuError, check, lib
+from ._lib import ALGO_AUTO, ALGO_DIRECT, ALGO_MARGINAL, ALGO_MFMA, FiuError, check, lib
 
 __all__ = ["Dataset", "Distribution", "Result", "GpuSearch", "Config", "combine_batch",
            "contingency_batch", "mi_batch", "ct_size", "synth_spec", "synth_generate",
            "search_batch", "batch_config", "Server",
-           "FiuError", "ALGO_AUTO", "ALGO_DIRECT", "ALGO_MARGINAL", "version"]
+           "FiuError", "ALGO_AUTO", "ALGO_DIRECT", "ALGO_MARGINAL", "ALGO_MFMA", "version"]
 
 
 def version() -> str:

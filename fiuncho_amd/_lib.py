@@ -22,7 +22,7 @@ FIU_ERR_REJECTED = 6
 FIU_ERR_CANCELLED = 7
 FIU_ERR_PENDING = 8
 
-ALGO_AUTO, ALGO_DIRECT, ALGO_MARGINAL = 0, 1, 2
+ALGO_AUTO, ALGO_DIRECT, ALGO_MARGINAL, ALGO_MFMA = 0, 1, 2, 3
 
 # Every symbol include/fiuncho_amd.h declares (checked by the CPU tests).
 EXPORTS = (

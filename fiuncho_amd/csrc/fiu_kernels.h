@@ -9,6 +9,12 @@ hipError_t launch_scan_direct(int order, const DevData &D, const ScanParams &P, 
 hipError_t launch_scan_marginal(int order, const DevData &D, const ScanParams &P, TopState *st, hipStream_t s);
 hipError_t launch_tau_from_hist(const uint32_t *hist, uint32_t N, float delta, TopState *st, hipStream_t s);
 hipError_t launch_pair_table(const DevData &D, const ScanParams &P, uint32_t *pair_out, hipStream_t s);
+// Order 2 on the matrix cores: the pair triangle of T x T 64-SNP tiles is cut
+// into super-blocks of 8 x 8 tiles; a launch scores super-blocks [s0, s1)
+// into the top-N state.
+int64_t mfma2_superblocks(int T);
+hipError_t launch_mfma2(const DevData &D, const ScanParams &P, TopState *st, int64_t s0, int64_t s1, int T,
+                        hipStream_t s);
 hipError_t launch_merge(TopState *st, Cand *top, const Cand *cand, uint32_t N, uint32_t cap, uint32_t chunk_id,
                         uint32_t *redo, hipStream_t s);
 hipError_t launch_transpose(const uint64_t *src, uint4 *dst, int n, int n_pad, int cls, uint64_t cw, uint64_t tw,

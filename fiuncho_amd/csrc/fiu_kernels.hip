@@ -787,6 +787,160 @@ k_scan3m(const DevData D, const ScanParams P, TopState *st)
 }
 
 // ---------------------------------------------------------------------------
+// Order 2 on the matrix cores.  With one int8 {0,1} indicator row per SNP and
+// genotype (I_g[s][k] = 1 iff sample k of the class has genotype g at SNP s),
+// the four marginal pair counts are dot products over the samples:
+// f(x, y)[a][b] = sum_k I_x[a][k] I_y[b][k] -- a symmetric int8 GEMM per class
+// (v_mfma_i32_32x32x32_i8, exact int32 accumulation); the other five cells
+// follow from the singles and the class total exactly as in the popcount scan.
+// ---------------------------------------------------------------------------
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
+
+constexpr int kMfmaTile = 64;       // SNPs per tile side (per workgroup)
+constexpr int kMfmaK = 128;         // samples per stage: one 128-bit quad per row
+constexpr int kMfmaRow = 144;       // LDS row pitch in bytes (128 + 16: the 16 rows a
+                                    // ds_read_b128 phase touches hit distinct banks)
+constexpr int kMfmaSuper = 8;       // tiles per side of the super-blocks run per XCD
+
+// Four bits of v (from bit `lo`) as four bytes 0/1: n * 0x00204081 puts bit i
+// of the nibble n at bit 8i (the shifted copies do not overlap), the mask
+// keeps those bits.
+__device__ __forceinline__ uint32_t nibble_bytes(uint32_t v, int lo)
+{
+    return ((v >> lo & 15u) * 0x00204081u) & 0x01010101u;
+}
+
+// 128 bits of a quad as 128 indicator bytes in one LDS row: bit j of 32-bit
+// word e becomes byte 32 e + j.  Rows of both operands use the same order, so
+// dot products over the samples are unchanged.
+__device__ __forceinline__ void expand_quad(const uint4 &q, int8_t *row)
+{
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const uint32_t v = comp(q, e);
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+            *(uint4 *)(row + 32 * e + 16 * h) = make_uint4(nibble_bytes(v, 16 * h), nibble_bytes(v, 16 * h + 4),
+                                                         nibble_bytes(v, 16 * h + 8), nibble_bytes(v, 16 * h + 12));
+    }
+}
+
+// Tile u of the upper triangle of the T x T grid of 64-SNP tiles, row-major
+// (row ta holds tiles ta..T-1).
+__device__ __forceinline__ void tri_tile(int64_t u, int T, int &ta, int &tb)
+{
+    // ta = largest r with r*T - r(r-1)/2 <= u
+    const double tt = (double)T + 0.5;
+    int r = (int)(tt - sqrt(tt * tt - 2.0 * (double)u));
+    r = max(0, min(r, T - 1));
+    while (r > 0 && (int64_t)r * T - (int64_t)r * (r - 1) / 2 > u) --r;
+    while (r + 1 < T && (int64_t)(r + 1) * T - (int64_t)(r + 1) * r / 2 <= u) ++r;
+    ta = r;
+    tb = (int)(u - ((int64_t)r * T - (int64_t)r * (r - 1) / 2)) + r;
+}
+
+// One workgroup = one 64 x 64 tile (4 waves, 32 x 32 SNPs each).  The
+// launch covers super-blocks [s0, s1) of the triangle of S x S super-blocks
+// (8 x 8 tiles each): block v runs on XCD v % 8, and the 64 blocks an XCD
+// holds at once are one super-block, whose 8 + 8 row panels they share in L2.
+// Each stage loads one 128-bit quad of the bit tables per row (A: 64 SNPs x
+// genotypes 0/1, B likewise) and expands it to int8 in LDS.
+__global__ void __launch_bounds__(kBlock)
+k_mfma2(const DevData D, const ScanParams P, TopState *st, int64_t s0, int64_t s1, int T)
+{
+    __shared__ __attribute__((aligned(16))) int8_t lds[2][2][2 * kMfmaTile][kMfmaRow];  // [buf][A/B][row][k]
+    const int S = (T + kMfmaSuper - 1) / kMfmaSuper;
+    const int64_t k = blockIdx.x >> 3, su = s0 + (k >> 6) * 8 + (blockIdx.x & 7);
+    if (su >= s1) return;
+    int sa, sc;
+    tri_tile(su, S, sa, sc);
+    const int slot = (int)(k & 63);
+    const int ta = sa * kMfmaSuper + (slot >> 3), tb = sc * kMfmaSuper + (slot & 7);
+    if (ta > tb || tb >= T) return;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6), wa = w >> 1, wb = w & 1;
+    const int np = D.n_pad;
+    const int a0 = ta * kMfmaTile, b0 = tb * kMfmaTile;
+    // this thread's staging row: operand m (A / B), genotype g, SNP
+    const int m_ld = tid >> 7, row_ld = tid & 127, g_ld = row_ld >> 6;
+    const int snp_ld = (m_ld ? b0 : a0) + (row_ld & 63);
+    v16i acc[2][2][2];
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+        for (int g = 0; g < 2; ++g)
+#pragma unroll
+            for (int h = 0; h < 2; ++h) acc[c][g][h] = (v16i){0};
+#pragma unroll
+    for (int cls = 0; cls < 2; ++cls) {
+        const int iters = D.quads[cls];
+        const uint4 *src = D.bits + D.class_off[cls] + (uint64_t)g_ld * np + snp_ld;   // quad q at + q * 3 * np
+        uint4 st_reg = src[0];
+        expand_quad(st_reg, lds[0][m_ld][row_ld]);
+        __syncthreads();
+        for (int it = 0; it < iters; ++it) {
+            const int buf = it & 1;
+            if (it + 1 < iters) st_reg = src[(uint64_t)(it + 1) * 3 * np];
+#pragma unroll
+            for (int s = 0; s < kMfmaK / 32; ++s) {
+                const int kk = s * 32 + (lane >> 5) * 16, r = lane & 31;
+                v4i fa[2], fb[2];
+#pragma unroll
+                for (int g = 0; g < 2; ++g) {
+                    fa[g] = *(const v4i *)&lds[buf][0][g * kMfmaTile + wa * 32 + r][kk];
+                    fb[g] = *(const v4i *)&lds[buf][1][g * kMfmaTile + wb * 32 + r][kk];
+                }
+#pragma unroll
+                for (int g = 0; g < 2; ++g)
+#pragma unroll
+                    for (int h = 0; h < 2; ++h)
+                        acc[cls][g][h] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[g], fb[h], acc[cls][g][h], 0, 0, 0);
+            }
+            if (it + 1 < iters) expand_quad(st_reg, lds[buf ^ 1][m_ld][row_ld]);
+            __syncthreads();
+        }
+    }
+    // epilogue: lane holds column b = b0 + wb*32 + lane%32 and rows
+    // a = a0 + wa*32 + 8r + 4(lane/32) + q, r, q < 4
+    const float tau = st->tau;
+    const uint64_t tau_combo = st->tau_combo;
+    const int b = b0 + wb * 32 + (lane & 31);
+    const bool b_ok = b < D.n;
+    uint32_t sb[2][2] = {{0, 0}, {0, 0}};
+    if (b_ok) {
+        const uint32_t *sp = D.singles + (size_t)b * 6;
+        sb[0][0] = sp[0], sb[0][1] = sp[1], sb[1][0] = sp[3], sb[1][1] = sp[4];
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int i = r * 4 + q;
+            const int a = a0 + wa * 32 + 8 * r + 4 * (lane >> 5) + q;
+            const int64_t rel = (int64_t)a - P.offset;
+            const bool valid = b_ok && a < b && rel >= 0 && rel % P.step == 0;
+            const uint32_t *sa = D.singles + (size_t)a * 6;
+            uint32_t m0[9], m1[9];
+#pragma unroll
+            for (int cls = 0; cls < 2; ++cls) {
+                uint32_t *m = cls ? m1 : m0;
+                m[0] = (uint32_t)acc[cls][0][0][i];
+                m[1] = (uint32_t)acc[cls][0][1][i];
+                m[3] = (uint32_t)acc[cls][1][0][i];
+                m[4] = (uint32_t)acc[cls][1][1][i];
+                m[2] = sa[cls * 3 + 0];
+                m[5] = sa[cls * 3 + 1];
+                m[6] = sb[cls][0];
+                m[7] = sb[cls][1];
+                m[8] = D.total[cls];
+                mobius2(m);
+            }
+            score_and_push<9>(valid, m0, m1, ((uint64_t)a << P.combo_bits) | (uint64_t)b, tau, tau_combo, P, st);
+        }
+}
+
+// ---------------------------------------------------------------------------
 // Top-N merge (one workgroup): current list + candidates -> best N, unique.
 // ---------------------------------------------------------------------------
 // Body shared by the single-scan and the batched merge (1024 threads).
@@ -1175,6 +1329,22 @@ hipError_t launch_batch(const BatchJob *jobs, const BatchSeg *segs0, const Batch
 hipError_t launch_tau_from_hist(const uint32_t *hist, uint32_t N, float delta, TopState *st, hipStream_t s)
 {
     hipLaunchKernelGGL(k_tau_from_hist, dim3(1), dim3(64), 0, s, hist, N, delta, st);
+    return hipGetLastError();
+}
+
+int64_t mfma2_superblocks(int T)
+{
+    const int64_t S = (T + kMfmaSuper - 1) / kMfmaSuper;
+    return S * (S + 1) / 2;
+}
+
+hipError_t launch_mfma2(const DevData &D, const ScanParams &P, TopState *st, int64_t s0, int64_t s1, int T,
+                        hipStream_t s)
+{
+    if (s1 <= s0) return hipSuccess;
+    // 64 blocks per super-block, 8 super-blocks (one per XCD) per round
+    const int64_t rounds = (s1 - s0 + 7) / 8;
+    hipLaunchKernelGGL(k_mfma2, dim3((unsigned)(rounds * 8 * 64)), dim3(kBlock), 0, s, D, P, st, s0, s1, T);
     return hipGetLastError();
 }
 

@@ -98,9 +98,12 @@ fiu_status fiu_dataset_destroy(fiu_dataset *ds);
 typedef enum {
     FIU_ALGO_AUTO = 0,     /* fastest exact path the dataset allows */
     FIU_ALGO_DIRECT = 1,   /* AND+popcount of all 3^k cells (the reference's) */
-    FIU_ALGO_MARGINAL = 2  /* 2^k cells + inclusion-exclusion from exact
+    FIU_ALGO_MARGINAL = 2, /* 2^k cells + inclusion-exclusion from exact
                               lower-order counts; needs every SNP to cover the
                               same samples (checked at upload) */
+    FIU_ALGO_MFMA = 3      /* order 2 only: the marginal pair counts as int8
+                              indicator dot products on the matrix cores
+                              (int32 accumulation, exact); same requirement */
 } fiu_algorithm;
 
 /* fiu_config.flags */

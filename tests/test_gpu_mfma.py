@@ -1,0 +1,77 @@
+"""GPU: the order-2 matrix-core scan (FIU_ALGO_MFMA).
+
+The marginal pair counts come from int8 indicator dot products accumulated in
+int32 (exact); everything downstream is the popcount scan's.  Bar: the same
+top-N (combinations, order and float MI bit for bit) as the oracle's FP64
+ranking and as the popcount scan.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import fiuncho_amd as fa
+import oracle as O
+
+from .test_gpu_parity import synth
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("snps,cases,ctrls,outputs,seed", [
+    (70, 129, 63, 40, 21),       # one partial tile, ragged classes
+    (128, 512, 512, 100, 22),    # two full tiles per side
+    (200, 1000, 777, 300, 26),   # several tiles, K of 8 and 7 stages
+    (65, 40, 2000, 2080, 27),    # every pair, one SNP in the last tile
+])
+def test_mfma_vs_oracle(snps, cases, ctrls, outputs, seed):
+    print(f"case: mfma {snps}x{cases}+{ctrls}", flush=True)
+    d, od = synth(snps, cases, ctrls, seed)
+    oc, om, _ = O.search(od, 2, outputs, threads=4, mi_mode=2)
+    s = fa.GpuSearch(algorithm=fa.ALGO_MFMA)
+    c, m = s.run_arrays(d, 2, fa.Distribution(snps, 1), outputs)
+    assert s.last_stats.algorithm == fa.ALGO_MFMA
+    assert c.tolist() == oc.tolist() and np.array_equal(m, om)
+    assert s.last_stats.combinations == math.comb(snps, 2)
+
+
+def test_mfma_matches_popcount_scan():
+    print("case: mfma vs popcount 1500 SNPs", flush=True)
+    d, _ = synth(1500, 3000, 2500, 31, planted=(100, 1200))
+    ref = fa.GpuSearch(algorithm=fa.ALGO_MARGINAL).run_arrays(d, 2, fa.Distribution(1500, 1), 500)
+    c, m = fa.GpuSearch(algorithm=fa.ALGO_MFMA).run_arrays(d, 2, fa.Distribution(1500, 1), 500)
+    assert c.tolist() == ref[0].tolist() and np.array_equal(m, ref[1])
+    assert c[0].tolist() == [100, 1200]
+
+
+def test_mfma_distribution_and_chunks():
+    # distribution shares (masked in the epilogue) and tiny tile chunks with
+    # the candidate re-scan
+    d, od = synth(90, 300, 280, 41)   # 4005 pairs: every one fits in the top-4096
+    full = fa.GpuSearch(algorithm=fa.ALGO_MARGINAL).run_arrays(d, 2, fa.Distribution(90, 1), 4096)
+    ref = {tuple(c): v for c, v in zip(full[0].tolist(), full[1])}
+    seen = {}
+    for off in range(3):
+        c, m = fa.GpuSearch(algorithm=fa.ALGO_MFMA).run_arrays(d, 2, fa.Distribution(90, 1).layer(3, off), 4096)
+        for cc, mm in zip(c.tolist(), m):
+            assert cc[0] % 3 == off and tuple(cc) not in seen
+            seen[tuple(cc)] = mm
+    assert seen == ref
+    s = fa.GpuSearch(algorithm=fa.ALGO_MFMA, chunk_combos=4096 * 2, candidate_cap=256, sample_threshold=False)
+    c, m = s.run_arrays(d, 2, fa.Distribution(90, 1), 64)
+    oc, om, _ = O.search(od, 2, 64, threads=2, mi_mode=2)
+    assert c.tolist() == oc.tolist() and np.array_equal(m, om)
+    assert s.last_stats.redo_chunks > 0
+
+
+def test_mfma_refuses_what_it_cannot_do():
+    d, _ = synth(40, 100, 100, 51)
+    with pytest.raises(fa.FiuError):
+        fa.GpuSearch(algorithm=fa.ALGO_MFMA).run(d, 3, fa.Distribution(40, 2), 5)
+    rng = np.random.default_rng(52)
+    geno = rng.integers(0, 3, size=(20, 150)).astype(np.int8)
+    geno[0, 5] = 3   # a sample outside every row of SNP 0
+    od = O.pack(geno, (np.arange(150) >= 80).astype(np.uint8))
+    inc = fa.Dataset.from_words(od.words, 20, od.cases_words, od.ctrls_words, 80, 70)
+    with pytest.raises(fa.FiuError):
+        fa.GpuSearch(algorithm=fa.ALGO_MFMA).run(inc, 2, fa.Distribution(20, 1), 5)

@@ -211,12 +211,15 @@ def test_search_vs_oracle_fp64(snps, cases, ctrls, order, outputs, seed):
     print(f"case: synthetic {snps}x{cases}+{ctrls} k={order}", flush=True)
     d, od = synth(snps, cases, ctrls, seed)
     oc, om, _ = O.search(od, order, outputs, threads=4, mi_mode=2)
-    for algorithm in (fa.ALGO_AUTO, fa.ALGO_DIRECT, fa.ALGO_MARGINAL):
+    algos = (fa.ALGO_AUTO, fa.ALGO_DIRECT, fa.ALGO_MARGINAL) + ((fa.ALGO_MFMA,) if order == 2 else ())
+    for algorithm in algos:
         s = fa.GpuSearch(algorithm=algorithm)
         combos, mi = s.run_arrays(d, order, fa.Distribution(snps, order - 1), outputs)
         used = s.last_stats.algorithm
         if algorithm == fa.ALGO_DIRECT or order > 3:
             assert used == fa.ALGO_DIRECT
+        elif order == 2 and algorithm != fa.ALGO_MARGINAL:
+            assert used == fa.ALGO_MFMA   # the order-2 default
         else:
             assert used == fa.ALGO_MARGINAL
         assert combos.tolist() == oc.tolist(), (algorithm, used)
@@ -310,16 +313,24 @@ def test_small_chunks_and_candidate_overflow():
 
 
 @pytest.mark.parametrize("order", [2, 3])
-def test_sampled_threshold_with_late_signal(order):
-    # the strongest combinations sit in the last prefixes (far from the
-    # sampled ones): the starting threshold must not cut any of them
-    print(f"case: late signal k={order}", flush=True)
-    n = 40
-    spec = fa.synth_spec(n, 300, 300, seed=81, planted=tuple(range(n - order, n)))
+@pytest.mark.parametrize("where", ["first", "last"])
+def test_sampled_threshold_with_planted_signal(order, where):
+    # the threshold sample is the tail of the distribution (at most one
+    # chunk of combinations): with the strongest combinations in the first
+    # prefixes (outside the sample) or in the last ones (inside it), the
+    # starting threshold must cut none of them
+    print(f"case: planted {where} k={order}", flush=True)
+    n, half = 40, (300 if order == 2 else 1500)   # enough samples for the planted one to rank first
+    planted = tuple(range(order)) if where == "first" else tuple(range(n - order, n))
+    spec = fa.synth_spec(n, half, half, seed=81, planted=planted)
     words, geno, cw, tw = fa.synth_generate(spec, words=True, geno=True)
-    d = fa.Dataset.from_words(words, n, cw, tw, 300, 300)
-    od = O.pack(geno, np.r_[np.zeros(300, np.uint8), np.ones(300, np.uint8)])
+    d = fa.Dataset.from_words(words, n, cw, tw, half, half)
+    od = O.pack(geno, np.r_[np.zeros(half, np.uint8), np.ones(half, np.uint8)])
     oc, om, _ = O.search(od, order, 30, threads=2, mi_mode=2)
-    for chunk in (0, 50):
-        c, m = fa.GpuSearch(chunk_combos=chunk).run_arrays(d, order, fa.Distribution(n, order - 1), 30)
-        assert c.tolist() == oc.tolist() and np.array_equal(m, om)
+    assert oc[0].tolist() == list(planted)
+    algos = (fa.ALGO_AUTO, fa.ALGO_MFMA) if order == 2 else (fa.ALGO_AUTO,)
+    for algorithm in algos:
+        for chunk in (0, 50):
+            c, m = fa.GpuSearch(algorithm=algorithm, chunk_combos=chunk).run_arrays(
+                d, order, fa.Distribution(n, order - 1), 30)
+            assert c.tolist() == oc.tolist() and np.array_equal(m, om), (algorithm, chunk)
