@@ -175,6 +175,60 @@ def bench_batch(count=64):
             "steps": st.steps, "device_s": st.device_ms / 1e3, "results_identical": same}
 
 
+MFMA_I8_PEAK = 5.0e15    # dense int8 MFMA ops per second (MI355X)
+MFMA_FP4_PEAK = 10.1e15  # dense fp4 MFMA flops per second (MI355X)
+
+
+def bench_order2(snps=50_000, half=16_000, outputs=1000):
+    """Order-2 exhaustive scan, 50,000 SNPs x 32,000 samples (16,000 /
+    16,000), MAF ~ U(0.05, 0.5) with 20% of SNPs ~ U(0.01, 0.05), top-1,000:
+    the popcount scan against the int8 matrix-core scan, same dataset,
+    identical results required."""
+    import fiuncho_amd as fa
+    spec = fa.synth_spec(snps, half, half, seed=1, low_frac=0.2)
+    d = fa.Dataset.synth(spec)
+    dist = fa.Distribution(snps, 1)
+    out = {}
+    res = {}
+    for name, algo, int8 in (("popcount", fa.ALGO_MARGINAL, False), ("mfma_int8", fa.ALGO_MFMA, True),
+                             ("mfma", fa.ALGO_MFMA, False)):
+        s = fa.GpuSearch(algorithm=algo, mfma_int8=int8)
+        s.run_arrays(d, 2, dist, outputs)   # warm
+        best = None
+        for _ in range(3):
+            t0 = time.perf_counter()
+            c, m = s.run_arrays(d, 2, dist, outputs)
+            el = time.perf_counter() - t0
+            if best is None or el < best[0]:
+                best = (el, s.last_stats.scan_ms, s.last_stats.launches)
+        res[name] = (c, m)
+        out[name] = {"wall_s": best[0], "scan_ms": best[1], "launches": best[2],
+                     "combo_samples_per_s": math.comb(snps, 2) * 2 * half / best[0]}
+        print(f"[bench] order 2 {name}: {best[0]:.3f} s wall, {best[1]:.1f} ms scan", file=sys.stderr, flush=True)
+    d.close()
+    # matrix-core work actually issued: 4 products per pair and class over
+    # the padded samples (128-sample quads), int8 MACs = 2 ops
+    quads = math.ceil(math.ceil(half / 512) * 8 / 2)
+    tiles = (math.ceil(snps / 64) * (math.ceil(snps / 64) + 1)) // 2
+    mfma_ops = tiles * 64 * 64 * 4 * 2 * (quads * 128) * 2
+    out["mfma_int8"]["mfma_tops"] = mfma_ops / (out["mfma_int8"]["scan_ms"] / 1e3) / 1e12
+    out["mfma_int8"]["frac_of_peak"] = mfma_ops / (out["mfma_int8"]["scan_ms"] / 1e3) / MFMA_I8_PEAK
+    out["mfma_int8"]["peak"] = "int8 dense 5.0 POP/s"
+    out["mfma"]["operands"] = "fp4 (e2m1) x fp4, fp32 accumulation (exact: counts < 2^24)"
+    out["mfma"]["mfma_tops"] = mfma_ops / (out["mfma"]["scan_ms"] / 1e3) / 1e12
+    out["mfma"]["frac_of_peak"] = mfma_ops / (out["mfma"]["scan_ms"] / 1e3) / MFMA_FP4_PEAK
+    out["mfma"]["peak"] = "fp4 dense 10.1 PFLOP/s"
+    same = all(res["popcount"][0].tolist() == res[k][0].tolist() and np.array_equal(res["popcount"][1], res[k][1])
+               for k in ("mfma_int8", "mfma"))
+    return {"workload": f"order-2 exhaustive scan, {snps:,} SNPs x {2 * half:,} samples ({half:,}/{half:,}), "
+                        "20% of SNPs with MAF < 0.05, top-1,000",
+            **out, "int8_kernel_speedup": out["popcount"]["scan_ms"] / out["mfma_int8"]["scan_ms"],
+            "kernel_speedup": out["popcount"]["scan_ms"] / out["mfma"]["scan_ms"],
+            "wall_speedup": out["popcount"]["wall_s"] / out["mfma"]["wall_s"], "results_identical": same,
+            "kept": "mfma (AUTO default for order 2)" if out["mfma"]["scan_ms"] < out["popcount"]["scan_ms"] else
+                    "popcount"}
+
+
 def percentile(xs, q):
     return float(np.percentile(np.asarray(xs, float), q)) if xs else None
 
@@ -288,6 +342,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the uneven-batch comparison")
     ap.add_argument("--no-serve", action="store_true", help="skip the served-stream comparison")
+    ap.add_argument("--no-order2", action="store_true", help="skip the order-2 popcount / matrix-core comparison")
     ap.add_argument("--batch-datasets", type=int, default=64)
     ap.add_argument("--serve-requests", type=int, default=300)
     ap.add_argument("--only-extra", action="store_true",
@@ -320,6 +375,8 @@ def main():
 
     if args.only_extra:
         extra = {}
+        if not args.no_order2:
+            extra["order2"] = bench_order2()
         if not args.no_batch:
             extra["batched"] = bench_batch(args.batch_datasets)
         if not args.no_serve:
@@ -392,6 +449,8 @@ def main():
         "top1": {"combination": list(top[0][0]), "mi": top[0][1]} if top else None,
     }
     ds.close()
+    if not args.no_order2:
+        line["order2"] = bench_order2()
     if not args.no_batch:
         line["batched"] = bench_batch(args.batch_datasets)
     if not args.no_serve:

@@ -49,6 +49,7 @@ class Config(C.Structure):
 
 
 FLAG_NO_THRESHOLD_SAMPLE = 1
+FLAG_MFMA_INT8 = 2
 
 
 class SearchStats(C.Structure):

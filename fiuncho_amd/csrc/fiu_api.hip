@@ -499,9 +499,12 @@ void search(fiu_dataset *d, int order, int32_t step, int32_t offset, uint32_t ou
         if (order == 2 && algo_req != FIU_ALGO_MARGINAL) algo = FIU_ALGO_MFMA;
     }
     st.algorithm = algo;
-    // matrix-core scans walk super-blocks of 8 x 8 tiles of 64 x 64 pairs
+    // matrix-core scans walk super-blocks of 8 x 8 tiles of 64 x 64 pairs;
+    // fp4 operands accumulate in fp32, exact while every count is below 2^24
     const int T = d->dev.n_pad / 64;
     const int64_t n_super = mfma2_superblocks(T);
+    const bool mfma_fp4 = !(cfg && (cfg->flags & FIU_FLAG_MFMA_INT8)) && d->cases < (1u << 24) &&
+                          d->ctrls < (1u << 24);
 
     const uint32_t cap = cfg && cfg->candidate_cap ? cfg->candidate_cap : (1u << 20);
     d->cand.alloc(cap);
@@ -563,7 +566,7 @@ void search(fiu_dataset *d, int order, int32_t step, int32_t offset, uint32_t ou
         P.j1 = chunks[ci].j1;
         if (timed) check(hipEventRecord(d->events[ev], s), "event");
         if (algo == FIU_ALGO_MFMA) {
-            check(launch_mfma2(d->dev, P, d->state.p, chunks[ci].j0, chunks[ci].j1, T, s), "scan launch");
+            check(launch_mfma2(d->dev, P, d->state.p, chunks[ci].j0, chunks[ci].j1, T, mfma_fp4, s), "scan launch");
         } else if (algo == FIU_ALGO_MARGINAL) {
             check(launch_scan_marginal(order, d->dev, P, d->state.p, s), "scan launch");
         } else {

@@ -11,9 +11,9 @@ hipError_t launch_tau_from_hist(const uint32_t *hist, uint32_t N, float delta, T
 hipError_t launch_pair_table(const DevData &D, const ScanParams &P, uint32_t *pair_out, hipStream_t s);
 // Order 2 on the matrix cores: the pair triangle of T x T 64-SNP tiles is cut
 // into super-blocks of 8 x 8 tiles; a launch scores super-blocks [s0, s1)
-// into the top-N state.
+// into the top-N state, with fp4 (e2m1) or int8 operands.
 int64_t mfma2_superblocks(int T);
-hipError_t launch_mfma2(const DevData &D, const ScanParams &P, TopState *st, int64_t s0, int64_t s1, int T,
+hipError_t launch_mfma2(const DevData &D, const ScanParams &P, TopState *st, int64_t s0, int64_t s1, int T, bool fp4,
                         hipStream_t s);
 hipError_t launch_merge(TopState *st, Cand *top, const Cand *cand, uint32_t N, uint32_t cap, uint32_t chunk_id,
                         uint32_t *redo, hipStream_t s);

@@ -20,6 +20,8 @@
 
 #include <math.h>
 
+#include <type_traits>
+
 namespace fiu {
 
 template <int N> struct Pow3 { static constexpr int v = 3 * Pow3<N - 1>::v; };
@@ -787,21 +789,55 @@ k_scan3m(const DevData D, const ScanParams P, TopState *st)
 }
 
 // ---------------------------------------------------------------------------
-// Order 2 on the matrix cores.  With one int8 {0,1} indicator row per SNP and
+// Order 2 on the matrix cores.  With one {0,1} indicator row per SNP and
 // genotype (I_g[s][k] = 1 iff sample k of the class has genotype g at SNP s),
 // the four marginal pair counts are dot products over the samples:
-// f(x, y)[a][b] = sum_k I_x[a][k] I_y[b][k] -- a symmetric int8 GEMM per class
-// (v_mfma_i32_32x32x32_i8, exact int32 accumulation); the other five cells
-// follow from the singles and the class total exactly as in the popcount scan.
+// f(x, y)[a][b] = sum_k I_x[a][k] I_y[b][k] -- a symmetric GEMM per class; the
+// other five cells follow from the singles and the class total exactly as in
+// the popcount scan.  Two operand formats, both exact:
+//   int8: v_mfma_i32_32x32x32_i8, int32 accumulation;
+//   fp4 (e2m1, 1.0 = nibble 0x2): v_mfma_f32_32x32x64_f8f6f4, fp32
+//     accumulation of 0/1 products -- exact while a count stays below 2^24
+//     (checked on the host); half the bytes per product and twice the rate.
 // ---------------------------------------------------------------------------
 typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v8i __attribute__((ext_vector_type(8)));
 typedef int v16i __attribute__((ext_vector_type(16)));
+typedef float v16f __attribute__((ext_vector_type(16)));
 
-constexpr int kMfmaTile = 64;       // SNPs per tile side (per workgroup)
-constexpr int kMfmaK = 128;         // samples per stage: one 128-bit quad per row
-constexpr int kMfmaRow = 144;       // LDS row pitch in bytes (128 + 16: the 16 rows a
-                                    // ds_read_b128 phase touches hit distinct banks)
+constexpr int kMfmaTile = 64;       // SNPs per tile side (per workgroup); a stage is one
+                                    // 128-sample quad per row
 constexpr int kMfmaSuper = 8;       // tiles per side of the super-blocks run per XCD
+
+// LDS row pitch per format: the data (128 int8 or 64 bytes of fp4) + 16, so
+// that the 16 rows one ds_read_b128 phase touches hit distinct banks.
+template <bool FP4> struct MfmaFmt {
+    static constexpr int row = FP4 ? 80 : 144;
+    static constexpr int substeps = FP4 ? 2 : 4;   // MFMAs per 128-sample stage and accumulator
+};
+
+// Eight bits (b < 256) as eight e2m1 nibbles, bit i -> nibble i = 0x2 (1.0)
+// or 0x0: the bits are spread 4 apart by three shift-or-mask steps, then
+// moved to bit 1 of each nibble.
+__device__ __forceinline__ uint32_t byte_fp4(uint32_t b)
+{
+    b = (b | (b << 12)) & 0x000F000Fu;
+    b = (b | (b << 6)) & 0x03030303u;
+    b = (b | (b << 3)) & 0x11111111u;
+    return b << 1;
+}
+
+// 128 bits of a quad as 64 bytes of e2m1 indicators in one LDS row: sample
+// 32 e + j (bit j of word e) is nibble 32 e + j.
+__device__ __forceinline__ void expand_quad_fp4(const uint4 &q, int8_t *row)
+{
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const uint32_t v = comp(q, e);
+        *(uint4 *)(row + 16 * e) = make_uint4(byte_fp4(v & 255u), byte_fp4(v >> 8 & 255u),
+                                              byte_fp4(v >> 16 & 255u), byte_fp4(v >> 24));
+    }
+}
 
 // Four bits of v (from bit `lo`) as four bytes 0/1: n * 0x00204081 puts bit i
 // of the nibble n at bit 8i (the shifted copies do not overlap), the mask
@@ -845,11 +881,14 @@ __device__ __forceinline__ void tri_tile(int64_t u, int T, int &ta, int &tb)
 // (8 x 8 tiles each): block v runs on XCD v % 8, and the 64 blocks an XCD
 // holds at once are one super-block, whose 8 + 8 row panels they share in L2.
 // Each stage loads one 128-bit quad of the bit tables per row (A: 64 SNPs x
-// genotypes 0/1, B likewise) and expands it to int8 in LDS.
+// genotypes 0/1, B likewise) and expands it to the operand format in LDS.
+template <bool FP4>
 __global__ void __launch_bounds__(kBlock)
 k_mfma2(const DevData D, const ScanParams P, TopState *st, int64_t s0, int64_t s1, int T)
 {
-    __shared__ __attribute__((aligned(16))) int8_t lds[2][2][2 * kMfmaTile][kMfmaRow];  // [buf][A/B][row][k]
+    typedef MfmaFmt<FP4> F;
+    typedef typename std::conditional<FP4, v16f, v16i>::type Acc;
+    __shared__ __attribute__((aligned(16))) int8_t lds[2][2][2 * kMfmaTile][F::row];  // [buf][A/B][row][k]
     const int S = (T + kMfmaSuper - 1) / kMfmaSuper;
     const int64_t k = blockIdx.x >> 3, su = s0 + (k >> 6) * 8 + (blockIdx.x & 7);
     if (su >= s1) return;
@@ -865,25 +904,30 @@ k_mfma2(const DevData D, const ScanParams P, TopState *st, int64_t s0, int64_t s
     // this thread's staging row: operand m (A / B), genotype g, SNP
     const int m_ld = tid >> 7, row_ld = tid & 127, g_ld = row_ld >> 6;
     const int snp_ld = (m_ld ? b0 : a0) + (row_ld & 63);
-    v16i acc[2][2][2];
+    Acc acc[2][2][2];
 #pragma unroll
     for (int c = 0; c < 2; ++c)
 #pragma unroll
         for (int g = 0; g < 2; ++g)
 #pragma unroll
-            for (int h = 0; h < 2; ++h) acc[c][g][h] = (v16i){0};
+            for (int h = 0; h < 2; ++h) acc[c][g][h] = (Acc){0};
+    auto expand = [&](const uint4 &q, int buf) {
+        if constexpr (FP4) expand_quad_fp4(q, lds[buf][m_ld][row_ld]);
+        else expand_quad(q, lds[buf][m_ld][row_ld]);
+    };
 #pragma unroll
     for (int cls = 0; cls < 2; ++cls) {
         const int iters = D.quads[cls];
         const uint4 *src = D.bits + D.class_off[cls] + (uint64_t)g_ld * np + snp_ld;   // quad q at + q * 3 * np
         uint4 st_reg = src[0];
-        expand_quad(st_reg, lds[0][m_ld][row_ld]);
+        expand(st_reg, 0);
         __syncthreads();
         for (int it = 0; it < iters; ++it) {
             const int buf = it & 1;
             if (it + 1 < iters) st_reg = src[(uint64_t)(it + 1) * 3 * np];
 #pragma unroll
-            for (int s = 0; s < kMfmaK / 32; ++s) {
+            for (int s = 0; s < F::substeps; ++s) {
+                // 16 operand bytes per lane: 16 int8 or 32 fp4 samples of row r
                 const int kk = s * 32 + (lane >> 5) * 16, r = lane & 31;
                 v4i fa[2], fb[2];
 #pragma unroll
@@ -894,10 +938,20 @@ k_mfma2(const DevData D, const ScanParams P, TopState *st, int64_t s0, int64_t s
 #pragma unroll
                 for (int g = 0; g < 2; ++g)
 #pragma unroll
-                    for (int h = 0; h < 2; ++h)
-                        acc[cls][g][h] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[g], fb[h], acc[cls][g][h], 0, 0, 0);
+                    for (int h = 0; h < 2; ++h) {
+                        if constexpr (FP4) {
+                            const v8i a8 = {fa[g][0], fa[g][1], fa[g][2], fa[g][3], 0, 0, 0, 0};
+                            const v8i b8 = {fb[h][0], fb[h][1], fb[h][2], fb[h][3], 0, 0, 0, 0};
+                            // formats 4/4 = e2m1; scale operands 0 select the unscaled product
+                            acc[cls][g][h] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a8, b8, acc[cls][g][h],
+                                                                                           4, 4, 0, 0, 0, 0);
+                        } else {
+                            acc[cls][g][h] =
+                                __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[g], fb[h], acc[cls][g][h], 0, 0, 0);
+                        }
+                    }
             }
-            if (it + 1 < iters) expand_quad(st_reg, lds[buf ^ 1][m_ld][row_ld]);
+            if (it + 1 < iters) expand(st_reg, buf ^ 1);
             __syncthreads();
         }
     }
@@ -1338,13 +1392,15 @@ int64_t mfma2_superblocks(int T)
     return S * (S + 1) / 2;
 }
 
-hipError_t launch_mfma2(const DevData &D, const ScanParams &P, TopState *st, int64_t s0, int64_t s1, int T,
+hipError_t launch_mfma2(const DevData &D, const ScanParams &P, TopState *st, int64_t s0, int64_t s1, int T, bool fp4,
                         hipStream_t s)
 {
     if (s1 <= s0) return hipSuccess;
     // 64 blocks per super-block, 8 super-blocks (one per XCD) per round
     const int64_t rounds = (s1 - s0 + 7) / 8;
-    hipLaunchKernelGGL(k_mfma2, dim3((unsigned)(rounds * 8 * 64)), dim3(kBlock), 0, s, D, P, st, s0, s1, T);
+    const dim3 grid((unsigned)(rounds * 8 * 64)), block(kBlock);
+    if (fp4) hipLaunchKernelGGL((k_mfma2<true>), grid, block, 0, s, D, P, st, s0, s1, T);
+    else hipLaunchKernelGGL((k_mfma2<false>), grid, block, 0, s, D, P, st, s0, s1, T);
     return hipGetLastError();
 }
 

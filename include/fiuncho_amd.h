@@ -109,6 +109,11 @@ typedef enum {
 /* fiu_config.flags */
 #define FIU_FLAG_NO_THRESHOLD_SAMPLE 1u  /* skip the pass that seeds the top-N
                                             threshold from a sample */
+#define FIU_FLAG_MFMA_INT8 2u            /* FIU_ALGO_MFMA: int8 operands with
+                                            int32 accumulation instead of the
+                                            default fp4 (e2m1) operands with
+                                            fp32 accumulation (exact below
+                                            2^24 samples per class) */
 
 typedef struct {
     uint32_t struct_size;    /* sizeof(fiu_config) */

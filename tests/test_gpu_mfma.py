@@ -1,7 +1,9 @@
 """GPU: the order-2 matrix-core scan (FIU_ALGO_MFMA).
 
-The marginal pair counts come from int8 indicator dot products accumulated in
-int32 (exact); everything downstream is the popcount scan's.  Bar: the same
+The marginal pair counts come from indicator dot products on the matrix cores
+-- int8 operands with int32 accumulation, or fp4 (e2m1) operands with fp32
+accumulation, both exact at these sizes; everything downstream is the popcount
+scan's.  Bar: the same
 top-N (combinations, order and float MI bit for bit) as the oracle's FP64
 ranking and as the popcount scan.
 """
@@ -18,17 +20,18 @@ from .test_gpu_parity import synth
 pytestmark = pytest.mark.gpu
 
 
+@pytest.mark.parametrize("int8", [False, True])
 @pytest.mark.parametrize("snps,cases,ctrls,outputs,seed", [
     (70, 129, 63, 40, 21),       # one partial tile, ragged classes
     (128, 512, 512, 100, 22),    # two full tiles per side
     (200, 1000, 777, 300, 26),   # several tiles, K of 8 and 7 stages
     (65, 40, 2000, 2080, 27),    # every pair, one SNP in the last tile
 ])
-def test_mfma_vs_oracle(snps, cases, ctrls, outputs, seed):
-    print(f"case: mfma {snps}x{cases}+{ctrls}", flush=True)
+def test_mfma_vs_oracle(snps, cases, ctrls, outputs, seed, int8):
+    print(f"case: mfma {snps}x{cases}+{ctrls} int8={int8}", flush=True)
     d, od = synth(snps, cases, ctrls, seed)
     oc, om, _ = O.search(od, 2, outputs, threads=4, mi_mode=2)
-    s = fa.GpuSearch(algorithm=fa.ALGO_MFMA)
+    s = fa.GpuSearch(algorithm=fa.ALGO_MFMA, mfma_int8=int8)
     c, m = s.run_arrays(d, 2, fa.Distribution(snps, 1), outputs)
     assert s.last_stats.algorithm == fa.ALGO_MFMA
     assert c.tolist() == oc.tolist() and np.array_equal(m, om)
@@ -39,9 +42,10 @@ def test_mfma_matches_popcount_scan():
     print("case: mfma vs popcount 1500 SNPs", flush=True)
     d, _ = synth(1500, 3000, 2500, 31, planted=(100, 1200))
     ref = fa.GpuSearch(algorithm=fa.ALGO_MARGINAL).run_arrays(d, 2, fa.Distribution(1500, 1), 500)
-    c, m = fa.GpuSearch(algorithm=fa.ALGO_MFMA).run_arrays(d, 2, fa.Distribution(1500, 1), 500)
-    assert c.tolist() == ref[0].tolist() and np.array_equal(m, ref[1])
-    assert c[0].tolist() == [100, 1200]
+    for int8 in (False, True):
+        c, m = fa.GpuSearch(algorithm=fa.ALGO_MFMA, mfma_int8=int8).run_arrays(d, 2, fa.Distribution(1500, 1), 500)
+        assert c.tolist() == ref[0].tolist() and np.array_equal(m, ref[1])
+        assert c[0].tolist() == [100, 1200]
 
 
 def test_mfma_distribution_and_chunks():
