@@ -5,16 +5,22 @@ cases / 8,000 controls), HWE genotypes with MAF ~ U(0.05, 0.5), seed 1, one
 planted triple (17, 4242, 9001) with a random 27-cell penetrance table in
 [0.3, 0.7] from seed 2; top-100 triples by mutual information.
 
-A step scans one slice of the full scan: the combinations whose pair prefix
-has lexicographic rank = i (mod SLICES), i.e. Distribution(n, 2, SLICES, i);
---steps K scans slices 0..K-1 (K = SLICES covers every triple once).  The
-metric is genotype-combination x samples per second over the timed steps.
+A step is one whole order-3 scan (all 1.67e11 triples; --slices S makes a
+step one Distribution(n, 2, S, i) share instead).  The metric is
+genotype-combination x samples per second over the timed steps.
 
 Also reported on the same JSON line:
-  roofline      the scan kernel against the int32 VALU peak (AND + popcount),
-                algorithmic ops per combination x sample = 2*3^k/32 (k = 3:
-                1.6875), durations from HIP events recorded on the library's
-                stream around every scan launch;
+  roofline      the dominant scan kernel against its peak: the order-3
+                matrix-core scan (k_mfma3) against the dense fp4 MFMA rate,
+                8 cells x 2 flops per combination x sample; the popcount scans
+                against the int32 VALU peak with the reference algorithm's
+                2*3^k/32 ops per combination x sample (k = 3: 1.6875);
+                durations from HIP events on the library's stream around every
+                scan launch;
+  order2        50,000 SNPs x 32,000 samples at order 2: popcount vs int8 vs
+                fp4 matrix-core scans (identical results required);
+  batched       64 uneven datasets, one search per call vs one batched call;
+  serving       a stream of scan requests, plain vs the scan server;
   cpu_baseline  the reference ThreadedSearch compiled from its own sources
                 (oracle/_ref, all host cores) or, when that library is absent,
                 the oracle's AVX-512 restatement, on a prefix sample of the
@@ -35,7 +41,7 @@ sys.path.insert(0, ROOT)
 
 N_SNPS, N_CASES, N_CTRLS = 10_000, 8_000, 8_000
 PLANTED = (17, 4242, 9001)
-ORDER, OUTPUTS, SLICES = 3, 100, 64
+ORDER, OUTPUTS = 3, 100
 VALU_PEAK = 256 * 64 * 2.4e9          # int32 VALU lane-ops per second (MI355X)
 # Bytes fetched beyond L2 per triple by the order-3 marginal scan kernel:
 # rocprofv3 FETCH_SIZE per dispatch x 2.0 (the factor a plain 1 GiB copy
@@ -335,9 +341,11 @@ def write_raw(path, geno, cases):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=16)
+    ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--algorithm", default="auto", choices=["auto", "direct", "marginal"])
+    ap.add_argument("--slices", type=int, default=1,
+                    help="a step scans Distribution(n, 2, SLICES, i): 1 = the whole order-3 scan")
+    ap.add_argument("--algorithm", default="auto", choices=["auto", "direct", "marginal", "mfma"])
     ap.add_argument("--cpu-sample-step", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the uneven-batch comparison")
@@ -368,8 +376,8 @@ def main():
               "The scan shards by Distribution layers, one search per GPU, with a host merge "
               "of the per-GPU top-N lists.", file=sys.stderr)
         sys.exit(2)
-    if args.steps < 1 or args.steps + args.warmup > SLICES:
-        sys.exit(f"--steps + --warmup must be within 1..{SLICES}")
+    if args.steps < 1 or args.slices < 1 or args.warmup < 0 or args.warmup > 4096:
+        sys.exit("--steps and --slices must be >= 1, --warmup within 0..4096")
 
     import fiuncho_amd as fa
 
@@ -383,17 +391,19 @@ def main():
             extra["serving"] = bench_serve(args.serve_requests)
         print(json.dumps(extra), flush=True)
         return
-    algo = {"auto": fa.ALGO_AUTO, "direct": fa.ALGO_DIRECT, "marginal": fa.ALGO_MARGINAL}[args.algorithm]
+    algo = {"auto": fa.ALGO_AUTO, "direct": fa.ALGO_DIRECT, "marginal": fa.ALGO_MARGINAL,
+            "mfma": fa.ALGO_MFMA}[args.algorithm]
     t0 = time.perf_counter()
     spec = fa.synth_spec(N_SNPS, N_CASES, N_CTRLS, seed=1, planted=PLANTED, penetrance_seed=2)
     words, geno, cw, tw = fa.synth_generate(spec, words=True, geno=not args.no_cpu)
     ds = fa.Dataset.from_words(words, N_SNPS, cw, tw, N_CASES, N_CTRLS)
     print(f"[bench] dataset generated and uploaded in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     search = fa.GpuSearch(algorithm=algo)
-    # warmup on the last slices (never timed), then the timed slices 0..K-1
+    S = args.slices
+    # warmup (never timed): small shares of the scan build the pair table and
+    # load the code; then the timed steps
     for w in range(args.warmup):
-        search.run_arrays(ds, ORDER, fa.Distribution(N_SNPS, 2, SLICES, SLICES - 1 - w), OUTPUTS)
-    fa.lib()  # library loaded
+        search.run_arrays(ds, ORDER, fa.Distribution(N_SNPS, 2, 4096, 4095 - w), OUTPUTS)
     results = []
     combos = 0
     scan_ms = 0.0
@@ -401,12 +411,12 @@ def main():
     used_algo = None
     t_start = time.perf_counter()
     for i in range(args.steps):
-        c, m = search.run_arrays(ds, ORDER, fa.Distribution(N_SNPS, 2, SLICES, i), OUTPUTS)
+        c, m = search.run_arrays(ds, ORDER, fa.Distribution(N_SNPS, 2, S, i % S), OUTPUTS)
         st = search.last_stats
         scan_ms += st.scan_ms
         launches += st.launches
         used_algo = st.algorithm
-        combos += slice_combos(N_SNPS, ORDER, SLICES, i)
+        combos += st.combinations
         results += list(zip(map(tuple, c.tolist()), m.tolist()))
         print(f"[bench] step {i} done: {st.scan_ms / 1e3:.2f} s device", file=sys.stderr, flush=True)
     elapsed = time.perf_counter() - t_start
@@ -414,15 +424,28 @@ def main():
     top = results[:OUTPUTS]
     unit_samples = N_CASES + N_CTRLS
     value = combos * unit_samples / elapsed
-    ops = combos * alg_ops_per_combo(ORDER, N_CASES, N_CTRLS)
-    achieved = ops / (scan_ms / 1e3)
-    algo_name = {1: "direct", 2: "marginal"}.get(used_algo, str(used_algo))
-    # operations the kernel executes per triple: AND + popcount of the 8
-    # (marginal) or 27 (direct) cells over every stored 32-bit word
-    words32 = 2 * (cw + tw)
-    executed = combos * 2 * (8 if algo_name == "marginal" else 27) * words32
+    algo_name = {1: "direct", 2: "marginal", 3: "mfma"}.get(used_algo, str(used_algo))
     per_triple = TRAFFIC_BYTES_PER_TRIPLE.get(algo_name)
     traffic = per_triple * combos / max(launches, 1) if per_triple else None
+    ref_ops = combos * alg_ops_per_combo(ORDER, N_CASES, N_CTRLS)   # the reference's AND + popcount ops
+    if algo_name == "mfma":
+        # matrix-core work: 8 cells x (cases + ctrls) fp4 multiply-adds per triple
+        flops = combos * 8 * 2 * unit_samples
+        roofline = {"bound": "mfma", "achieved": flops / (scan_ms / 1e3) / 1e12, "peak": MFMA_FP4_PEAK / 1e12,
+                    "unit": "TFLOP/s", "frac": flops / (scan_ms / 1e3) / MFMA_FP4_PEAK, "traffic": traffic,
+                    "kernel": "k_mfma3", "operands": "fp4 (e2m1) x fp4, fp32 accumulation (exact)",
+                    "flops_per_combo_sample": 16,
+                    "reference_algorithm_valu_frac": ref_ops / (scan_ms / 1e3) / VALU_PEAK}
+    else:
+        # operations the kernel executes per triple: AND + popcount of the 8
+        # (marginal) or 27 (direct) cells over every stored 32-bit word
+        executed = combos * 2 * (8 if algo_name == "marginal" else 27) * 2 * (cw + tw)
+        roofline = {"bound": "valu", "achieved": ref_ops / (scan_ms / 1e3) / 1e12, "peak": VALU_PEAK / 1e12,
+                    "unit": "TOP/s", "frac": ref_ops / (scan_ms / 1e3) / VALU_PEAK, "traffic": traffic,
+                    "kernel": "k_scan3m" if algo_name == "marginal" else "k_scan_direct<3>",
+                    "ops_per_combo_sample": alg_ops_per_combo(ORDER, N_CASES, N_CTRLS) / unit_samples,
+                    "executed_frac": executed / (scan_ms / 1e3) / VALU_PEAK}
+    roofline.update({"launches": launches, "avg_launch_ms": scan_ms / max(launches, 1)})
     line = {
         "metric": "genotype-combos*samples/s at order 3",
         "value": value,
@@ -437,15 +460,10 @@ def main():
         "dtype": "u32",
         "data": "synthetic (seeded HWE genotypes, planted triple 17/4242/9001)",
         "config": {"workload": "order-3 exhaustive scan, 10,000 SNPs x 16,000 samples (8,000/8,000), top-100",
-                   "slices": f"{args.steps} of {SLICES} Distribution(10000, 2, {SLICES}, i) slices",
+                   "step": "one full scan" if S == 1 else f"one Distribution(10000, 2, {S}, i) share",
                    "combinations": combos,
                    "algorithm": algo_name},
-        "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": VALU_PEAK / 1e12, "unit": "TOP/s",
-                     "frac": achieved / VALU_PEAK, "traffic": traffic,
-                     "kernel": "k_scan3m" if algo_name == "marginal" else "k_scan_direct<3>",
-                     "ops_per_combo_sample": alg_ops_per_combo(ORDER, N_CASES, N_CTRLS) / unit_samples,
-                     "executed_frac": executed / (scan_ms / 1e3) / VALU_PEAK,
-                     "launches": launches, "avg_launch_ms": scan_ms / max(launches, 1)},
+        "roofline": roofline,
         "top1": {"combination": list(top[0][0]), "mi": top[0][1]} if top else None,
     }
     ds.close()

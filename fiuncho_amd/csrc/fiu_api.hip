@@ -472,7 +472,8 @@ void search(fiu_dataset *d, int order, int32_t step, int32_t offset, uint32_t ou
     if (algo_req < FIU_ALGO_AUTO || algo_req > FIU_ALGO_MFMA) fail(FIU_ERR_INVALID, "unknown algorithm");
     if ((algo_req == FIU_ALGO_MARGINAL || algo_req == FIU_ALGO_MFMA) && !d->dev.uniform)
         fail(FIU_ERR_INVALID, "marginal algorithm needs every SNP to cover the same samples");
-    if (algo_req == FIU_ALGO_MFMA && order != 2) fail(FIU_ERR_UNSUPPORTED, "the matrix-core scan is order 2 only");
+    if (algo_req == FIU_ALGO_MFMA && order != 2 && order != 3)
+        fail(FIU_ERR_UNSUPPORTED, "the matrix-core scan covers orders 2 and 3");
     fiu_search_stats st{};
     st.algorithm = FIU_ALGO_DIRECT;
     const int n = (int)d->snps, M = order - 1;
@@ -491,12 +492,19 @@ void search(fiu_dataset *d, int order, int32_t step, int32_t offset, uint32_t ou
     check(hipSetDevice(d->device), "hipSetDevice");
     ensure_binom(d, M, s);
     int algo = FIU_ALGO_DIRECT;
+    const bool small_classes = d->cases < (1u << 24) && d->ctrls < (1u << 24);   // fp32-exact counts
     if (algo_req != FIU_ALGO_DIRECT && d->dev.uniform && (order == 2 || order == 3)) {
         algo = FIU_ALGO_MARGINAL;
         if (order == 3 && !ensure_pair_table(d, s)) algo = FIU_ALGO_DIRECT;  // table does not fit
-        // order 2: the matrix-core scan is the faster exact path (3.4x on
+        // order 2: the matrix-core scan is the faster exact path (4.4x on
         // 50,000 SNPs x 32,000 samples, also ahead on small inputs)
         if (order == 2 && algo_req != FIU_ALGO_MARGINAL) algo = FIU_ALGO_MFMA;
+        // order 3: the matrix-core scan walks every (b, c) of its first SNPs,
+        // so by default it takes whole scans (a round-robin share of the
+        // prefixes is masked, which wastes the other shares' work)
+        if (order == 3 && algo == FIU_ALGO_MARGINAL && small_classes &&
+            (algo_req == FIU_ALGO_MFMA || (algo_req == FIU_ALGO_AUTO && step == 1)))
+            algo = FIU_ALGO_MFMA;
     }
     st.algorithm = algo;
     // matrix-core scans walk super-blocks of 8 x 8 tiles of 64 x 64 pairs;
@@ -518,10 +526,23 @@ void search(fiu_dataset *d, int order, int32_t step, int32_t offset, uint32_t ou
     const double per_avg = (double)(n - M) / (double)(M + 1);
     const uint64_t target = cfg && cfg->chunk_combos ? cfg->chunk_combos : (1ull << 28);
     std::vector<Chunk> chunks;
-    if (algo == FIU_ALGO_MFMA) {
+    if (algo == FIU_ALGO_MFMA && order == 2) {
         // chunks of super-blocks (262,144 pairs each) instead of prefixes
         const int64_t len = std::max<int64_t>(1, (int64_t)(target / (64 * 4096)));
         for (int64_t t = 0; t < n_super; t += len) chunks.push_back({t, std::min(n_super, t + len)});
+    } else if (algo == FIU_ALGO_MFMA) {
+        // chunks of first SNPs a, about `target` triples (a, b, c) each
+        int64_t a = 0;
+        while (a < n - 2) {
+            uint64_t acc = 0;
+            int64_t e = a;
+            while (e < n - 2 && (acc < target || e == a) && e - a < 65535) {
+                acc += binom_sat((uint64_t)(n - 1 - e), 2);
+                ++e;
+            }
+            chunks.push_back({a, e});
+            a = e;
+        }
     } else {
         int64_t j = 0;
         const int64_t len = (int64_t)std::max<double>(1.0, (double)target / std::max(per_avg, 1.0));
@@ -565,8 +586,10 @@ void search(fiu_dataset *d, int order, int32_t step, int32_t offset, uint32_t ou
         P.j0 = chunks[ci].j0;
         P.j1 = chunks[ci].j1;
         if (timed) check(hipEventRecord(d->events[ev], s), "event");
-        if (algo == FIU_ALGO_MFMA) {
+        if (algo == FIU_ALGO_MFMA && order == 2) {
             check(launch_mfma2(d->dev, P, d->state.p, chunks[ci].j0, chunks[ci].j1, T, mfma_fp4, s), "scan launch");
+        } else if (algo == FIU_ALGO_MFMA) {
+            check(launch_mfma3(d->dev, P, d->state.p, (int)chunks[ci].j0, (int)chunks[ci].j1, T, s), "scan launch");
         } else if (algo == FIU_ALGO_MARGINAL) {
             check(launch_scan_marginal(order, d->dev, P, d->state.p, s), "scan launch");
         } else {

@@ -13,6 +13,9 @@ hipError_t launch_pair_table(const DevData &D, const ScanParams &P, uint32_t *pa
 // into super-blocks of 8 x 8 tiles; a launch scores super-blocks [s0, s1)
 // into the top-N state, with fp4 (e2m1) or int8 operands.
 int64_t mfma2_superblocks(int T);
+// Order 3 on the matrix cores (fp4): first SNPs [a0, a1), every (b, c) past
+// them, with the pair table in P.pair.
+hipError_t launch_mfma3(const DevData &D, const ScanParams &P, TopState *st, int a0, int a1, int T, hipStream_t s);
 hipError_t launch_mfma2(const DevData &D, const ScanParams &P, TopState *st, int64_t s0, int64_t s1, int T, bool fp4,
                         hipStream_t s);
 hipError_t launch_merge(TopState *st, Cand *top, const Cand *cand, uint32_t N, uint32_t cap, uint32_t chunk_id,

@@ -919,12 +919,17 @@ k_mfma2(const DevData D, const ScanParams P, TopState *st, int64_t s0, int64_t s
     for (int cls = 0; cls < 2; ++cls) {
         const int iters = D.quads[cls];
         const uint4 *src = D.bits + D.class_off[cls] + (uint64_t)g_ld * np + snp_ld;   // quad q at + q * 3 * np
-        uint4 st_reg = src[0];
-        expand(st_reg, 0);
+        // quads it+1 and it+2 wait in registers, quad it+3 is loaded during
+        // stage it: two stages of latency hiding for every load
+        uint4 q1 = make_uint4(0, 0, 0, 0), q2 = q1;
+        expand(src[0], 0);
+        if (iters > 1) q1 = src[(uint64_t)3 * np];
+        if (iters > 2) q2 = src[(uint64_t)6 * np];
         __syncthreads();
         for (int it = 0; it < iters; ++it) {
             const int buf = it & 1;
-            if (it + 1 < iters) st_reg = src[(uint64_t)(it + 1) * 3 * np];
+            uint4 q3 = make_uint4(0, 0, 0, 0);
+            if (it + 3 < iters) q3 = src[(uint64_t)(it + 3) * 3 * np];
 #pragma unroll
             for (int s = 0; s < F::substeps; ++s) {
                 // 16 operand bytes per lane: 16 int8 or 32 fp4 samples of row r
@@ -951,8 +956,10 @@ k_mfma2(const DevData D, const ScanParams P, TopState *st, int64_t s0, int64_t s
                         }
                     }
             }
-            if (it + 1 < iters) expand(st_reg, buf ^ 1);
+            if (it + 1 < iters) expand(q1, buf ^ 1);
             __syncthreads();
+            q1 = q2;
+            q2 = q3;
         }
     }
     // epilogue: lane holds column b = b0 + wb*32 + lane%32 and rows
@@ -991,6 +998,178 @@ k_mfma2(const DevData D, const ScanParams P, TopState *st, int64_t s0, int64_t s
                 mobius2(m);
             }
             score_and_push<9>(valid, m0, m1, ((uint64_t)a << P.combo_bits) | (uint64_t)b, tau, tau_combo, P, st);
+        }
+}
+
+// ---------------------------------------------------------------------------
+// Order 3 on the matrix cores.  For a fixed first SNP a, the eight counts
+// f(x, y, z), x, y, z in {0, 1}, of the triples (a, b, c) are sums over the
+// samples of I_x[a] I_y[b] I_z[c] = (I_y[b] AND I_x[a]) . I_z[c]: an fp4
+// product of b rows masked by a row of a with c rows.  The other 19 cells
+// follow from the pair table, the singles and the class total exactly as in
+// the popcount scan (same inclusion-exclusion, same MI and top-N code).
+// One workgroup: first SNP a, 64 b's x 32 c's, 4 waves of 32 b x 16 c; each
+// v_mfma_f32_16x16x128_f8f6f4 multiplies 16 b rows by 16 c rows over 128
+// samples (one bit-table quad).  Grid: y = first SNP a0 + y, x = the tiles
+// (tb <= tc) of the 64-SNP triangle from a's tile on, each split into two
+// 32-c halves.
+// ---------------------------------------------------------------------------
+typedef float v4f __attribute__((ext_vector_type(4)));
+constexpr int kM3Rows = 2 * 64 + 2 * 32 + 2;   // b rows (y, b), c rows (z, c), a rows (x)
+constexpr int kM3Pitch = 80;                   // 64 bytes of fp4 + 16
+
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2)))   // fits in 256 registers
+k_mfma3(const DevData D, const ScanParams P, TopState *st, int a0, int T)
+{
+    __shared__ __attribute__((aligned(16))) int8_t lds[2][kM3Rows][kM3Pitch];
+    const int a = a0 + (int)blockIdx.y;
+    if (a >= D.n - 2) return;
+    const int ta = (a + 1) >> 6, m = T - ta;
+    const int64_t u = blockIdx.x;
+    if (u >= (int64_t)m * (m + 1)) return;
+    int r, cc;
+    tri_tile(u >> 1, m, r, cc);
+    const int b0 = (ta + r) * 64, c0 = (ta + cc) * 64 + (int)(u & 1) * 32;
+    if (c0 >= D.n) return;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6), wb = w >> 1, wc = w & 1;
+    const int np = D.n_pad;
+    // staging row of this thread: b rows 0..127, c rows 128..191, a rows 192..193
+    int snp_ld = -1, g_ld = 0;
+    if (tid < 128) snp_ld = b0 + (tid & 63), g_ld = tid >> 6;
+    else if (tid < 192) snp_ld = c0 + ((tid - 128) & 31), g_ld = (tid - 128) >> 5;
+    else if (tid < kM3Rows) snp_ld = a, g_ld = tid - 192;
+    v4f acc[2][2][2][2][2];   // [class][x][y][b block][z]
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+            for (int y = 0; y < 2; ++y)
+#pragma unroll
+                for (int bb = 0; bb < 2; ++bb)
+#pragma unroll
+                    for (int z = 0; z < 2; ++z) acc[c][x][y][bb][z] = (v4f){0.f, 0.f, 0.f, 0.f};
+    const int kk = (lane >> 4) * 16, rr = lane & 15;
+#pragma unroll
+    for (int cls = 0; cls < 2; ++cls) {
+        const int iters = D.quads[cls];
+        const uint4 *src = snp_ld >= 0 ? D.bits + D.class_off[cls] + (uint64_t)g_ld * np + snp_ld : nullptr;
+        // quads it+1 and it+2 wait in registers, quad it+3 is loaded during
+        // stage it: two stages of latency hiding for every load
+        uint4 q1 = make_uint4(0, 0, 0, 0), q2 = q1;
+        if (src) {
+            expand_quad_fp4(src[0], lds[0][tid]);
+            if (iters > 1) q1 = src[(uint64_t)3 * np];
+            if (iters > 2) q2 = src[(uint64_t)6 * np];
+        }
+        __syncthreads();
+        for (int it = 0; it < iters; ++it) {
+            const int buf = it & 1;
+            uint4 q3 = make_uint4(0, 0, 0, 0);
+            if (src && it + 3 < iters) q3 = src[(uint64_t)(it + 3) * 3 * np];
+            v4i ma[2], fb[2][2], fc[2];
+#pragma unroll
+            for (int x = 0; x < 2; ++x) ma[x] = *(const v4i *)&lds[buf][192 + x][kk];
+#pragma unroll
+            for (int y = 0; y < 2; ++y)
+#pragma unroll
+                for (int bb = 0; bb < 2; ++bb) fb[y][bb] = *(const v4i *)&lds[buf][y * 64 + wb * 32 + bb * 16 + rr][kk];
+#pragma unroll
+            for (int z = 0; z < 2; ++z) fc[z] = *(const v4i *)&lds[buf][128 + z * 32 + wc * 16 + rr][kk];
+#pragma unroll
+            for (int x = 0; x < 2; ++x)
+#pragma unroll
+                for (int y = 0; y < 2; ++y)
+#pragma unroll
+                    for (int bb = 0; bb < 2; ++bb) {
+                        const v4i av = fb[y][bb] & ma[x];   // nibble 0x2 & 0x2 = 1.0, else 0
+                        const v8i a8 = {av[0], av[1], av[2], av[3], 0, 0, 0, 0};
+#pragma unroll
+                        for (int z = 0; z < 2; ++z) {
+                            const v8i c8 = {fc[z][0], fc[z][1], fc[z][2], fc[z][3], 0, 0, 0, 0};
+                            acc[cls][x][y][bb][z] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(
+                                a8, c8, acc[cls][x][y][bb][z], 4, 4, 0, 0, 0, 0);
+                        }
+                    }
+            if (src && it + 1 < iters) expand_quad_fp4(q1, lds[buf ^ 1][tid]);
+            __syncthreads();
+            q1 = q2;
+            q2 = q3;
+        }
+    }
+    // epilogue: lane holds column c = c0 + wc*16 + lane%16 and, per b block
+    // bb, rows b = b0 + wb*32 + 16 bb + 4(lane/16) + j, j < 4
+    const int n = D.n;
+    const float tau = st->tau;
+    const uint64_t tau_combo = st->tau_combo;
+    const int c = c0 + wc * 16 + rr;
+    const bool c_ok = c < n && c > a;
+    const uint32_t *pair = P.pair;
+    const uint32_t *sa = D.singles + (size_t)a * 6;
+    uint32_t sc[2][2] = {{0, 0}, {0, 0}}, fac[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (c_ok) {
+        const uint32_t *sp = D.singles + (size_t)c * 6;
+        sc[0][0] = sp[0], sc[0][1] = sp[1], sc[1][0] = sp[3], sc[1][1] = sp[4];
+        const uint4 *pac = (const uint4 *)(pair + pair_index(a, c, n) * 8);
+        const uint4 u0 = pac[0], u1 = pac[1];
+        fac[0] = u0.x, fac[1] = u0.y, fac[2] = u0.z, fac[3] = u0.w;
+        fac[4] = u1.x, fac[5] = u1.y, fac[6] = u1.z, fac[7] = u1.w;
+    }
+    const int64_t rank_a = (int64_t)a * (2 * (int64_t)n - a - 1) / 2;   // rank of the prefix (a, a+1)
+#pragma unroll
+    for (int bb = 0; bb < 2; ++bb)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int b = b0 + wb * 32 + bb * 16 + 4 * (lane >> 4) + j;
+            const int64_t rel = rank_a + (b - a - 1) - P.offset;
+            const bool valid = c_ok && b > a && c > b && rel >= 0 && rel % P.step == 0;
+            uint32_t fab[8] = {0, 0, 0, 0, 0, 0, 0, 0}, fbc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            uint32_t sb[6] = {0, 0, 0, 0, 0, 0};
+            if (valid) {
+                const uint4 *pab = (const uint4 *)(pair + pair_index(a, b, n) * 8);
+                const uint4 *pbc = (const uint4 *)(pair + pair_index(b, c, n) * 8);
+                const uint4 p0 = pab[0], p1 = pab[1], q0 = pbc[0], q1 = pbc[1];
+                fab[0] = p0.x, fab[1] = p0.y, fab[2] = p0.z, fab[3] = p0.w;
+                fab[4] = p1.x, fab[5] = p1.y, fab[6] = p1.z, fab[7] = p1.w;
+                fbc[0] = q0.x, fbc[1] = q0.y, fbc[2] = q0.z, fbc[3] = q0.w;
+                fbc[4] = q1.x, fbc[5] = q1.y, fbc[6] = q1.z, fbc[7] = q1.w;
+                const uint32_t *sp = D.singles + (size_t)b * 6;
+#pragma unroll
+                for (int e = 0; e < 6; ++e) sb[e] = sp[e];
+            }
+            uint32_t m0[27], m1[27];
+#pragma unroll
+            for (int cls = 0; cls < 2; ++cls) {
+                uint32_t *mm = cls ? m1 : m0;
+                // mm[x*9 + y*3 + z], x, y, z in {0, 1, 2 = any}
+#pragma unroll
+                for (int x = 0; x < 2; ++x)
+#pragma unroll
+                    for (int y = 0; y < 2; ++y) {
+#pragma unroll
+                        for (int z = 0; z < 2; ++z) mm[x * 9 + y * 3 + z] = (uint32_t)acc[cls][x][y][bb][z][j];
+                        mm[x * 9 + y * 3 + 2] = fab[cls * 4 + x * 2 + y];
+                    }
+#pragma unroll
+                for (int x = 0; x < 2; ++x)
+#pragma unroll
+                    for (int z = 0; z < 2; ++z) mm[x * 9 + 6 + z] = fac[cls * 4 + x * 2 + z];
+#pragma unroll
+                for (int y = 0; y < 2; ++y)
+#pragma unroll
+                    for (int z = 0; z < 2; ++z) mm[18 + y * 3 + z] = fbc[cls * 4 + y * 2 + z];
+                mm[0 * 9 + 8] = sa[cls * 3 + 0];
+                mm[1 * 9 + 8] = sa[cls * 3 + 1];
+                mm[18 + 0 * 3 + 2] = sb[cls * 3 + 0];
+                mm[18 + 1 * 3 + 2] = sb[cls * 3 + 1];
+                mm[18 + 6 + 0] = sc[cls][0];
+                mm[18 + 6 + 1] = sc[cls][1];
+                mm[26] = D.total[cls];
+                mobius3(mm);
+            }
+            const uint64_t combo = ((((uint64_t)a << P.combo_bits) | (uint64_t)b) << P.combo_bits) | (uint64_t)c;
+            score_and_push<27>(valid, m0, m1, combo, tau, tau_combo, P, st);
         }
 }
 
@@ -1401,6 +1580,16 @@ hipError_t launch_mfma2(const DevData &D, const ScanParams &P, TopState *st, int
     const dim3 grid((unsigned)(rounds * 8 * 64)), block(kBlock);
     if (fp4) hipLaunchKernelGGL((k_mfma2<true>), grid, block, 0, s, D, P, st, s0, s1, T);
     else hipLaunchKernelGGL((k_mfma2<false>), grid, block, 0, s, D, P, st, s0, s1, T);
+    return hipGetLastError();
+}
+
+hipError_t launch_mfma3(const DevData &D, const ScanParams &P, TopState *st, int a0, int a1, int T, hipStream_t s)
+{
+    if (a1 <= a0) return hipSuccess;
+    const int m = T - ((a0 + 1) >> 6);   // tiles from the first SNP's tile on (the most for a0)
+    if (m <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_mfma3, dim3((unsigned)m * (unsigned)(m + 1), (unsigned)(a1 - a0)), dim3(kBlock), 0, s, D, P,
+                       st, a0, T);
     return hipGetLastError();
 }
 

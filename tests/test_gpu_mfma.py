@@ -71,7 +71,7 @@ def test_mfma_distribution_and_chunks():
 def test_mfma_refuses_what_it_cannot_do():
     d, _ = synth(40, 100, 100, 51)
     with pytest.raises(fa.FiuError):
-        fa.GpuSearch(algorithm=fa.ALGO_MFMA).run(d, 3, fa.Distribution(40, 2), 5)
+        fa.GpuSearch(algorithm=fa.ALGO_MFMA).run(d, 4, fa.Distribution(40, 3), 5)
     rng = np.random.default_rng(52)
     geno = rng.integers(0, 3, size=(20, 150)).astype(np.int8)
     geno[0, 5] = 3   # a sample outside every row of SNP 0
