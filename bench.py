@@ -43,11 +43,11 @@ N_SNPS, N_CASES, N_CTRLS = 10_000, 8_000, 8_000
 PLANTED = (17, 4242, 9001)
 ORDER, OUTPUTS = 3, 100
 VALU_PEAK = 256 * 64 * 2.4e9          # int32 VALU lane-ops per second (MI355X)
-# Bytes fetched beyond L2 per triple by the order-3 marginal scan kernel:
-# rocprofv3 FETCH_SIZE per dispatch x 2.0 (the factor a plain 1 GiB copy
-# shows) + WRITE_SIZE x 1.0, divided by the triples of those dispatches
-# (profiles/k_scan3m_order3_10000x16000.md).
-TRAFFIC_BYTES_PER_TRIPLE = {"marginal": 1563.0}
+# Bytes fetched beyond L2 per triple by the order-3 scan kernels: rocprofv3
+# FETCH_SIZE per dispatch x 2.0 (the factor a plain 1 GiB copy shows) +
+# WRITE_SIZE x 1.0, divided by the triples of those dispatches
+# (profiles/k_scan3m_order3_10000x16000.md, profiles/k_mfma3_order3_10000x16000.md).
+TRAFFIC_BYTES_PER_TRIPLE = {"marginal": 1563.0, "mfma": 100.6}
 
 
 def alg_ops_per_combo(order, cases, ctrls):

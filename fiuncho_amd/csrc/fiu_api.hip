@@ -325,8 +325,9 @@ void enqueue_upload(fiu_dataset *d, StagedUpload &up, hipStream_t s)
     up.staging_d = nullptr;
 }
 
-void mi_constants(uint32_t cases, uint32_t ctrls, ScanParams &P)
+void mi_constants(uint32_t cases, uint32_t ctrls, bool complete, ScanParams &P)
 {
+    P.bounded = complete ? 1u : 0u;
     const double n = (double)cases + (double)ctrls;
     double hy = 0.0;
     if (cases) hy -= ((double)cases / n) * std::log((double)cases / n);
@@ -335,6 +336,8 @@ void mi_constants(uint32_t cases, uint32_t ctrls, ScanParams &P)
     P.hy64 = hy;
     P.hy = (float)hy;
     P.inv_n = (float)(1.0 / n);
+    P.frac_c = (float)((double)cases / n);
+    P.chi_scale = cases && ctrls ? (float)(n / ((double)cases * (double)ctrls)) : 0.0f;
     // bound on |screen - exact| (a few 1e-6 nats) with a 5x margin
     P.delta = 2e-5f;
 }
@@ -377,6 +380,25 @@ bool ensure_pair_table(fiu_dataset *d, hipStream_t s)
     check(launch_pair_table(d->dev, P, d->pair.p, s), "pair table launch");
     check(hipStreamSynchronize(s), "pair table sync");
     d->pair_ready = true;
+    return true;
+}
+
+bool ensure_fp4_image(fiu_dataset *d, hipStream_t s)
+{
+    std::lock_guard<std::mutex> lk(d->prep_mu);
+    if (d->fp4_ready) return true;
+    const uint64_t units = fp4_image_units(d->dev, 0) + fp4_image_units(d->dev, 1);
+    if (d->arena) {
+        if (units * 16 > d->arena->largest_free()) return false;
+    } else {
+        size_t free_b = 0, total_b = 0;
+        check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+        if (units * 16 + (64ull << 20) > free_b) return false;
+    }
+    d->fp4.alloc(units);
+    check(launch_fp4_image(d->dev, d->fp4.p, s), "fp4 image launch");
+    check(hipStreamSynchronize(s), "fp4 image sync");
+    d->fp4_ready = true;
     return true;
 }
 
@@ -503,16 +525,17 @@ void search(fiu_dataset *d, int order, int32_t step, int32_t offset, uint32_t ou
         // so by default it takes whole scans (a round-robin share of the
         // prefixes is masked, which wastes the other shares' work)
         if (order == 3 && algo == FIU_ALGO_MARGINAL && small_classes &&
-            (algo_req == FIU_ALGO_MFMA || (algo_req == FIU_ALGO_AUTO && step == 1)))
+            (algo_req == FIU_ALGO_MFMA || (algo_req == FIU_ALGO_AUTO && step == 1)) && ensure_fp4_image(d, s))
             algo = FIU_ALGO_MFMA;
     }
     st.algorithm = algo;
     // matrix-core scans walk super-blocks of 8 x 8 tiles of 64 x 64 pairs;
-    // fp4 operands accumulate in fp32, exact while every count is below 2^24
+    // fp4 operands accumulate in fp32, exact while every count is below 2^24;
+    // they come from the dataset's fp4 image (int8 when it does not fit)
     const int T = d->dev.n_pad / 64;
     const int64_t n_super = mfma2_superblocks(T);
-    const bool mfma_fp4 = !(cfg && (cfg->flags & FIU_FLAG_MFMA_INT8)) && d->cases < (1u << 24) &&
-                          d->ctrls < (1u << 24);
+    const bool mfma_fp4 = algo == FIU_ALGO_MFMA && !(cfg && (cfg->flags & FIU_FLAG_MFMA_INT8)) && small_classes &&
+                          ensure_fp4_image(d, s);
 
     const uint32_t cap = cfg && cfg->candidate_cap ? cfg->candidate_cap : (1u << 20);
     d->cand.alloc(cap);
@@ -574,7 +597,7 @@ void search(fiu_dataset *d, int order, int32_t step, int32_t offset, uint32_t ou
     P.combo_bits = bits;
     P.tag = 0;
     P.pair = d->pair_ready ? d->pair.p : nullptr;
-    mi_constants(d->cases, d->ctrls, P);
+    mi_constants(d->cases, d->ctrls, d->complete(), P);
 
     while (d->events.size() < 2 * chunks.size() + 2) {
         hipEvent_t e;
@@ -587,9 +610,12 @@ void search(fiu_dataset *d, int order, int32_t step, int32_t offset, uint32_t ou
         P.j1 = chunks[ci].j1;
         if (timed) check(hipEventRecord(d->events[ev], s), "event");
         if (algo == FIU_ALGO_MFMA && order == 2) {
-            check(launch_mfma2(d->dev, P, d->state.p, chunks[ci].j0, chunks[ci].j1, T, mfma_fp4, s), "scan launch");
+            check(launch_mfma2(d->dev, P, d->state.p, chunks[ci].j0, chunks[ci].j1, T, mfma_fp4 ? d->fp4.p : nullptr,
+                               s),
+                  "scan launch");
         } else if (algo == FIU_ALGO_MFMA) {
-            check(launch_mfma3(d->dev, P, d->state.p, (int)chunks[ci].j0, (int)chunks[ci].j1, T, s), "scan launch");
+            check(launch_mfma3(d->dev, P, d->state.p, d->fp4.p, (int)chunks[ci].j0, (int)chunks[ci].j1, T, s),
+                  "scan launch");
         } else if (algo == FIU_ALGO_MARGINAL) {
             check(launch_scan_marginal(order, d->dev, P, d->state.p, s), "scan launch");
         } else {
@@ -838,7 +864,7 @@ fiu_status fiu_contingency_batch(const fiu_dataset *ds, const int32_t *combos, u
         check(launch_contingency_batch(ds->dev, dc.p, order, n, cts, dout.p, ds->stream), "contingency");
         if (out_mi) {
             ScanParams P{};
-            mi_constants(ds->cases, ds->ctrls, P);
+            mi_constants(ds->cases, ds->ctrls, false, P);
             dmi.alloc(n);
             check(launch_mi_batch(dout.p, n, cts, P.n64, P.hy64, dmi.p, ds->stream), "mi");
             check(hipMemcpyAsync(out_mi, dmi.p, n * 4, hipMemcpyDeviceToHost, ds->stream), "copy back");
@@ -866,7 +892,7 @@ fiu_status fiu_mi_batch(const uint32_t *tables, size_t n, uint16_t order, uint32
         dm.alloc(n);
         check(hipMemcpyAsync(dt.p, tables, n * 2 * cts * 4, hipMemcpyHostToDevice, s), "copy");
         ScanParams P{};
-        mi_constants(cases, ctrls, P);
+        mi_constants(cases, ctrls, false, P);
         check(launch_mi_batch(dt.p, n, cts, P.n64, P.hy64, dm.p, s), "mi");
         check(hipMemcpyAsync(out_mi, dm.p, n * 4, hipMemcpyDeviceToHost, s), "copy back");
         check(hipStreamSynchronize(s), "sync");

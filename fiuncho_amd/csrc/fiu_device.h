@@ -66,6 +66,9 @@ struct ScanParams {
     float delta;             // screening margin on MI
     float hy;                // H(Y) in nats (from FP64)
     float inv_n;             // 1 / (cases + ctrls)
+    float frac_c;            // cases / (cases + ctrls)
+    float chi_scale;         // (cases + ctrls) / (cases * ctrls), 0 when a class is empty
+    uint32_t bounded;        // 1: complete genotype tables, the MI upper bound applies
     uint32_t tag;
     double hy64, n64;        // H(Y) and cases+ctrls in FP64
     const uint32_t *pair;    // marginal pair table (order 3), or null

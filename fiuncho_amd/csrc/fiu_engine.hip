@@ -393,7 +393,7 @@ void Engine::activate(Job *j, int slot)
     P.tag = (uint32_t)slot;
     P.pair = j->kind == kKindScan3m ? d->pair.p : nullptr;
     P.hist = nullptr;
-    mi_constants(d->cases, d->ctrls, P);
+    mi_constants(d->cases, d->ctrls, d->complete(), P);
     B.st = states_d_.p + slot;
     B.top = top_d_.p + (size_t)slot * kMaxOutputs;
     B.outputs = j->outputs;

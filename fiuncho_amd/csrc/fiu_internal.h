@@ -125,8 +125,10 @@ int bit_width(uint64_t x);
 // = offset (mod step).
 uint64_t count_combinations(uint64_t n, int m, uint64_t step, uint64_t offset);
 
-// H(Y), 1/N and the screening margin of a dataset's class sizes.
-void mi_constants(uint32_t cases, uint32_t ctrls, ScanParams &P);
+// H(Y), 1/N and the screening margin of a dataset's class sizes.  `complete`:
+// every sample has a genotype in {0, 1, 2} at every SNP, so that a table's
+// cells sum to the class sizes and the MI upper bound applies.
+void mi_constants(uint32_t cases, uint32_t ctrls, bool complete, ScanParams &P);
 
 }  // namespace fiu
 
@@ -150,14 +152,21 @@ struct fiu_dataset {
     fiu::DBuf<uint32_t> redo;
     fiu::DBuf<uint32_t> pair;        // marginal pair table (order-3 marginal scans)
     fiu::DBuf<uint32_t> hist;        // screening-MI histogram of the threshold sampling pass
+    fiu::DBuf<uint4> fp4;            // fp4 image of genotype rows 0/1 (matrix-core scans)
+    bool fp4_ready = false;
     bool pair_ready = false;
     std::vector<hipEvent_t> events;
     std::mutex prep_mu;              // binom / pair table preparation
+
+    // every sample has a genotype in {0, 1, 2} at every SNP: the cells of a
+    // table sum to the class sizes
+    bool complete() const { return dev.uniform && dev.total[0] == cases && dev.total[1] == ctrls; }
 
     void use_arena(fiu::Arena *a)
     {
         arena = a;
         bits.arena = singles.arena = flags.arena = binom.arena = redo.arena = pair.arena = hist.arena = a;
+        fp4.arena = a;
         cand.arena = top.arena = a;
         state.arena = a;
     }
@@ -211,6 +220,9 @@ void ensure_binom(fiu_dataset *d, int m, hipStream_t s);
 // (hipMalloc datasets) or in the arena.
 bool ensure_pair_table(fiu_dataset *d, hipStream_t s);
 inline size_t pair_table_bytes(size_t snps) { return snps < 2 ? 0 : snps * (snps - 1) / 2 * 32; }
+// fp4 image of genotype rows 0 and 1 (4 bits per sample and row); false when
+// it does not fit.
+bool ensure_fp4_image(fiu_dataset *d, hipStream_t s);
 
 // Top-N entries -> ascending SNP indices (order per entry) and float MI.
 void unpack_results(const Cand *top, uint32_t count, int order, int bits, int32_t *out_combos, float *out_mi);

@@ -11,13 +11,20 @@ hipError_t launch_tau_from_hist(const uint32_t *hist, uint32_t N, float delta, T
 hipError_t launch_pair_table(const DevData &D, const ScanParams &P, uint32_t *pair_out, hipStream_t s);
 // Order 2 on the matrix cores: the pair triangle of T x T 64-SNP tiles is cut
 // into super-blocks of 8 x 8 tiles; a launch scores super-blocks [s0, s1)
-// into the top-N state, with fp4 (e2m1) or int8 operands.
+// into the top-N state, with fp4 (e2m1) operands from the dataset's fp4 image
+// or, when `fp4` is null, int8 operands expanded from the bit tables.
 int64_t mfma2_superblocks(int T);
+// fp4 image of genotype rows 0 and 1 (operands of the matrix-core scans):
+// uint4 units per class, and the kernel that writes both classes (class 1
+// after class 0).
+uint64_t fp4_image_units(const DevData &D, int cls);
+hipError_t launch_fp4_image(const DevData &D, uint4 *out, hipStream_t s);
 // Order 3 on the matrix cores (fp4): first SNPs [a0, a1), every (b, c) past
-// them, with the pair table in P.pair.
-hipError_t launch_mfma3(const DevData &D, const ScanParams &P, TopState *st, int a0, int a1, int T, hipStream_t s);
-hipError_t launch_mfma2(const DevData &D, const ScanParams &P, TopState *st, int64_t s0, int64_t s1, int T, bool fp4,
+// them, with the pair table in P.pair and the dataset's fp4 image.
+hipError_t launch_mfma3(const DevData &D, const ScanParams &P, TopState *st, const uint4 *fp4, int a0, int a1, int T,
                         hipStream_t s);
+hipError_t launch_mfma2(const DevData &D, const ScanParams &P, TopState *st, int64_t s0, int64_t s1, int T,
+                        const uint4 *fp4, hipStream_t s);
 hipError_t launch_merge(TopState *st, Cand *top, const Cand *cand, uint32_t N, uint32_t cap, uint32_t chunk_id,
                         uint32_t *redo, hipStream_t s);
 hipError_t launch_transpose(const uint64_t *src, uint4 *dst, int n, int n_pad, int cls, uint64_t cw, uint64_t tw,

@@ -21,6 +21,7 @@
 #include <math.h>
 
 #include <type_traits>
+#include <utility>
 
 namespace fiu {
 
@@ -143,6 +144,34 @@ __device__ __forceinline__ float mi_screen(const uint32_t *a0, const uint32_t *a
     return fmaf(s, inv_n * 0.69314718f, hy);
 }
 
+// Upper bound on the MI in nats, one reciprocal per cell and no logarithm:
+// MI = KL(P_XY || P_X P_Y) <= ln(1 + chi2/N) <= chi2/N (Jensen's inequality
+// on E_P[P_XY / (P_X P_Y)]), and with two classes
+//   chi2/N = N/(n_c n_t) sum_x d_x^2 / n_x,  d_x = n_cx - n_x n_c/N.
+// FP32 error: below 2e-6 relative plus 5e-7 absolute (each |d_x| error is
+// under 1.2e-7 (n_x n_c/N + |d_x|), and sum_x |d_x| <= 2 min(n_c, n_t)).
+template <int CELLS>
+__device__ __forceinline__ float mi_bound(const uint32_t *a0, const uint32_t *a1, float frac_c, float chi_scale)
+{
+    float s = 0.0f;
+#pragma unroll
+    for (int x = 0; x < CELLS; ++x) {
+        const float fx = (float)(a0[x] + a1[x]);
+        const float d = fmaf(-fx, frac_c, (float)a0[x]);
+        s = fmaf(d * d, __builtin_amdgcn_rcpf(fmaxf(fx, 1.0f)), s);
+    }
+    return s * chi_scale;
+}
+
+// The bound rejects a combination when even its bound, raised by far more
+// than the bound's own FP32 error plus the screen's (2e-6), stays below the
+// screening threshold: the screen would then reject it as well, so the set
+// of combinations that reach the exact MI is unchanged.
+__device__ __forceinline__ bool bound_passes(float bound, float tau, float delta)
+{
+    return fmaf(bound, 1.0001f, 4e-6f) >= tau - delta;
+}
+
 // p*ln(p) in FP64 for p > 0, else 0 (kept out of line: it is only reached for
 // the few combinations that pass the FP32 screen).
 __device__ __noinline__ double plogp(double p)
@@ -208,7 +237,7 @@ __device__ __forceinline__ void score_and_push(bool valid, const uint32_t *a0, c
 {
     bool want = false;
     float mf = 0.0f;
-    if (valid) {
+    if (valid && (!P.bounded || bound_passes(mi_bound<CELLS>(a0, a1, P.frac_c, P.chi_scale), tau, P.delta))) {
         const float ms = mi_screen<CELLS>(a0, a1, P.hy, P.inv_n);
         if (ms >= tau - P.delta) {
             mf = (float)mi_exact<CELLS>(a0, a1, P.n64, P.hy64);
@@ -827,17 +856,34 @@ __device__ __forceinline__ uint32_t byte_fp4(uint32_t b)
     return b << 1;
 }
 
-// 128 bits of a quad as 64 bytes of e2m1 indicators in one LDS row: sample
-// 32 e + j (bit j of word e) is nibble 32 e + j.
-__device__ __forceinline__ void expand_quad_fp4(const uint4 &q, int8_t *row)
+// fp4 image of a dataset's genotype rows 0 and 1 (built once per dataset):
+// the 128 samples of quad q, genotype g, SNP s of a class are 64 bytes = 4
+// uint4 at class_off[c] + 4 ((2 q + g) n_pad + s), nibble j = sample j, so
+// the matrix-core scans copy operand rows instead of expanding bits.
+__global__ void k_fp4_image(const uint4 *__restrict__ bits, DevData D, uint4 *__restrict__ out, uint64_t off0,
+                            uint64_t off1)
 {
+    const uint64_t id = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t per0 = 2ull * D.quads[0] * D.n_pad, total = per0 + 2ull * D.quads[1] * D.n_pad;
+    if (id >= total) return;
+    const int cls = id < per0 ? 0 : 1;
+    const uint64_t v = cls ? id - per0 : id;
+    const int s = (int)(v % D.n_pad);
+    const uint64_t qg = v / D.n_pad;
+    const int g = (int)(qg & 1), q = (int)(qg >> 1);
+    const uint4 w = bits[D.class_off[cls] + ((uint64_t)q * 3 + g) * D.n_pad + s];
+    uint4 *dst = out + (cls ? off1 : off0) + 4 * v;
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-        const uint32_t v = comp(q, e);
-        *(uint4 *)(row + 16 * e) = make_uint4(byte_fp4(v & 255u), byte_fp4(v >> 8 & 255u),
-                                              byte_fp4(v >> 16 & 255u), byte_fp4(v >> 24));
+        const uint32_t x = comp(w, e);
+        dst[e] = make_uint4(byte_fp4(x & 255u), byte_fp4(x >> 8 & 255u), byte_fp4(x >> 16 & 255u), byte_fp4(x >> 24));
     }
 }
+
+struct Fp4Image {
+    const uint4 *base;
+    uint64_t class_off[2];   // uint4 offset of each class
+};
 
 // Four bits of v (from bit `lo`) as four bytes 0/1: n * 0x00204081 puts bit i
 // of the nibble n at bit 8i (the shifted copies do not overlap), the mask
@@ -880,11 +926,13 @@ __device__ __forceinline__ void tri_tile(int64_t u, int T, int &ta, int &tb)
 // launch covers super-blocks [s0, s1) of the triangle of S x S super-blocks
 // (8 x 8 tiles each): block v runs on XCD v % 8, and the 64 blocks an XCD
 // holds at once are one super-block, whose 8 + 8 row panels they share in L2.
-// Each stage loads one 128-bit quad of the bit tables per row (A: 64 SNPs x
-// genotypes 0/1, B likewise) and expands it to the operand format in LDS.
+// Each stage covers one 128-sample quad of 256 operand rows (A: 64 SNPs x
+// genotypes 0/1, B likewise).  fp4: the rows are copied from the dataset's
+// fp4 image (four 16-byte units per thread); int8: each thread loads one
+// 128-bit quad of the bit tables and expands it to bytes in LDS.
 template <bool FP4>
 __global__ void __launch_bounds__(kBlock)
-k_mfma2(const DevData D, const ScanParams P, TopState *st, int64_t s0, int64_t s1, int T)
+k_mfma2(const DevData D, const ScanParams P, TopState *st, const Fp4Image F4, int64_t s0, int64_t s1, int T)
 {
     typedef MfmaFmt<FP4> F;
     typedef typename std::conditional<FP4, v16f, v16i>::type Acc;
@@ -901,9 +949,14 @@ k_mfma2(const DevData D, const ScanParams P, TopState *st, int64_t s0, int64_t s
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6), wa = w >> 1, wb = w & 1;
     const int np = D.n_pad;
     const int a0 = ta * kMfmaTile, b0 = tb * kMfmaTile;
-    // this thread's staging row: operand m (A / B), genotype g, SNP
+    // int8 staging: this thread's row, operand m (A / B), genotype g, SNP
     const int m_ld = tid >> 7, row_ld = tid & 127, g_ld = row_ld >> 6;
     const int snp_ld = (m_ld ? b0 : a0) + (row_ld & 63);
+    // fp4 staging: segment seg of row q4 of each of the four 64-row panels;
+    // offsets in uint4 within a quad of the image (< 8 n_pad)
+    const int seg = tid & 3, q4 = tid >> 2;
+    const int ua0 = 4 * (a0 + q4) + seg, ua1 = 4 * (np + a0 + q4) + seg;
+    const int ub0 = 4 * (b0 + q4) + seg, ub1 = 4 * (np + b0 + q4) + seg;
     Acc acc[2][2][2];
 #pragma unroll
     for (int c = 0; c < 2; ++c)
@@ -911,25 +964,51 @@ k_mfma2(const DevData D, const ScanParams P, TopState *st, int64_t s0, int64_t s
         for (int g = 0; g < 2; ++g)
 #pragma unroll
             for (int h = 0; h < 2; ++h) acc[c][g][h] = (Acc){0};
-    auto expand = [&](const uint4 &q, int buf) {
-        if constexpr (FP4) expand_quad_fp4(q, lds[buf][m_ld][row_ld]);
-        else expand_quad(q, lds[buf][m_ld][row_ld]);
-    };
 #pragma unroll
     for (int cls = 0; cls < 2; ++cls) {
         const int iters = D.quads[cls];
         const uint4 *src = D.bits + D.class_off[cls] + (uint64_t)g_ld * np + snp_ld;   // quad q at + q * 3 * np
-        // quads it+1 and it+2 wait in registers, quad it+3 is loaded during
-        // stage it: two stages of latency hiding for every load
-        uint4 q1 = make_uint4(0, 0, 0, 0), q2 = q1;
-        expand(src[0], 0);
-        if (iters > 1) q1 = src[(uint64_t)3 * np];
-        if (iters > 2) q2 = src[(uint64_t)6 * np];
+        const uint4 *img = F4.base + F4.class_off[cls];                                // quad q at + q * 8 * np
+        // int8: quads it+1 and it+2 wait in registers, quad it+3 is loaded
+        // during stage it; fp4: the units of stage it+1 are loaded during
+        // stage it
+        uint4 q1 = make_uint4(0, 0, 0, 0), q2 = q1, p0 = q1, p1 = q1, p2 = q1, p3 = q1;
+        auto gload = [&](int q) {
+            const uint4 *s = img + (int64_t)q * 8 * np;
+            p0 = s[ua0];
+            p1 = s[ua1];
+            p2 = s[ub0];
+            p3 = s[ub1];
+        };
+        auto stage = [&](int buf) {
+            if constexpr (FP4) {
+                *(uint4 *)&lds[buf][0][q4][seg * 16] = p0;
+                *(uint4 *)&lds[buf][0][64 + q4][seg * 16] = p1;
+                *(uint4 *)&lds[buf][1][q4][seg * 16] = p2;
+                *(uint4 *)&lds[buf][1][64 + q4][seg * 16] = p3;
+            } else {
+                expand_quad(q1, lds[buf][m_ld][row_ld]);
+            }
+        };
+        if constexpr (FP4) {
+            gload(0);
+        } else {
+            q1 = src[0];
+        }
+        stage(0);
+        if constexpr (!FP4) {
+            if (iters > 1) q1 = src[(uint64_t)3 * np];
+            if (iters > 2) q2 = src[(uint64_t)6 * np];
+        }
         __syncthreads();
         for (int it = 0; it < iters; ++it) {
             const int buf = it & 1;
             uint4 q3 = make_uint4(0, 0, 0, 0);
-            if (it + 3 < iters) q3 = src[(uint64_t)(it + 3) * 3 * np];
+            if constexpr (FP4) {
+                if (it + 1 < iters) gload(it + 1);
+            } else {
+                if (it + 3 < iters) q3 = src[(uint64_t)(it + 3) * 3 * np];
+            }
 #pragma unroll
             for (int s = 0; s < F::substeps; ++s) {
                 // 16 operand bytes per lane: 16 int8 or 32 fp4 samples of row r
@@ -956,7 +1035,7 @@ k_mfma2(const DevData D, const ScanParams P, TopState *st, int64_t s0, int64_t s
                         }
                     }
             }
-            if (it + 1 < iters) expand(q1, buf ^ 1);
+            if (it + 1 < iters) stage(buf ^ 1);
             __syncthreads();
             q1 = q2;
             q2 = q3;
@@ -1015,13 +1094,25 @@ k_mfma2(const DevData D, const ScanParams P, TopState *st, int64_t s0, int64_t s
 // 32-c halves.
 // ---------------------------------------------------------------------------
 typedef float v4f __attribute__((ext_vector_type(4)));
+
+// f(integral_constant<int, 0>) ... f(integral_constant<int, N - 1>), inline.
+template <class F, int... I>
+__device__ __forceinline__ void static_for_impl(F &&f, std::integer_sequence<int, I...>)
+{
+    (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, class F> __device__ __forceinline__ void static_for(F &&f)
+{
+    static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
 constexpr int kM3Rows = 2 * 64 + 2 * 32 + 2;   // b rows (y, b), c rows (z, c), a rows (x)
 constexpr int kM3Pitch = 80;                   // 64 bytes of fp4 + 16
 
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2)))   // fits in 256 registers
-k_mfma3(const DevData D, const ScanParams P, TopState *st, int a0, int T)
+
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 2)))   // 256 registers, no spills
+k_mfma3(const DevData D, const ScanParams P, TopState *st, const Fp4Image F, int a0, int T)
 {
-    __shared__ __attribute__((aligned(16))) int8_t lds[2][kM3Rows][kM3Pitch];
+    __shared__ __attribute__((aligned(16))) int8_t lds[2][kM3Rows + 1][kM3Pitch];   // + a spare row
     const int a = a0 + (int)blockIdx.y;
     if (a >= D.n - 2) return;
     const int ta = (a + 1) >> 6, m = T - ta;
@@ -1034,11 +1125,18 @@ k_mfma3(const DevData D, const ScanParams P, TopState *st, int a0, int T)
     const int tid = threadIdx.x, lane = tid & 63;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6), wb = w >> 1, wc = w & 1;
     const int np = D.n_pad;
-    // staging row of this thread: b rows 0..127, c rows 128..191, a rows 192..193
-    int snp_ld = -1, g_ld = 0;
-    if (tid < 128) snp_ld = b0 + (tid & 63), g_ld = tid >> 6;
-    else if (tid < 192) snp_ld = c0 + ((tid - 128) & 31), g_ld = (tid - 128) >> 5;
-    else if (tid < kM3Rows) snp_ld = a, g_ld = tid - 192;
+    // staging: 4 x kM3Rows 16-byte units per stage.  Unit v = tid + 256 k is
+    // row v / 4 (b rows 0..127 = (y, b), c rows 128..191 = (z, c), a rows
+    // 192..193 = x), segment v % 4; units 0..767 (k < 3) are b and c rows,
+    // the 8 units of the a rows are taken by threads 0..7 (k = 3; the other
+    // threads load the same 8 units, broadcast, and store them to a spare
+    // row).  Offsets in uint4 within a quad (< 8 n_pad).
+    const int seg = tid & 3, q4 = tid >> 2;                     // q4 = 0..63
+    const int u0 = 4 * (b0 + q4) + seg;                         // row q4: genotype 0 of b0 + q4
+    const int u1 = 4 * (np + b0 + q4) + seg;                    // row 64 + q4: genotype 1
+    const int u2 = 4 * ((q4 >> 5) * np + c0 + (q4 & 31)) + seg; // row 128 + q4: (z, c)
+    const int u3 = 4 * ((q4 & 1) * np + a) + seg;               // a row x = q4 & 1
+    const int row3 = tid < 8 ? 192 + q4 : kM3Rows;              // threads 8.. write the spare row
     v4f acc[2][2][2][2][2];   // [class][x][y][b block][z]
 #pragma unroll
     for (int c = 0; c < 2; ++c)
@@ -1054,20 +1152,28 @@ k_mfma3(const DevData D, const ScanParams P, TopState *st, int a0, int T)
 #pragma unroll
     for (int cls = 0; cls < 2; ++cls) {
         const int iters = D.quads[cls];
-        const uint4 *src = snp_ld >= 0 ? D.bits + D.class_off[cls] + (uint64_t)g_ld * np + snp_ld : nullptr;
-        // quads it+1 and it+2 wait in registers, quad it+3 is loaded during
-        // stage it: two stages of latency hiding for every load
-        uint4 q1 = make_uint4(0, 0, 0, 0), q2 = q1;
-        if (src) {
-            expand_quad_fp4(src[0], lds[0][tid]);
-            if (iters > 1) q1 = src[(uint64_t)3 * np];
-            if (iters > 2) q2 = src[(uint64_t)6 * np];
-        }
+        const uint4 *img = F.base + F.class_off[cls];
+        const int64_t qstride = 8 * (int64_t)np;   // uint4 per quad: 2 genotypes x n_pad x 4
+        uint4 p0, p1, p2, p3;
+        auto gload = [&](int q) {
+            const uint4 *src = img + q * qstride;
+            p0 = src[u0];
+            p1 = src[u1];
+            p2 = src[u2];
+            p3 = src[u3];
+        };
+        auto lstore = [&](int buf) {
+            *(uint4 *)&lds[buf][q4][seg * 16] = p0;
+            *(uint4 *)&lds[buf][64 + q4][seg * 16] = p1;
+            *(uint4 *)&lds[buf][128 + q4][seg * 16] = p2;
+            *(uint4 *)&lds[buf][row3][seg * 16] = p3;
+        };
+        gload(0);
+        lstore(0);
         __syncthreads();
         for (int it = 0; it < iters; ++it) {
             const int buf = it & 1;
-            uint4 q3 = make_uint4(0, 0, 0, 0);
-            if (src && it + 3 < iters) q3 = src[(uint64_t)(it + 3) * 3 * np];
+            if (it + 1 < iters) gload(it + 1);
             v4i ma[2], fb[2][2], fc[2];
 #pragma unroll
             for (int x = 0; x < 2; ++x) ma[x] = *(const v4i *)&lds[buf][192 + x][kk];
@@ -1092,10 +1198,8 @@ k_mfma3(const DevData D, const ScanParams P, TopState *st, int a0, int T)
                                 a8, c8, acc[cls][x][y][bb][z], 4, 4, 0, 0, 0, 0);
                         }
                     }
-            if (src && it + 1 < iters) expand_quad_fp4(q1, lds[buf ^ 1][tid]);
+            if (it + 1 < iters) lstore(buf ^ 1);
             __syncthreads();
-            q1 = q2;
-            q2 = q3;
         }
     }
     // epilogue: lane holds column c = c0 + wc*16 + lane%16 and, per b block
@@ -1117,10 +1221,9 @@ k_mfma3(const DevData D, const ScanParams P, TopState *st, int a0, int T)
         fac[4] = u1.x, fac[5] = u1.y, fac[6] = u1.z, fac[7] = u1.w;
     }
     const int64_t rank_a = (int64_t)a * (2 * (int64_t)n - a - 1) / 2;   // rank of the prefix (a, a+1)
-#pragma unroll
-    for (int bb = 0; bb < 2; ++bb)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
+    // eight instances (bb, j), so that every accumulator index is a constant
+    static_for<8>([&](auto BJ) __attribute__((always_inline)) {
+            constexpr int bb = decltype(BJ)::value >> 2, j = decltype(BJ)::value & 3;
             const int b = b0 + wb * 32 + bb * 16 + 4 * (lane >> 4) + j;
             const int64_t rel = rank_a + (b - a - 1) - P.offset;
             const bool valid = c_ok && b > a && c > b && rel >= 0 && rel % P.step == 0;
@@ -1170,7 +1273,7 @@ k_mfma3(const DevData D, const ScanParams P, TopState *st, int a0, int T)
             }
             const uint64_t combo = ((((uint64_t)a << P.combo_bits) | (uint64_t)b) << P.combo_bits) | (uint64_t)c;
             score_and_push<27>(valid, m0, m1, combo, tau, tau_combo, P, st);
-        }
+        });
 }
 
 // ---------------------------------------------------------------------------
@@ -1571,25 +1674,45 @@ int64_t mfma2_superblocks(int T)
     return S * (S + 1) / 2;
 }
 
-hipError_t launch_mfma2(const DevData &D, const ScanParams &P, TopState *st, int64_t s0, int64_t s1, int T, bool fp4,
-                        hipStream_t s)
+hipError_t launch_mfma2(const DevData &D, const ScanParams &P, TopState *st, int64_t s0, int64_t s1, int T,
+                        const uint4 *fp4, hipStream_t s)
 {
     if (s1 <= s0) return hipSuccess;
     // 64 blocks per super-block, 8 super-blocks (one per XCD) per round
     const int64_t rounds = (s1 - s0 + 7) / 8;
     const dim3 grid((unsigned)(rounds * 8 * 64)), block(kBlock);
-    if (fp4) hipLaunchKernelGGL((k_mfma2<true>), grid, block, 0, s, D, P, st, s0, s1, T);
-    else hipLaunchKernelGGL((k_mfma2<false>), grid, block, 0, s, D, P, st, s0, s1, T);
+    Fp4Image F;
+    F.base = fp4;
+    F.class_off[0] = 0;
+    F.class_off[1] = fp4_image_units(D, 0);
+    if (fp4) hipLaunchKernelGGL((k_mfma2<true>), grid, block, 0, s, D, P, st, F, s0, s1, T);
+    else hipLaunchKernelGGL((k_mfma2<false>), grid, block, 0, s, D, P, st, F, s0, s1, T);
     return hipGetLastError();
 }
 
-hipError_t launch_mfma3(const DevData &D, const ScanParams &P, TopState *st, int a0, int a1, int T, hipStream_t s)
+uint64_t fp4_image_units(const DevData &D, int cls) { return 8ull * D.quads[cls] * D.n_pad; }
+
+hipError_t launch_fp4_image(const DevData &D, uint4 *out, hipStream_t s)
+{
+    const uint64_t total = 2ull * (D.quads[0] + D.quads[1]) * D.n_pad;
+    if (!total) return hipSuccess;
+    hipLaunchKernelGGL(k_fp4_image, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, D.bits, D, out, 0ull,
+                       fp4_image_units(D, 0));
+    return hipGetLastError();
+}
+
+hipError_t launch_mfma3(const DevData &D, const ScanParams &P, TopState *st, const uint4 *fp4, int a0, int a1, int T,
+                        hipStream_t s)
 {
     if (a1 <= a0) return hipSuccess;
     const int m = T - ((a0 + 1) >> 6);   // tiles from the first SNP's tile on (the most for a0)
     if (m <= 0) return hipSuccess;
+    Fp4Image F;
+    F.base = fp4;
+    F.class_off[0] = 0;
+    F.class_off[1] = fp4_image_units(D, 0);
     hipLaunchKernelGGL(k_mfma3, dim3((unsigned)m * (unsigned)(m + 1), (unsigned)(a1 - a0)), dim3(kBlock), 0, s, D, P,
-                       st, a0, T);
+                       st, F, a0, T);
     return hipGetLastError();
 }
 

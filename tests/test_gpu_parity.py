@@ -211,15 +211,15 @@ def test_search_vs_oracle_fp64(snps, cases, ctrls, order, outputs, seed):
     print(f"case: synthetic {snps}x{cases}+{ctrls} k={order}", flush=True)
     d, od = synth(snps, cases, ctrls, seed)
     oc, om, _ = O.search(od, order, outputs, threads=4, mi_mode=2)
-    algos = (fa.ALGO_AUTO, fa.ALGO_DIRECT, fa.ALGO_MARGINAL) + ((fa.ALGO_MFMA,) if order == 2 else ())
+    algos = (fa.ALGO_AUTO, fa.ALGO_DIRECT, fa.ALGO_MARGINAL) + ((fa.ALGO_MFMA,) if order <= 3 else ())
     for algorithm in algos:
         s = fa.GpuSearch(algorithm=algorithm)
         combos, mi = s.run_arrays(d, order, fa.Distribution(snps, order - 1), outputs)
         used = s.last_stats.algorithm
         if algorithm == fa.ALGO_DIRECT or order > 3:
             assert used == fa.ALGO_DIRECT
-        elif order == 2 and algorithm != fa.ALGO_MARGINAL:
-            assert used == fa.ALGO_MFMA   # the order-2 default
+        elif algorithm != fa.ALGO_MARGINAL:
+            assert used == fa.ALGO_MFMA   # the default of whole order-2 and order-3 scans
         else:
             assert used == fa.ALGO_MARGINAL
         assert combos.tolist() == oc.tolist(), (algorithm, used)
@@ -291,6 +291,26 @@ def test_incomplete_genotypes_use_exact_counts():
         assert combos.tolist() == oc.tolist() and np.array_equal(mi, om)
     with pytest.raises(fa.FiuError):
         fa.GpuSearch(algorithm=fa.ALGO_MARGINAL).run(d, 2, fa.Distribution(20, 1), 5)
+
+
+def test_uncalled_sample_keeps_exact_mi():
+    # one sample per class uncalled at every SNP: every SNP covers the same
+    # samples (the marginal and matrix-core scans apply) but the tables sum
+    # to less than the class sizes, so the MI is over sub-probabilities and
+    # the chi-square upper bound must not be used to reject combinations
+    rng = np.random.default_rng(62)
+    geno = rng.integers(0, 3, size=(40, 300)).astype(np.int8)
+    geno[:, 7] = 3
+    geno[:, 250] = 3
+    is_ctrl = (np.arange(300) >= 160).astype(np.uint8)
+    od = O.pack(geno, is_ctrl)
+    d = fa.Dataset.from_words(od.words, 40, od.cases_words, od.ctrls_words, 160, 140)
+    for order in (2, 3):
+        print(f"case: uncalled sample k={order}", flush=True)
+        oc, om, _ = O.search(od, order, 60, threads=2, mi_mode=2)
+        for algorithm in (fa.ALGO_AUTO, fa.ALGO_MARGINAL, fa.ALGO_DIRECT):
+            combos, mi = fa.GpuSearch(algorithm=algorithm).run_arrays(d, order, fa.Distribution(40, order - 1), 60)
+            assert combos.tolist() == oc.tolist() and np.array_equal(mi, om), (order, algorithm)
 
 
 def test_small_chunks_and_candidate_overflow():
