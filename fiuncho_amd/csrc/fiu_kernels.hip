@@ -1105,14 +1105,47 @@ template <int N, class F> __device__ __forceinline__ void static_for(F &&f)
 {
     static_for_impl(f, std::make_integer_sequence<int, N>{});
 }
-constexpr int kM3Rows = 2 * 64 + 2 * 32 + 2;   // b rows (y, b), c rows (z, c), a rows (x)
-constexpr int kM3Pitch = 80;                   // 64 bytes of fp4 + 16
+// A stage (one 128-sample quad of one class) in LDS: 13 groups of 16 rows x
+// 64 bytes -- groups 0..7 the b rows (y, b), 8..11 the c rows (z, c), 12 the
+// two a rows (x).  Segment s (16 bytes, 32 samples) of row r of a group is
+// stored at segment s ^ m3_swz(r): the 16 rows one fragment read touches at
+// one segment then fall on distinct banks (rows 4 apart share a bank
+// quarter and differ in m3_swz; rows 2 apart likewise for 32 banks).
+constexpr int kM3Group = 16 * 64;
+constexpr int kM3Stage = 13 * kM3Group;
+constexpr int kM3Ring = 4;   // stages resident: the one read + up to 3 loading
+__device__ __forceinline__ int m3_swz(int r) { return ((r >> 1) ^ (r >> 3)) & 3; }
 
+// Direct-to-LDS load of 16 bytes per lane: lane l's bytes land at
+// lds_group + 16 l (lds_group wave-uniform).
+__device__ __forceinline__ void dma16(const uint4 *src, int8_t *lds_group)
+{
+    __builtin_amdgcn_global_load_lds((const void *)src, (__attribute__((address_space(3))) void *)lds_group, 16, 0, 0);
+}
+
+// Wait until at most N vector-memory operations of this wave are pending
+// (vmcnt N; lgkm and exp counters not waited on).
+template <int N> __device__ __forceinline__ void wait_vmcnt()
+{
+    static_assert(N >= 0 && N < 16, "vmcnt immediate");
+    __builtin_amdgcn_s_waitcnt(0x0f70 | N);
+}
+
+// Workgroup barrier with no memory fence (a fence would drain every pending
+// direct-to-LDS load, including the ones meant to stay in flight); the
+// waits are explicit, the empty asm statements keep the compiler from moving
+// memory accesses across the barrier.
+__device__ __forceinline__ void stage_barrier()
+{
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+}
 
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 2)))   // 256 registers, no spills
 k_mfma3(const DevData D, const ScanParams P, TopState *st, const Fp4Image F, int a0, int T)
 {
-    __shared__ __attribute__((aligned(16))) int8_t lds[2][kM3Rows + 1][kM3Pitch];   // + a spare row
+    __shared__ __attribute__((aligned(1024))) int8_t lds[kM3Ring][kM3Stage];
     const int a = a0 + (int)blockIdx.y;
     if (a >= D.n - 2) return;
     const int ta = (a + 1) >> 6, m = T - ta;
@@ -1125,18 +1158,18 @@ k_mfma3(const DevData D, const ScanParams P, TopState *st, const Fp4Image F, int
     const int tid = threadIdx.x, lane = tid & 63;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6), wb = w >> 1, wc = w & 1;
     const int np = D.n_pad;
-    // staging: 4 x kM3Rows 16-byte units per stage.  Unit v = tid + 256 k is
-    // row v / 4 (b rows 0..127 = (y, b), c rows 128..191 = (z, c), a rows
-    // 192..193 = x), segment v % 4; units 0..767 (k < 3) are b and c rows,
-    // the 8 units of the a rows are taken by threads 0..7 (k = 3; the other
-    // threads load the same 8 units, broadcast, and store them to a spare
-    // row).  Offsets in uint4 within a quad (< 8 n_pad).
-    const int seg = tid & 3, q4 = tid >> 2;                     // q4 = 0..63
-    const int u0 = 4 * (b0 + q4) + seg;                         // row q4: genotype 0 of b0 + q4
-    const int u1 = 4 * (np + b0 + q4) + seg;                    // row 64 + q4: genotype 1
-    const int u2 = 4 * ((q4 >> 5) * np + c0 + (q4 & 31)) + seg; // row 128 + q4: (z, c)
-    const int u3 = 4 * ((q4 & 1) * np + a) + seg;               // a row x = q4 & 1
-    const int row3 = tid < 8 ? 192 + q4 : kM3Rows;              // threads 8.. write the spare row
+    // staging: each wave issues 4 direct-to-LDS loads per stage, groups 2w
+    // and 2w + 1 (b rows (y, b) = (g / 4, b0 + 16 (g % 4) + r)), group 8 + w
+    // (c rows (z, c) = (w / 2, c0 + 16 (w % 2) + r)) and lanes 2w, 2w + 1 of
+    // group 12 (a row x = r).  Lane l fills row r = l / 4, stored segment
+    // l % 4, so it loads the row's segment (l % 4) ^ m3_swz(r).  Byte
+    // offsets within a quad of the image (< 128 n_pad).
+    const int lr = lane >> 2, ls = (lane & 3) ^ m3_swz(lr);
+    const uint32_t ub0 = 16u * (uint32_t)(4 * ((w >> 1) * np + b0 + 32 * (w & 1) + lr) + ls);        // group 2w
+    const uint32_t ub1 = 16u * (uint32_t)(4 * ((w >> 1) * np + b0 + 32 * (w & 1) + 16 + lr) + ls);   // 2w + 1
+    const uint32_t uc = 16u * (uint32_t)(4 * ((w >> 1) * np + c0 + 16 * (w & 1) + lr) + ls);         // 8 + w
+    const bool a_lane = (lane >> 1) == w;                                       // lanes 2w, 2w + 1 (< 8)
+    const uint32_t ua = 16u * (uint32_t)(4 * ((lr & 1) * np + a) + ls);        // group 12, row x = lr
     v4f acc[2][2][2][2][2];   // [class][x][y][b block][z]
 #pragma unroll
     for (int c = 0; c < 2; ++c)
@@ -1148,41 +1181,49 @@ k_mfma3(const DevData D, const ScanParams P, TopState *st, const Fp4Image F, int
                 for (int bb = 0; bb < 2; ++bb)
 #pragma unroll
                     for (int z = 0; z < 2; ++z) acc[c][x][y][bb][z] = (v4f){0.f, 0.f, 0.f, 0.f};
-    const int kk = (lane >> 4) * 16, rr = lane & 15;
+    // fragment reads: lane reads row rr of a group at segment kc (32 samples
+    // 32 kc ..), stored at segment kc ^ m3_swz(rr); the a rows (m3_swz = 0)
+    // are read by every lane of a 16-lane group at once (broadcast)
+    const int rr = lane & 15, kc = lane >> 4;
+    const int fo = rr * 64 + ((kc ^ m3_swz(rr)) << 4), ao = kc << 4;
 #pragma unroll
     for (int cls = 0; cls < 2; ++cls) {
         const int iters = D.quads[cls];
         const uint4 *img = F.base + F.class_off[cls];
         const int64_t qstride = 8 * (int64_t)np;   // uint4 per quad: 2 genotypes x n_pad x 4
-        uint4 p0, p1, p2, p3;
-        auto gload = [&](int q) {
-            const uint4 *src = img + q * qstride;
-            p0 = src[u0];
-            p1 = src[u1];
-            p2 = src[u2];
-            p3 = src[u3];
+        auto issue = [&](int q) {
+            const char *g = (const char *)(img + q * qstride);
+            int8_t *s = lds[q % kM3Ring];
+            dma16((const uint4 *)(g + ub0), s + (2 * w) * kM3Group);
+            dma16((const uint4 *)(g + ub1), s + (2 * w + 1) * kM3Group);
+            dma16((const uint4 *)(g + uc), s + (8 + w) * kM3Group);
+            if (a_lane) dma16((const uint4 *)(g + ua), s + 12 * kM3Group);
         };
-        auto lstore = [&](int buf) {
-            *(uint4 *)&lds[buf][q4][seg * 16] = p0;
-            *(uint4 *)&lds[buf][64 + q4][seg * 16] = p1;
-            *(uint4 *)&lds[buf][128 + q4][seg * 16] = p2;
-            *(uint4 *)&lds[buf][row3][seg * 16] = p3;
-        };
-        gload(0);
-        lstore(0);
-        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < kM3Ring - 1; ++q)
+            if (q < iters) issue(q);
         for (int it = 0; it < iters; ++it) {
-            const int buf = it & 1;
-            if (it + 1 < iters) gload(it + 1);
+            // this wave's loads of stage it are complete once at most those of
+            // the stages after it (min(iters - 1 - it, kM3Ring - 2), 4 loads
+            // each) are pending; the barrier then publishes every wave's part
+            static_assert(kM3Ring == 4, "wait counts below");
+            const int ahead = min(iters - 1 - it, kM3Ring - 2);
+            if (ahead >= 2) wait_vmcnt<8>();
+            else if (ahead == 1) wait_vmcnt<4>();
+            else wait_vmcnt<0>();
+            stage_barrier();
+            const int8_t *sb = lds[it % kM3Ring];
             v4i ma[2], fb[2][2], fc[2];
 #pragma unroll
-            for (int x = 0; x < 2; ++x) ma[x] = *(const v4i *)&lds[buf][192 + x][kk];
+            for (int x = 0; x < 2; ++x) ma[x] = *(const v4i *)(sb + 12 * kM3Group + x * 64 + ao);
 #pragma unroll
             for (int y = 0; y < 2; ++y)
 #pragma unroll
-                for (int bb = 0; bb < 2; ++bb) fb[y][bb] = *(const v4i *)&lds[buf][y * 64 + wb * 32 + bb * 16 + rr][kk];
+                for (int bb = 0; bb < 2; ++bb) fb[y][bb] = *(const v4i *)(sb + (4 * y + 2 * wb + bb) * kM3Group + fo);
 #pragma unroll
-            for (int z = 0; z < 2; ++z) fc[z] = *(const v4i *)&lds[buf][128 + z * 32 + wc * 16 + rr][kk];
+            for (int z = 0; z < 2; ++z) fc[z] = *(const v4i *)(sb + (8 + 2 * z + wc) * kM3Group + fo);
+            // every wave is past stage it - 1, whose slot now takes stage it + 3
+            if (it + kM3Ring - 1 < iters) issue(it + kM3Ring - 1);
 #pragma unroll
             for (int x = 0; x < 2; ++x)
 #pragma unroll
@@ -1198,9 +1239,10 @@ k_mfma3(const DevData D, const ScanParams P, TopState *st, const Fp4Image F, int
                                 a8, c8, acc[cls][x][y][bb][z], 4, 4, 0, 0, 0, 0);
                         }
                     }
-            if (it + 1 < iters) lstore(buf ^ 1);
-            __syncthreads();
         }
+        // the next class's first loads reuse the ring: every wave must be
+        // done reading this class's last stages
+        stage_barrier();
     }
     // epilogue: lane holds column c = c0 + wc*16 + lane%16 and, per b block
     // bb, rows b = b0 + wb*32 + 16 bb + 4(lane/16) + j, j < 4
