@@ -1089,9 +1089,9 @@ k_mfma2(const DevData D, const ScanParams P, TopState *st, const Fp4Image F4, in
 // the popcount scan (same inclusion-exclusion, same MI and top-N code).
 // One workgroup: first SNP a, 64 b's x 32 c's, 4 waves of 32 b x 16 c; each
 // v_mfma_f32_16x16x128_f8f6f4 multiplies 16 b rows by 16 c rows over 128
-// samples (one bit-table quad).  Grid: y = first SNP a0 + y, x = the tiles
-// (tb <= tc) of the 64-SNP triangle from a's tile on, each split into two
-// 32-c halves.
+// samples (one bit-table quad).  Work items: the first SNPs a0 .. a0+na-1
+// times the tiles (tb <= tc) of the 64-SNP triangle from a's tile on, each
+// split into two 32-c halves.
 // ---------------------------------------------------------------------------
 typedef float v4f __attribute__((ext_vector_type(4)));
 
@@ -1145,13 +1145,18 @@ __device__ __forceinline__ void stage_barrier()
 }
 
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 2)))   // 256 registers, no spills
-k_mfma3(const DevData D, const ScanParams P, TopState *st, const Fp4Image F, int a0, int T)
+k_mfma3(const DevData D, const ScanParams P, TopState *st, const Fp4Image F, int a0, int na, int T)
 {
     __shared__ __attribute__((aligned(1024))) int8_t lds[kM3Ring][kM3Stage];
-    const int a = a0 + (int)blockIdx.y;
+    // block order: workgroup L runs on XCD L % 8; XCD k takes the tiles
+    // u = 8 i + k, i = 0, 1, ..., and for each tile all na first SNPs in a
+    // row, so that the workgroups resident on one XCD share their b and c
+    // rows in its L2 (only the a rows differ)
+    const int64_t L = blockIdx.x, s = L >> 3;
+    const int a = a0 + (int)(s % na);
+    const int64_t u = (s / na) * 8 + (L & 7);
     if (a >= D.n - 2) return;
     const int ta = (a + 1) >> 6, m = T - ta;
-    const int64_t u = blockIdx.x;
     if (u >= (int64_t)m * (m + 1)) return;
     int r, cc;
     tri_tile(u >> 1, m, r, cc);
@@ -1757,8 +1762,9 @@ hipError_t launch_mfma3(const DevData &D, const ScanParams &P, TopState *st, con
     F.base = fp4;
     F.class_off[0] = 0;
     F.class_off[1] = fp4_image_units(D, 0);
-    hipLaunchKernelGGL(k_mfma3, dim3((unsigned)m * (unsigned)(m + 1), (unsigned)(a1 - a0)), dim3(kBlock), 0, s, D, P,
-                       st, F, a0, T);
+    const int64_t tiles8 = ((int64_t)m * (m + 1) + 7) / 8, na = a1 - a0;
+    if (tiles8 * na * 8 > 0x7fffffffLL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_mfma3, dim3((unsigned)(tiles8 * na * 8)), dim3(kBlock), 0, s, D, P, st, F, a0, (int)na, T);
     return hipGetLastError();
 }
 
