@@ -1118,11 +1118,14 @@ constexpr int kM3Stage = 13 * kM3Group;
 constexpr int kM3Ring = 4;   // stages resident: the one read + up to 3 loading
 __device__ __forceinline__ int m3_swz(int r) { return (r >> 1) & 3; }
 
-// Direct-to-LDS load of 16 bytes per lane: lane l's bytes land at
-// lds_group + 16 l (lds_group wave-uniform).
-__device__ __forceinline__ void dma16(const uint4 *src, int8_t *lds_group)
+// Direct-to-LDS buffer load of 16 bytes per lane from byte offset voff of
+// the resource: lane l's bytes land at lds_group + 16 l (lds_group
+// wave-uniform).  The per-lane offsets stay fixed while the resource base
+// moves (scalar arithmetic only).
+__device__ __forceinline__ void dma16(__amdgpu_buffer_rsrc_t r, uint32_t voff, int8_t *lds_group)
 {
-    __builtin_amdgcn_global_load_lds((const void *)src, (__attribute__((address_space(3))) void *)lds_group, 16, 0, 0);
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (__attribute__((address_space(3))) void *)lds_group, 16, voff, 0, 0,
+                                            0);
 }
 
 // Wait until at most N vector-memory operations of this wave are pending
@@ -1199,12 +1202,14 @@ k_mfma3(const DevData D, const ScanParams P, TopState *st, const Fp4Image F, int
         const uint4 *img = F.base + F.class_off[cls];
         const int64_t qstride = 8 * (int64_t)np;   // uint4 per quad: 2 genotypes x n_pad x 4
         auto issue = [&](int q) {
-            const char *g = (const char *)(img + q * qstride);
+            // resource over quad q of the class image (8 n_pad uint4)
+            const __amdgpu_buffer_rsrc_t rq =
+                __builtin_amdgcn_make_buffer_rsrc((void *)(img + q * qstride), (short)0, 128 * np, 0x00020000);
             int8_t *s = lds[q % kM3Ring];
-            dma16((const uint4 *)(g + ub0), s + (2 * w) * kM3Group);
-            dma16((const uint4 *)(g + ub1), s + (2 * w + 1) * kM3Group);
-            dma16((const uint4 *)(g + uc), s + (8 + w) * kM3Group);
-            if (a_lane) dma16((const uint4 *)(g + ua), s + 12 * kM3Group);
+            dma16(rq, ub0, s + (2 * w) * kM3Group);
+            dma16(rq, ub1, s + (2 * w + 1) * kM3Group);
+            dma16(rq, uc, s + (8 + w) * kM3Group);
+            if (a_lane) dma16(rq, ua, s + 12 * kM3Group);
         };
 #pragma unroll
         for (int q = 0; q < kM3Ring - 1; ++q)
