@@ -126,8 +126,9 @@ __device__ inline void unrank(uint64_t r, int n, int m, const uint64_t *binom, i
 // MI = H(Y) + (1/N) sum_x [n_cx ln(n_cx/n_x) + n_tx ln(n_tx/n_x)], whose terms
 // all have one sign (no cancellation inside the sum).  Error < 2e-6 nats for
 // N < 2^24; only used to decide which combinations get the exact value.
-template <int CELLS>
-__device__ __forceinline__ float mi_screen(const uint32_t *a0, const uint32_t *a1, float hy, float inv_n)
+// Counts as uint32_t or as FP32 holding the same integers (exact below 2^24).
+template <int CELLS, class T>
+__device__ __forceinline__ float mi_screen(const T *a0, const T *a1, float hy, float inv_n)
 {
     // hardware v_rcp_f32 / v_log_f32 (log2) on normal arguments: every ratio
     // is clamped to >= 2^-100, and a zero count multiplies its log by zero
@@ -150,8 +151,8 @@ __device__ __forceinline__ float mi_screen(const uint32_t *a0, const uint32_t *a
 //   chi2/N = N/(n_c n_t) sum_x d_x^2 / n_x,  d_x = n_cx - n_x n_c/N.
 // FP32 error: below 2e-6 relative plus 5e-7 absolute (each |d_x| error is
 // under 1.2e-7 (n_x n_c/N + |d_x|), and sum_x |d_x| <= 2 min(n_c, n_t)).
-template <int CELLS>
-__device__ __forceinline__ float mi_bound(const uint32_t *a0, const uint32_t *a1, float frac_c, float chi_scale)
+template <int CELLS, class T>
+__device__ __forceinline__ float mi_bound(const T *a0, const T *a1, float frac_c, float chi_scale)
 {
     float s = 0.0f;
 #pragma unroll
@@ -230,8 +231,8 @@ __device__ __forceinline__ void push_candidates(bool want, float mi, uint64_t co
 
 // Epilogue shared by every scan kernel: screening MI, exact MI for the
 // survivors, key test against the current N-th best, push.
-template <int CELLS>
-__device__ __forceinline__ void score_and_push(bool valid, const uint32_t *a0, const uint32_t *a1,
+template <int CELLS, class T>
+__device__ __forceinline__ void score_and_push(bool valid, const T *a0, const T *a1,
                                                uint64_t combo, float tau, uint64_t tau_combo,
                                                const ScanParams &P, TopState *st)
 {
@@ -240,7 +241,14 @@ __device__ __forceinline__ void score_and_push(bool valid, const uint32_t *a0, c
     if (valid && (!P.bounded || bound_passes(mi_bound<CELLS>(a0, a1, P.frac_c, P.chi_scale), tau, P.delta))) {
         const float ms = mi_screen<CELLS>(a0, a1, P.hy, P.inv_n);
         if (ms >= tau - P.delta) {
-            mf = (float)mi_exact<CELLS>(a0, a1, P.n64, P.hy64);
+            if constexpr (std::is_same<T, uint32_t>::value) {
+                mf = (float)mi_exact<CELLS>(a0, a1, P.n64, P.hy64);
+            } else {
+                uint32_t u0[CELLS], u1[CELLS];
+#pragma unroll
+                for (int x = 0; x < CELLS; ++x) u0[x] = (uint32_t)a0[x], u1[x] = (uint32_t)a1[x];
+                mf = (float)mi_exact<CELLS>(u0, u1, P.n64, P.hy64);
+            }
             want = better(mf, combo, tau, tau_combo);
         }
     }
@@ -483,7 +491,7 @@ k_scan_direct(const DevData D, const ScanParams P, TopState *st)
 
 // In-place transform of one class's 3^3 table from the marginal basis
 // (index 2 = "any genotype") to genotype 2, dimension by dimension.
-__device__ __forceinline__ void mobius3(uint32_t *m)
+template <class T> __device__ __forceinline__ void mobius3(T *m)
 {
 #pragma unroll
     for (int ij = 0; ij < 9; ++ij) m[ij * 3 + 2] -= m[ij * 3 + 0] + m[ij * 3 + 1];
@@ -495,7 +503,7 @@ __device__ __forceinline__ void mobius3(uint32_t *m)
     for (int jk = 0; jk < 9; ++jk) m[18 + jk] -= m[jk] + m[9 + jk];
 }
 
-__device__ __forceinline__ void mobius2(uint32_t *m)
+template <class T> __device__ __forceinline__ void mobius2(T *m)
 {
 #pragma unroll
     for (int i = 0; i < 3; ++i) m[i * 3 + 2] -= m[i * 3 + 0] + m[i * 3 + 1];
@@ -806,7 +814,7 @@ __device__ __forceinline__ void scan3m_unit(const DevData &D, const ScanParams &
 }
 
 template <int PW, bool HIST>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3)))   // <= 168 registers
 k_scan3m(const DevData D, const ScanParams P, TopState *st)
 {
     __shared__ uint32_t lds_hist[HIST ? kHistBins : 1];
@@ -1059,21 +1067,24 @@ k_mfma2(const DevData D, const ScanParams P, TopState *st, const Fp4Image F4, in
             const int i = r * 4 + q;
             const int a = a0 + wa * 32 + 8 * r + 4 * (lane >> 5) + q;
             const int64_t rel = (int64_t)a - P.offset;
-            const bool valid = b_ok && a < b && rel >= 0 && rel % P.step == 0;
+            const bool valid = b_ok && a < b && rel >= 0 && (P.step == 1 || rel % P.step == 0);
             const uint32_t *sa = D.singles + (size_t)a * 6;
-            uint32_t m0[9], m1[9];
+            // counts as FP32 (fp4 products) or uint32_t (int8 products): both
+            // exact, and the inclusion-exclusion stays exact in either
+            typedef typename std::conditional<FP4, float, uint32_t>::type Cnt;
+            Cnt m0[9], m1[9];
 #pragma unroll
             for (int cls = 0; cls < 2; ++cls) {
-                uint32_t *m = cls ? m1 : m0;
-                m[0] = (uint32_t)acc[cls][0][0][i];
-                m[1] = (uint32_t)acc[cls][0][1][i];
-                m[3] = (uint32_t)acc[cls][1][0][i];
-                m[4] = (uint32_t)acc[cls][1][1][i];
-                m[2] = sa[cls * 3 + 0];
-                m[5] = sa[cls * 3 + 1];
-                m[6] = sb[cls][0];
-                m[7] = sb[cls][1];
-                m[8] = D.total[cls];
+                Cnt *m = cls ? m1 : m0;
+                m[0] = (Cnt)acc[cls][0][0][i];
+                m[1] = (Cnt)acc[cls][0][1][i];
+                m[3] = (Cnt)acc[cls][1][0][i];
+                m[4] = (Cnt)acc[cls][1][1][i];
+                m[2] = (Cnt)sa[cls * 3 + 0];
+                m[5] = (Cnt)sa[cls * 3 + 1];
+                m[6] = (Cnt)sb[cls][0];
+                m[7] = (Cnt)sb[cls][1];
+                m[8] = (Cnt)D.total[cls];
                 mobius2(m);
             }
             score_and_push<9>(valid, m0, m1, ((uint64_t)a << P.combo_bits) | (uint64_t)b, tau, tau_combo, P, st);
@@ -1282,7 +1293,7 @@ k_mfma3(const DevData D, const ScanParams P, TopState *st, const Fp4Image F, int
             constexpr int bb = decltype(BJ)::value >> 2, j = decltype(BJ)::value & 3;
             const int b = b0 + wb * 32 + bb * 16 + 4 * (lane >> 4) + j;
             const int64_t rel = rank_a + (b - a - 1) - P.offset;
-            const bool valid = c_ok && b > a && c > b && rel >= 0 && rel % P.step == 0;
+            const bool valid = c_ok && b > a && c > b && rel >= 0 && (P.step == 1 || rel % P.step == 0);
             uint32_t fab[8] = {0, 0, 0, 0, 0, 0, 0, 0}, fbc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
             uint32_t sb[6] = {0, 0, 0, 0, 0, 0};
             if (valid) {
@@ -1297,34 +1308,36 @@ k_mfma3(const DevData D, const ScanParams P, TopState *st, const Fp4Image F, int
 #pragma unroll
                 for (int e = 0; e < 6; ++e) sb[e] = sp[e];
             }
-            uint32_t m0[27], m1[27];
+            // counts in FP32: the products are exact integers below 2^24, and
+            // so is every step of the inclusion-exclusion
+            float m0[27], m1[27];
 #pragma unroll
             for (int cls = 0; cls < 2; ++cls) {
-                uint32_t *mm = cls ? m1 : m0;
+                float *mm = cls ? m1 : m0;
                 // mm[x*9 + y*3 + z], x, y, z in {0, 1, 2 = any}
 #pragma unroll
                 for (int x = 0; x < 2; ++x)
 #pragma unroll
                     for (int y = 0; y < 2; ++y) {
 #pragma unroll
-                        for (int z = 0; z < 2; ++z) mm[x * 9 + y * 3 + z] = (uint32_t)acc[cls][x][y][bb][z][j];
-                        mm[x * 9 + y * 3 + 2] = fab[cls * 4 + x * 2 + y];
+                        for (int z = 0; z < 2; ++z) mm[x * 9 + y * 3 + z] = acc[cls][x][y][bb][z][j];
+                        mm[x * 9 + y * 3 + 2] = (float)fab[cls * 4 + x * 2 + y];
                     }
 #pragma unroll
                 for (int x = 0; x < 2; ++x)
 #pragma unroll
-                    for (int z = 0; z < 2; ++z) mm[x * 9 + 6 + z] = fac[cls * 4 + x * 2 + z];
+                    for (int z = 0; z < 2; ++z) mm[x * 9 + 6 + z] = (float)fac[cls * 4 + x * 2 + z];
 #pragma unroll
                 for (int y = 0; y < 2; ++y)
 #pragma unroll
-                    for (int z = 0; z < 2; ++z) mm[18 + y * 3 + z] = fbc[cls * 4 + y * 2 + z];
-                mm[0 * 9 + 8] = sa[cls * 3 + 0];
-                mm[1 * 9 + 8] = sa[cls * 3 + 1];
-                mm[18 + 0 * 3 + 2] = sb[cls * 3 + 0];
-                mm[18 + 1 * 3 + 2] = sb[cls * 3 + 1];
-                mm[18 + 6 + 0] = sc[cls][0];
-                mm[18 + 6 + 1] = sc[cls][1];
-                mm[26] = D.total[cls];
+                    for (int z = 0; z < 2; ++z) mm[18 + y * 3 + z] = (float)fbc[cls * 4 + y * 2 + z];
+                mm[0 * 9 + 8] = (float)sa[cls * 3 + 0];
+                mm[1 * 9 + 8] = (float)sa[cls * 3 + 1];
+                mm[18 + 0 * 3 + 2] = (float)sb[cls * 3 + 0];
+                mm[18 + 1 * 3 + 2] = (float)sb[cls * 3 + 1];
+                mm[18 + 6 + 0] = (float)sc[cls][0];
+                mm[18 + 6 + 1] = (float)sc[cls][1];
+                mm[26] = (float)D.total[cls];
                 mobius3(mm);
             }
             const uint64_t combo = ((((uint64_t)a << P.combo_bits) | (uint64_t)b) << P.combo_bits) | (uint64_t)c;
@@ -1443,7 +1456,7 @@ typedef const BatchJob ConstJob;  // host pass: kernel bodies are only parsed
 #endif
 
 template <int LAUNCH>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(LAUNCH == 0 ? 3 : 1)))
 k_batch_scan(const BatchJob *__restrict__ jobs, const BatchSeg *__restrict__ segs, BatchHeader *hdr)
 {
     const int lane = threadIdx.x & 63;
