@@ -224,7 +224,7 @@ size_t staged_host_bytes(size_t snps, size_t cw, size_t tw, int order)
 }
 
 fiu_dataset *stage_dataset(const uint64_t *words, size_t snps, size_t cw, size_t tw, uint32_t cases,
-                           uint32_t ctrls, int order, Arena *arena, Arena *pinned, StagedUpload &up)
+                           uint32_t ctrls, int order, bool want_fp4, Arena *arena, Arena *pinned, StagedUpload &up)
 {
     if (!words && snps) fail(FIU_ERR_INVALID, "words is NULL");
     if (snps && (cw == 0 || tw == 0) && (cases || ctrls)) fail(FIU_ERR_INVALID, "zero words per row");
@@ -298,6 +298,18 @@ fiu_dataset *stage_dataset(const uint64_t *words, size_t snps, size_t cw, size_t
             }
         }
     }
+    if (want_fp4 && mfma_dataset(d.get()) && (order == 2 || (order == 3 && up.pair))) {
+        const size_t units = fp4_image_units(D, 0) + fp4_image_units(D, 1);
+        if (units * 16 <= arena->largest_free()) {
+            try {
+                d->fp4.alloc(units);
+                up.fp4 = true;
+                d->fp4_ready = true;    // in stream order, before any scan of it
+            } catch (const Failure &) {
+                up.fp4 = false;         // no room: the popcount kinds scan it
+            }
+        }
+    }
     return d.release();
 }
 
@@ -321,6 +333,7 @@ void enqueue_upload(fiu_dataset *d, StagedUpload &up, hipStream_t s)
         P.binom = d->binom.p;
         check(launch_pair_table(d->dev, P, d->pair.p, s), "pair table launch");
     }
+    if (up.fp4) check(launch_fp4_image(d->dev, d->fp4.p, s), "fp4 image launch");
     d->arena->release(up.staging_d);
     up.staging_d = nullptr;
 }

@@ -102,9 +102,20 @@ enum BatchKind : int32_t {
     kKindDirect2 = 2,  // order 2, all cells
     kKindDirect3 = 3,  // order 3, all cells
     kKindDirect4 = 4,  // order 4, all cells (its own launch: register budget)
+    kKindMfma2 = 5,    // order 2 on the matrix cores (fp4): units are 64 x 64 tiles of super-blocks
+    kKindMfma3 = 6,    // order 3 on the matrix cores (fp4): units are (first SNP, tile) pairs
+};
+
+// fp4 image of a dataset's genotype rows 0 and 1 (operands of the matrix-core
+// scans): the 128 samples of quad q, genotype g, SNP s of a class are 64
+// bytes = 4 uint4 at class_off[c] + 4 ((2 q + g) n_pad + s).
+struct Fp4Image {
+    const uint4 *base;
+    uint64_t class_off[2];   // uint4 offset of each class
 };
 
 __host__ __device__ inline int batch_group(int32_t kind) { return kind <= kKindScan3m ? kMarginalPW : 1; }
+__host__ __device__ inline bool batch_mfma(int32_t kind) { return kind == kKindMfma2 || kind == kKindMfma3; }
 
 struct BatchJob {
     DevData D;
@@ -115,10 +126,15 @@ struct BatchJob {
     int32_t kind;            // BatchKind
     int32_t chunks;          // tile chunks per prefix group
     int32_t tiles;           // tiles per chunk
+    Fp4Image F;              // matrix-core kinds: the dataset's fp4 image
+    int32_t T;               // matrix-core kinds: 64-SNP tiles (n_pad / 64)
+    int32_t pad_;
 };
 
 // A contiguous prefix range [j0, j1) of one job in one launch; its units are
-// numbered from `start` (prefix group major, tile chunk minor).
+// numbered from `start` (prefix group major, tile chunk minor).  Matrix-core
+// kinds: [j0, j1) are first SNPs (order 3) or super-blocks (order 2), and
+// `start` numbers the units of each XCD's queue (every XCD has as many).
 struct BatchSeg {
     uint64_t start;
     int64_t j0, j1;
@@ -132,7 +148,10 @@ struct BatchHeader {
     uint32_t units[2];       // units of each launch
     uint32_t nmerge;         // jobs to merge after the launches
     uint32_t head[2];        // work counters (cleared by the graph)
-    uint32_t pad_;
+    uint32_t mseg;           // segments of the matrix-core launch
+    uint32_t munits;         // units of each XCD's queue in it
+    uint32_t mhead[8];       // its per-XCD work counters (cleared by the graph)
+    uint32_t pad_[3];
 };
 
 __host__ __device__ inline uint64_t pair_index(int64_t b, int64_t c, int64_t n)

@@ -144,8 +144,11 @@ struct Job {
     uint32_t outputs = 0;
     // plan
     int kind = kKindDirect2, pw = 1, bits = 1, chunks = 1, tiles = 1;
+    // units of work: prefixes (popcount kinds), first SNPs (order 3 on the
+    // matrix cores) or super-blocks (order 2 on the matrix cores)
     int64_t n_pref = 0, next = 0, given = 0, seed = 1;
-    double instr_per_prefix = 1.0;
+    double instr_per_prefix = 1.0;    // popcount kinds: cost of one prefix
+    double instr_per_combo = 0.0;     // matrix-core kinds: cost of one combination
     std::deque<std::pair<int64_t, int64_t>> redo;
     int inflight = 0;
     int slot = -1;
@@ -229,7 +232,8 @@ class Engine {
     hipEvent_t ev_start_[2] = {}, ev_end_[2] = {}, ev_done_[2] = {};
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t exec_ = nullptr;
-    size_t max_segs_ = 0, off_merge_ = 0, off_segs0_ = 0, off_segs1_ = 0, desc_bytes_ = 0;
+    size_t max_segs_ = 0, max_msegs_ = 0, off_merge_ = 0, off_segs0_ = 0, off_segs1_ = 0, off_segs2_ = 0,
+           desc_bytes_ = 0;
     std::vector<Job *> slots_;
     std::vector<Job *> active_;
     std::mutex ready_mu_;
@@ -256,7 +260,10 @@ Engine::Engine(int device, const EngineConfig &cfg, Clock::time_point epoch) : d
     off_merge_ = 256;
     off_segs0_ = (off_merge_ + (size_t)A * 4 + 255) / 256 * 256;
     off_segs1_ = off_segs0_ + max_segs_ * sizeof(BatchSeg);
-    desc_bytes_ = off_segs1_ + max_segs_ * sizeof(BatchSeg);
+    // matrix-core segments: order-3 ranges are cut at every 64th first SNP
+    max_msegs_ = (size_t)A * 64 + 1024;
+    off_segs2_ = off_segs1_ + max_segs_ * sizeof(BatchSeg);
+    desc_bytes_ = off_segs2_ + max_msegs_ * sizeof(BatchSeg);
     jobs_d_.alloc(A);
     states_d_.alloc(A);
     cand_d_.alloc((size_t)A * cfg_.cand_cap);
@@ -276,13 +283,14 @@ Engine::Engine(int device, const EngineConfig &cfg, Clock::time_point epoch) : d
         check(hipEventCreateWithFlags(&ev_done_[p], hipEventDisableTiming), "hipEventCreate");
     }
     slots_.assign(A, nullptr);
-    const int b0 = batch_blocks(0), b1 = batch_blocks(1);
-    if (b0 <= 0 || b1 <= 0) fail(FIU_ERR_DEVICE, "cannot size the persistent scan launches");
-    // the step graph: counter reset, the two persistent scans, the merges
+    const int b0 = batch_blocks(0), b1 = batch_blocks(1), b2 = batch_blocks(2);
+    if (b0 <= 0 || b1 <= 0 || b2 <= 0) fail(FIU_ERR_DEVICE, "cannot size the persistent scan launches");
+    // the step graph: counter reset, the three persistent scans, the merges
     char *d = desc_d_.p;
     check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "graph capture");
     const hipError_t le = launch_batch(jobs_d_.p, (const BatchSeg *)(d + off_segs0_), (const BatchSeg *)(d + off_segs1_),
-                                       (const int32_t *)(d + off_merge_), (BatchHeader *)d, b0, b1, A, stream_);
+                                       (const BatchSeg *)(d + off_segs2_), (const int32_t *)(d + off_merge_),
+                                       (BatchHeader *)d, b0, b1, b2, A, stream_);
     const hipError_t ce = hipStreamEndCapture(stream_, &graph_);
     check(le, "batched step capture");
     check(ce, "graph capture end");
@@ -329,6 +337,23 @@ void Engine::plan_job(Job *j)
     j->n_pref = total_prefixes > (uint64_t)j->offset
                     ? (int64_t)((total_prefixes - (uint64_t)j->offset + (uint64_t)j->step - 1) / (uint64_t)j->step)
                     : 0;
+    j->rs.combinations = (j->step == 1 && j->offset == 0)
+                             ? binom_sat((uint64_t)n, (uint64_t)order)
+                             : count_combinations((uint64_t)n, M, (uint64_t)j->step, (uint64_t)j->offset);
+    // the matrix cores take order 2 (any distribution: the tiles mask other
+    // shares) and whole order-3 scans, as the single search does by default
+    const int T = d->dev.n_pad / 64;
+    if (mfma_dataset(d) && d->fp4_ready && (order == 2 || (order == 3 && d->pair_ready && j->step == 1))) {
+        j->kind = order == 2 ? kKindMfma2 : kKindMfma3;
+        j->pw = 1;
+        j->n_pref = order == 2 ? mfma2_superblocks(T) : std::max(0, n - 2);
+        // cost of one combination in the wave-instruction currency of the
+        // popcount kinds (measured: about 6x the popcount throughput)
+        j->instr_per_combo = 0.15 * ((double)d->dev.quads[0] + (double)d->dev.quads[1]) + 3.0;
+        j->seed = 1;
+        j->rs.algorithm = FIU_ALGO_MFMA;
+        return;
+    }
     const bool marginal = d->dev.uniform && (order == 2 || (order == 3 && d->pair_ready));
     j->kind = marginal ? (order == 2 ? kKindScan2m : kKindScan3m)
                        : (order == 2 ? kKindDirect2 : order == 3 ? kKindDirect3 : kKindDirect4);
@@ -347,9 +372,45 @@ void Engine::plan_job(Job *j)
     // first slice: about 16k combinations (its exact top-N seeds the threshold)
     j->seed = std::max<int64_t>(j->pw, (int64_t)(16384.0 / std::max(1.0, per_pref)) / j->pw * j->pw);
     j->rs.algorithm = marginal ? FIU_ALGO_MARGINAL : FIU_ALGO_DIRECT;
-    j->rs.combinations = (j->step == 1 && j->offset == 0)
-                             ? binom_sat((uint64_t)n, (uint64_t)order)
-                             : count_combinations((uint64_t)n, M, (uint64_t)j->step, (uint64_t)j->offset);
+}
+
+// Scans the matrix cores would take (order 2, whole order-3 scans): their
+// datasets get the fp4 image when it fits.
+bool wants_fp4(const Job *j) { return j->order == 2 || (j->order == 3 && j->step == 1); }
+
+void prepare_fp4(Job *j, hipStream_t s)
+{
+    fiu_dataset *d = j->ds;
+    if (wants_fp4(j) && mfma_dataset(d) && (j->order == 2 || d->pair_ready)) (void)ensure_fp4_image(d, s);
+}
+
+// Estimated cost of units [a, b) of a job.  Order 3 on the matrix cores: first
+// SNP x has C(n-1-x, 2) triples, so [a, b) has C(n-a, 3) - C(n-b, 3); order 2:
+// a super-block is 64 tiles of 64 x 64 pairs.
+double range_cost(const Job *j, int64_t a, int64_t b)
+{
+    if (b <= a) return 0.0;
+    if (j->kind == kKindMfma3) {
+        const double n = (double)j->ds->snps;
+        auto c3 = [](double m) { return m < 3.0 ? 0.0 : m * (m - 1.0) * (m - 2.0) / 6.0; };
+        return (c3(n - (double)a) - c3(n - (double)b)) * j->instr_per_combo;
+    }
+    if (j->kind == kKindMfma2) return (double)(b - a) * 64.0 * 4096.0 * j->instr_per_combo;
+    return (double)(b - a) * j->instr_per_prefix;
+}
+
+// Largest e in (from, limit] with range_cost(from, e) <= budget (at least
+// from + 1).
+int64_t range_end(const Job *j, int64_t from, int64_t limit, double budget)
+{
+    int64_t lo = from + 1, hi = limit;
+    if (lo >= hi) return hi;
+    while (lo < hi) {
+        const int64_t mid = lo + (hi - lo + 1) / 2;
+        if (range_cost(j, from, mid) <= budget) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
 }
 
 void Engine::ready(Job *j)
@@ -391,7 +452,7 @@ void Engine::activate(Job *j, int slot)
     P.cand_cap = cfg_.cand_cap;
     P.combo_bits = j->bits;
     P.tag = (uint32_t)slot;
-    P.pair = j->kind == kKindScan3m ? d->pair.p : nullptr;
+    P.pair = (j->kind == kKindScan3m || j->kind == kKindMfma3) ? d->pair.p : nullptr;
     P.hist = nullptr;
     mi_constants(d->cases, d->ctrls, d->complete(), P);
     B.st = states_d_.p + slot;
@@ -400,6 +461,12 @@ void Engine::activate(Job *j, int slot)
     B.kind = j->kind;
     B.chunks = j->chunks;
     B.tiles = j->tiles;
+    B.T = d->dev.n_pad / 64;
+    if (batch_mfma(j->kind)) {
+        B.F.base = d->fp4.p;
+        B.F.class_off[0] = 0;
+        B.F.class_off[1] = fp4_image_units(d->dev, 0);
+    }
     check(hipMemcpyAsync(jobs_d_.p + slot, &B, sizeof(B), hipMemcpyHostToDevice, stream_), "job upload");
     check(hipMemcpyAsync(states_d_.p + slot, init_h_, sizeof(TopState), hipMemcpyHostToDevice, stream_), "state init");
 }
@@ -420,10 +487,11 @@ bool Engine::launch_step()
     std::vector<double> want(cand.size());
     for (size_t i = 0; i < cand.size(); ++i) {
         Job *j = cand[i];
-        int64_t avail = j->n_pref - j->next;
-        for (auto &r : j->redo) avail += r.second - r.first;
+        double redo = 0.0;
+        int64_t redo_units = 0;
+        for (auto &r : j->redo) redo += range_cost(j, r.first, r.second), redo_units += r.second - r.first;
         const int64_t cap = std::max<int64_t>(j->seed, j->given);
-        want[i] = (double)std::min(avail, cap) * j->instr_per_prefix;
+        want[i] = redo + range_cost(j, j->next, std::min(j->n_pref, j->next + std::max<int64_t>(0, cap - redo_units)));
     }
     // water-filling: equal shares, what a small job does not need goes to the others
     std::vector<size_t> order(cand.size());
@@ -441,10 +509,60 @@ bool Engine::launch_step()
     BatchHeader *H = (BatchHeader *)desc;
     int32_t *merge = (int32_t *)(desc + off_merge_);
     BatchSeg *segs[2] = {(BatchSeg *)(desc + off_segs0_), (BatchSeg *)(desc + off_segs1_)};
-    uint32_t nseg[2] = {0, 0};
-    uint64_t units[2] = {0, 0};
+    BatchSeg *msegs = (BatchSeg *)(desc + off_segs2_);
+    uint32_t nseg[2] = {0, 0}, nmseg = 0;
+    uint64_t units[2] = {0, 0}, munits = 0;
     uint32_t nmerge = 0;
+    // matrix-core ranges: per-XCD queue units (see k_batch_mfma); order-3
+    // ranges are cut where the first SNP's tile changes (a constant tile
+    // count per segment)
+    auto add_mseg = [&](Job *j, int64_t a, int64_t b) {
+        const int T = j->ds->dev.n_pad / 64;
+        std::vector<std::pair<int64_t, int64_t>> parts;
+        if (j->kind == kKindMfma3) {
+            for (int64_t x = a; x < b;) {
+                const int64_t e = std::min(b, (x + 1) / 64 * 64 + 63);   // last a with the same (a + 1) / 64, + 1
+                parts.emplace_back(x, e);
+                x = e;
+            }
+        } else {
+            parts.emplace_back(a, b);
+        }
+        if (nmseg + parts.size() > max_msegs_) return false;
+        uint64_t total = 0;
+        for (auto &pr : parts) {
+            uint64_t u;
+            if (j->kind == kKindMfma3) {
+                const int64_t m = T - (pr.first + 1) / 64;
+                u = (uint64_t)((m * (m + 1) + 7) / 8) * (uint64_t)(pr.second - pr.first);
+            } else {
+                u = (uint64_t)((pr.second - pr.first + 7) / 8) * 64;
+            }
+            total += u;
+        }
+        if (munits + total > (1ull << 31)) return false;
+        for (auto &pr : parts) {
+            uint64_t u;
+            if (j->kind == kKindMfma3) {
+                const int64_t m = T - (pr.first + 1) / 64;
+                u = (uint64_t)((m * (m + 1) + 7) / 8) * (uint64_t)(pr.second - pr.first);
+            } else {
+                u = (uint64_t)((pr.second - pr.first + 7) / 8) * 64;
+            }
+            BatchSeg &sg = msegs[nmseg++];
+            sg.start = munits;
+            sg.j0 = pr.first;
+            sg.j1 = pr.second;
+            sg.job = j->slot;
+            sg.pad_ = 0;
+            munits += u;
+        }
+        R.ranges.emplace_back(j, a, b);
+        R.est_instr += range_cost(j, a, b);
+        return true;
+    };
     auto add_seg = [&](Job *j, int64_t a, int64_t b) {
+        if (batch_mfma(j->kind)) return add_mseg(j, a, b);
         const int L = j->kind == kKindDirect4 ? 1 : 0;
         if (nseg[L] >= max_segs_) return false;
         const uint64_t u = (uint64_t)((b - a + j->pw - 1) / j->pw) * (uint64_t)j->chunks;
@@ -462,18 +580,21 @@ bool Engine::launch_step()
     };
     for (size_t i = 0; i < cand.size(); ++i) {
         Job *j = cand[i];
-        int64_t p = (int64_t)(give[i] / j->instr_per_prefix);
-        p = std::max<int64_t>(p, j->pw);
+        double g = give[i];
         const size_t r0 = R.ranges.size();
         // re-scans first (whole ranges: they were slices before)
-        while (p > 0 && !j->redo.empty()) {
+        bool any = false;
+        while ((g > 0.0 || !any) && !j->redo.empty()) {
             const auto r = j->redo.front();
             if (!add_seg(j, r.first, r.second)) break;
             j->redo.pop_front();
-            p -= r.second - r.first;
+            g -= range_cost(j, r.first, r.second);
+            any = true;
         }
-        if (p > 0 && j->next < j->n_pref) {
-            int64_t e = j->next + p;
+        if ((g > 0.0 || !any) && j->next < j->n_pref) {
+            // at least one unit group; no more than everything given before
+            const int64_t cap = j->next + std::max<int64_t>(j->seed, j->given);
+            int64_t e = range_end(j, j->next, std::min(j->n_pref, std::max(cap, j->next + j->pw)), g);
             e = (e + j->pw - 1) / j->pw * j->pw;
             e = std::min(e, j->n_pref);
             if (add_seg(j, j->next, e)) {
@@ -496,12 +617,18 @@ bool Engine::launch_step()
     H->units[0] = (uint32_t)units[0];
     H->units[1] = (uint32_t)units[1];
     H->nmerge = nmerge;
+    H->mseg = nmseg;
+    H->munits = (uint32_t)munits;
     // descriptor: header + merge list + launch-0 segments in one copy, the
     // launch-1 segments in a second one when there are any
     check(hipMemcpyAsync(desc_d_.p, desc, off_segs0_ + nseg[0] * sizeof(BatchSeg), hipMemcpyHostToDevice, stream_),
           "step descriptor");
     if (nseg[1])
         check(hipMemcpyAsync(desc_d_.p + off_segs1_, desc + off_segs1_, nseg[1] * sizeof(BatchSeg),
+                             hipMemcpyHostToDevice, stream_),
+              "step descriptor");
+    if (nmseg)
+        check(hipMemcpyAsync(desc_d_.p + off_segs2_, desc + off_segs2_, nmseg * sizeof(BatchSeg),
                              hipMemcpyHostToDevice, stream_),
               "step descriptor");
     check(hipEventRecord(ev_start_[parity], stream_), "event");
@@ -514,7 +641,7 @@ bool Engine::launch_step()
     R.valid = true;
     ++step_no_;
     ++stats_.steps;
-    stats_.units += units[0] + units[1];
+    stats_.units += units[0] + units[1] + 8 * munits;
     return true;
 }
 
@@ -704,6 +831,8 @@ size_t Server::footprint(const Job *j) const
     size_t b = dataset_bytes(j->snps, j->cw, j->tw) + staging_bytes(j->snps, j->cw, j->tw);
     b += (size_t)(j->order + 1) * (j->snps + 1) * 8 + Arena::kAlign;   // binomial table
     if (j->order == 3) b += pair_table_bytes(j->snps) + Arena::kAlign;
+    // the fp4 image of the matrix-core scans is not counted: it is allocated
+    // at upload only when it fits, else the popcount kinds scan the dataset
     return b;
 }
 
@@ -796,7 +925,7 @@ void Server::ingest_main()
             if (!j->ds && host_footprint(j) > 0) {
                 // staged: host preparation now, device work queued by the scheduler
                 fiu_dataset *d = stage_dataset(j->words.data(), j->snps, j->cw, j->tw, j->cases, j->ctrls, j->order,
-                                               arena_.get(), pinned_.get(), j->up);
+                                               wants_fp4(j), arena_.get(), pinned_.get(), j->up);
                 if (!d) {
                     no_room = true;
                 } else {
@@ -815,6 +944,7 @@ void Server::ingest_main()
                 }
                 ensure_binom(j->ds, j->order - 1, ingest_stream_);
                 if (j->order == 3 && j->ds->dev.uniform) (void)ensure_pair_table(j->ds, ingest_stream_);
+                prepare_fp4(j, ingest_stream_);
                 Engine::plan_job(j);
             }
         } catch (const Failure &e) {
@@ -1017,6 +1147,7 @@ fiu_status fiu_search_batch(fiu_scan *scans, size_t n, const fiu_batch_config *c
             try {
                 ensure_binom(j->ds, std::max(1, j->order - 1), eng.prep_stream());
                 if (j->order == 3 && j->ds->dev.uniform) (void)ensure_pair_table(j->ds, eng.prep_stream());
+                prepare_fp4(j, eng.prep_stream());
                 Engine::plan_job(j);
             } catch (const Failure &e) {
                 j->scan->status = e.status;

@@ -198,6 +198,7 @@ struct StagedUpload {
     size_t n_words = 0, off_singles = 0, off_binom = 0, binom_n = 0;
     uint64_t *staging_d = nullptr;   // device copy of the words (arena)
     bool pair = false;               // build the order-3 pair table
+    bool fp4 = false;                // build the fp4 image (matrix-core scans)
 };
 
 // Host bytes (pinned) a staged upload of this shape needs for `order`.
@@ -205,9 +206,11 @@ size_t staged_host_bytes(size_t snps, size_t cw, size_t tw, int order);
 
 // Allocates the dataset's device buffers (and the device staging) from
 // `arena` and its pinned staging from `pinned`; returns null when either has
-// no room.  Throws on invalid shapes.
+// no room.  Throws on invalid shapes.  `want_fp4`: also the fp4 image when the
+// dataset qualifies for the matrix-core scans (mfma_dataset; order 3 also
+// needs the pair table) and it fits.
 fiu_dataset *stage_dataset(const uint64_t *words, size_t snps, size_t cw, size_t tw, uint32_t cases,
-                           uint32_t ctrls, int order, Arena *arena, Arena *pinned, StagedUpload &up);
+                           uint32_t ctrls, int order, bool want_fp4, Arena *arena, Arena *pinned, StagedUpload &up);
 // Copies, transposes and builds the pair table on `s`; the device staging is
 // released at once (later users of that memory are ordered after it on `s`),
 // the pinned staging must be kept until the work completes.
@@ -223,6 +226,13 @@ inline size_t pair_table_bytes(size_t snps) { return snps < 2 ? 0 : snps * (snps
 // fp4 image of genotype rows 0 and 1 (4 bits per sample and row); false when
 // it does not fit.
 bool ensure_fp4_image(fiu_dataset *d, hipStream_t s);
+// The dataset admits the matrix-core scans: every SNP covers the same samples
+// (the counts of the other cells follow by inclusion-exclusion) and both
+// classes are below 2^24 samples (FP32 counts are exact).
+inline bool mfma_dataset(const fiu_dataset *d)
+{
+    return d->dev.uniform && d->cases < (1u << 24) && d->ctrls < (1u << 24);
+}
 
 // Top-N entries -> ascending SNP indices (order per entry) and float MI.
 void unpack_results(const Cand *top, uint32_t count, int order, int bits, int32_t *out_combos, float *out_mi);

@@ -38,10 +38,12 @@ hipError_t launch_mi_batch(const uint32_t *tables, uint64_t n, uint64_t ct_size,
                            hipStream_t s);
 
 // Batched scans: resident workgroups of the persistent scan kernel of launch
-// 0 (orders 2-3) or 1 (order 4) on the current device, and the whole batched
-// step: counter reset, the two persistent scans, the per-job merges.
+// 0 (popcount orders 2-3), 1 (order 4) or 2 (matrix-core kinds, a multiple
+// of 8) on the current device, and the whole batched step: counter reset,
+// the three persistent scans, the per-job merges.
 int batch_blocks(int launch);
-hipError_t launch_batch(const BatchJob *jobs, const BatchSeg *segs0, const BatchSeg *segs1, const int32_t *merge_list,
-                        BatchHeader *hdr, int blocks0, int blocks1, int merge_blocks, hipStream_t s);
+hipError_t launch_batch(const BatchJob *jobs, const BatchSeg *segs0, const BatchSeg *segs1, const BatchSeg *segs2,
+                        const int32_t *merge_list, BatchHeader *hdr, int blocks0, int blocks1, int blocks2,
+                        int merge_blocks, hipStream_t s);
 
 }  // namespace fiu

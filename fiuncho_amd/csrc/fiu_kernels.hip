@@ -888,10 +888,6 @@ __global__ void k_fp4_image(const uint4 *__restrict__ bits, DevData D, uint4 *__
     }
 }
 
-struct Fp4Image {
-    const uint4 *base;
-    uint64_t class_off[2];   // uint4 offset of each class
-};
 
 // Four bits of v (from bit `lo`) as four bytes 0/1: n * 0x00204081 puts bit i
 // of the nibble n at bit 8i (the shifted copies do not overlap), the mask
@@ -938,19 +934,21 @@ __device__ __forceinline__ void tri_tile(int64_t u, int T, int &ta, int &tb)
 // genotypes 0/1, B likewise).  fp4: the rows are copied from the dataset's
 // fp4 image (four 16-byte units per thread); int8: each thread loads one
 // 128-bit quad of the bit tables and expands it to bytes in LDS.
+// LDS of one k_mfma2 workgroup: [buffer][A/B][row][k].
+template <bool FP4> using Mfma2Lds = int8_t[2][2][2 * kMfmaTile][MfmaFmt<FP4>::row];
+
+// One workgroup's work item: tile `slot` (of 64) of super-block su; returns at
+// once for tiles outside the triangle.  Every wave of the workgroup calls it
+// with the same arguments.
 template <bool FP4>
-__global__ void __launch_bounds__(kBlock)
-k_mfma2(const DevData D, const ScanParams P, TopState *st, const Fp4Image F4, int64_t s0, int64_t s1, int T)
+__device__ __forceinline__ void mfma2_tile(const DevData &D, const ScanParams &P, TopState *st, const Fp4Image &F4,
+                                           int64_t su, int slot, int T, Mfma2Lds<FP4> &lds)
 {
     typedef MfmaFmt<FP4> F;
     typedef typename std::conditional<FP4, v16f, v16i>::type Acc;
-    __shared__ __attribute__((aligned(16))) int8_t lds[2][2][2 * kMfmaTile][F::row];  // [buf][A/B][row][k]
     const int S = (T + kMfmaSuper - 1) / kMfmaSuper;
-    const int64_t k = blockIdx.x >> 3, su = s0 + (k >> 6) * 8 + (blockIdx.x & 7);
-    if (su >= s1) return;
     int sa, sc;
     tri_tile(su, S, sa, sc);
-    const int slot = (int)(k & 63);
     const int ta = sa * kMfmaSuper + (slot >> 3), tb = sc * kMfmaSuper + (slot & 7);
     if (ta > tb || tb >= T) return;
     const int tid = threadIdx.x, lane = tid & 63;
@@ -1091,6 +1089,16 @@ k_mfma2(const DevData D, const ScanParams P, TopState *st, const Fp4Image F4, in
         }
 }
 
+template <bool FP4>
+__global__ void __launch_bounds__(kBlock)
+k_mfma2(const DevData D, const ScanParams P, TopState *st, const Fp4Image F4, int64_t s0, int64_t s1, int T)
+{
+    __shared__ __attribute__((aligned(16))) Mfma2Lds<FP4> lds;
+    const int64_t k = blockIdx.x >> 3, su = s0 + (k >> 6) * 8 + (blockIdx.x & 7);
+    if (su >= s1) return;
+    mfma2_tile<FP4>(D, P, st, F4, su, (int)(k & 63), T, lds);
+}
+
 // ---------------------------------------------------------------------------
 // Order 3 on the matrix cores.  For a fixed first SNP a, the eight counts
 // f(x, y, z), x, y, z in {0, 1}, of the triples (a, b, c) are sums over the
@@ -1158,17 +1166,12 @@ __device__ __forceinline__ void stage_barrier()
     asm volatile("" ::: "memory");
 }
 
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 2)))   // 256 registers, no spills
-k_mfma3(const DevData D, const ScanParams P, TopState *st, const Fp4Image F, int a0, int na, int T)
+// One workgroup's work item: first SNP a, tile u (see k_mfma3); returns at
+// once for items outside the triangle.  Every wave of the workgroup calls it
+// with the same arguments.
+__device__ __forceinline__ void mfma3_tile(const DevData &D, const ScanParams &P, TopState *st, const Fp4Image &F,
+                                           int a, int64_t u, int T, int8_t (*lds)[kM3Stage])
 {
-    __shared__ __attribute__((aligned(1024))) int8_t lds[kM3Ring][kM3Stage];
-    // block order: workgroup L runs on XCD L % 8; XCD k takes the tiles
-    // u = 8 i + k, i = 0, 1, ..., and for each tile all na first SNPs in a
-    // row, so that the workgroups resident on one XCD share their b and c
-    // rows in its L2 (only the a rows differ)
-    const int64_t L = blockIdx.x, s = L >> 3;
-    const int a = a0 + (int)(s % na);
-    const int64_t u = (s / na) * 8 + (L & 7);
     if (a >= D.n - 2) return;
     const int ta = (a + 1) >> 6, m = T - ta;
     if (u >= (int64_t)m * (m + 1)) return;
@@ -1345,6 +1348,18 @@ k_mfma3(const DevData D, const ScanParams P, TopState *st, const Fp4Image F, int
         });
 }
 
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 2)))   // 256 registers, no spills
+k_mfma3(const DevData D, const ScanParams P, TopState *st, const Fp4Image F, int a0, int na, int T)
+{
+    __shared__ __attribute__((aligned(1024))) int8_t lds[kM3Ring][kM3Stage];
+    // block order: workgroup L runs on XCD L % 8; XCD k takes the tiles
+    // u = 8 i + k, i = 0, 1, ..., and for each tile all na first SNPs in a
+    // row, so that the workgroups resident on one XCD share their b and c
+    // rows in its L2 (only the a rows differ)
+    const int64_t L = blockIdx.x, s = L >> 3;
+    mfma3_tile(D, P, st, F, a0 + (int)(s % na), (s / na) * 8 + (L & 7), T, lds);
+}
+
 // ---------------------------------------------------------------------------
 // Top-N merge (one workgroup): current list + candidates -> best N, unique.
 // ---------------------------------------------------------------------------
@@ -1504,6 +1519,51 @@ k_batch_scan(const BatchJob *__restrict__ jobs, const BatchSeg *__restrict__ seg
     }
 }
 
+// Matrix-core kinds of a batched step: persistent workgroups with one work
+// queue per XCD (workgroup b runs on XCD b % 8 -- the dispatch order -- and
+// takes units from queue b % 8; every queue has 1/8 of the workgroups, so all
+// are drained wherever they run).  Queue x's unit v lies in the segment whose
+// start is the last <= v; with l = v - start, order 3: first SNP j0 + l % na,
+// tile 8 (l / na) + x (the XCD-ordered work of k_mfma3); order 2: super-block
+// j0 + 8 (l / 64) + x, tile l % 64 (that of k_mfma2).
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 2)))
+k_batch_mfma(const BatchJob *__restrict__ jobs, const BatchSeg *__restrict__ segs, BatchHeader *hdr)
+{
+    __shared__ __attribute__((aligned(1024))) int8_t lds[kM3Ring * kM3Stage];
+    __shared__ uint32_t unit_sh;
+    static_assert(sizeof(Mfma2Lds<true>) <= sizeof(lds), "LDS of the order-2 tiles");
+    const int x = blockIdx.x & 7;
+    const uint32_t nseg = hdr->mseg, units = hdr->munits;
+    for (;;) {
+        if (threadIdx.x == 0) unit_sh = atomicAdd(&hdr->mhead[x], 1u);
+        __syncthreads();
+        const uint32_t v = __builtin_amdgcn_readfirstlane(unit_sh);
+        __syncthreads();   // unit_sh is rewritten only after every wave has read it
+        if (v >= units) return;
+        uint32_t lo = 0, hi = nseg - 1;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi + 1) >> 1;
+            if (segs[mid].start <= v) lo = mid;
+            else hi = mid - 1;
+        }
+        const ConstJob &J = ((const ConstJob *)jobs)[segs[lo].job];
+        const DevData D = J.D;
+        const ScanParams P = J.P;
+        const Fp4Image F = J.F;
+        TopState *st = J.st;
+        const uint32_t l = v - (uint32_t)segs[lo].start;
+        const int64_t j0 = segs[lo].j0, j1 = segs[lo].j1;
+        if (J.kind == kKindMfma3) {
+            const uint32_t na = (uint32_t)(j1 - j0);
+            mfma3_tile(D, P, st, F, (int)(j0 + l % na), (int64_t)(l / na) * 8 + x, J.T,
+                       (int8_t(*)[kM3Stage])lds);
+        } else {
+            const int64_t su = j0 + (int64_t)(l / 64) * 8 + x;
+            if (su < j1) mfma2_tile<true>(D, P, st, F, su, (int)(l % 64), J.T, *(Mfma2Lds<true> *)lds);
+        }
+    }
+}
+
 // One workgroup per job of the launch: merge its candidates into its top-N
 // and record whether its candidate buffer overflowed in this launch.
 __global__ void __launch_bounds__(1024)
@@ -1526,6 +1586,7 @@ __global__ void k_batch_reset(BatchHeader *hdr)
         hdr->head[0] = 0;
         hdr->head[1] = 0;
     }
+    if (threadIdx.x < 8) hdr->mhead[threadIdx.x] = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -1714,19 +1775,23 @@ int batch_blocks(int launch)
     int dev = 0, cus = 0, per_cu = 0;
     if (hipGetDevice(&dev) != hipSuccess) return 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
-    const hipError_t e = launch == 0
-                             ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_scan<0>, kBlock, 0)
-                             : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_scan<1>, kBlock, 0);
+    const hipError_t e =
+        launch == 0   ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_scan<0>, kBlock, 0)
+        : launch == 1 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_scan<1>, kBlock, 0)
+                      : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_mfma, kBlock, 0);
     if (e != hipSuccess) return 0;
-    return std::max(1, per_cu) * cus;
+    const int blocks = std::max(1, per_cu) * cus;
+    return launch == 2 ? std::max(8, blocks / 8 * 8) : blocks;   // matrix-core queues: a multiple of 8
 }
 
-hipError_t launch_batch(const BatchJob *jobs, const BatchSeg *segs0, const BatchSeg *segs1, const int32_t *merge_list,
-                        BatchHeader *hdr, int blocks0, int blocks1, int merge_blocks, hipStream_t s)
+hipError_t launch_batch(const BatchJob *jobs, const BatchSeg *segs0, const BatchSeg *segs1, const BatchSeg *segs2,
+                        const int32_t *merge_list, BatchHeader *hdr, int blocks0, int blocks1, int blocks2,
+                        int merge_blocks, hipStream_t s)
 {
     hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, hdr);
     hipLaunchKernelGGL((k_batch_scan<0>), dim3(blocks0), dim3(kBlock), 0, s, jobs, segs0, hdr);
     if (blocks1 > 0) hipLaunchKernelGGL((k_batch_scan<1>), dim3(blocks1), dim3(kBlock), 0, s, jobs, segs1, hdr);
+    if (blocks2 > 0) hipLaunchKernelGGL(k_batch_mfma, dim3(blocks2), dim3(kBlock), 0, s, jobs, segs2, hdr);
     hipLaunchKernelGGL(k_batch_merge, dim3(merge_blocks), dim3(1024), 0, s, jobs, merge_list, hdr);
     return hipGetLastError();
 }

@@ -210,6 +210,7 @@ def test_server_resident_dataset_and_poll():
         out = srv.wait(rid, timeout_ms=50.0)
     assert same(out[:2], plain(d, 3, fa.Distribution(250, 2), 25))
     assert out[2].combinations == math.comb(250, 3)
+    assert out[2].algorithm == fa.ALGO_MFMA   # whole order-3 scans run on the matrix cores
     with pytest.raises(fa.FiuError):
         srv.submit(order=9, outputs=5, dataset=d)
     srv.close()
