@@ -187,8 +187,9 @@ fiu_status fiu_mi_batch(const uint32_t *tables, size_t n, uint16_t order,
 /* ------------------------------------------------------------------------ */
 /* No counterpart in the reference, which runs one search per process
  * (MPIEngine.h:135-137).  Many searches over different datasets, sizes and
- * orders advance together through batched device steps: one persistent
- * launch over a slice of every active search, then one top-N merge per
+ * orders advance together through batched device steps: persistent launches
+ * over a slice of every active search (popcount kinds, order 4, matrix-core
+ * kinds for order 2 and whole order-3 scans), then one top-N merge per
  * search, replayed from a HIP graph.  Each search returns exactly what
  * fiu_search returns for it alone. */
 
@@ -220,7 +221,9 @@ typedef struct {
 
 typedef struct {
     uint64_t steps;          /* batched steps launched */
-    uint64_t units;          /* wavefront work units executed */
+    uint64_t units;          /* work units claimed: wavefront units of the
+                                popcount kinds, workgroup units (tiles) of
+                                the matrix-core kinds */
     uint64_t redo_ranges;    /* prefix ranges scanned again (candidate overflow) */
     double device_ms;        /* device time of the steps (HIP events) */
     double total_ms;         /* wall time of the call */
@@ -236,8 +239,9 @@ fiu_status fiu_search_batch(fiu_scan *scans, size_t n, const fiu_batch_config *c
  * while new requests are admitted.  A request names a resident dataset, or
  * carries host tables (reference layout, copied at submit) that the server
  * uploads into its memory budget when the request's footprint (tables, pair
- * table, binomials) fits, and frees when the request completes.  Requests
- * that can never fit are refused with FIU_ERR_REJECTED. */
+ * table, binomials) fits, and frees when the request completes; the fp4
+ * image of the matrix-core scans is added only when it fits as well.
+ * Requests that can never fit are refused with FIU_ERR_REJECTED. */
 typedef struct fiu_server fiu_server;
 
 typedef struct {
