@@ -198,12 +198,13 @@ class Result:
 # Search
 # ---------------------------------------------------------------------------
 def Config(algorithm=ALGO_AUTO, chunk_combos=0, candidate_cap=0, stream=None, sample_threshold=True,
-           mfma_int8=False):
+           mfma_int8=False, variant=0):
     cfg = _lib.Config()
     lib().fiu_config_init(C.byref(cfg))
     cfg.algorithm, cfg.chunk_combos, cfg.candidate_cap = algorithm, chunk_combos, candidate_cap
     cfg.flags = (0 if sample_threshold else _lib.FLAG_NO_THRESHOLD_SAMPLE) | (_lib.FLAG_MFMA_INT8 if mfma_int8 else 0)
     cfg.stream = stream
+    cfg.variant = variant
     return cfg
 
 
@@ -211,9 +212,9 @@ class GpuSearch:
     """Search on one MI355X (drop-in for ThreadedSearch, ThreadedSearch.h:28)."""
 
     def __init__(self, algorithm=ALGO_AUTO, chunk_combos=0, candidate_cap=0, sample_threshold=True,
-                 mfma_int8=False):
+                 mfma_int8=False, variant=0):
         self.config = Config(algorithm, chunk_combos, candidate_cap, sample_threshold=sample_threshold,
-                             mfma_int8=mfma_int8)
+                             mfma_int8=mfma_int8, variant=variant)
         self.last_stats = None
 
     def run_arrays(self, dataset: Dataset, order: int, distribution: Distribution, outputs: int):

@@ -31,7 +31,7 @@ EXPORTS = (
     "fiu_dataset_destroy", "fiu_config_init", "fiu_search", "fiu_ct_size",
     "fiu_combine_batch", "fiu_contingency_batch", "fiu_mi_batch",
     "fiu_batch_config_init", "fiu_search_batch", "fiu_server_create", "fiu_server_submit",
-    "fiu_server_wait", "fiu_server_query", "fiu_server_destroy",
+    "fiu_server_wait", "fiu_server_query", "fiu_server_destroy", "fiu_set_tuning", "fiu_get_tuning",
     "fiu_synth_spec_init", "fiu_synth_generate", "fiu_dataset_synth",
 )
 
@@ -45,7 +45,14 @@ class FiuError(RuntimeError):
 class Config(C.Structure):
     _fields_ = [("struct_size", C.c_uint32), ("algorithm", C.c_int32),
                 ("chunk_combos", C.c_uint64), ("candidate_cap", C.c_uint32),
-                ("flags", C.c_uint32), ("stream", C.c_void_p)]
+                ("flags", C.c_uint32), ("stream", C.c_void_p), ("variant", C.c_int32),
+                ("reserved", C.c_int32)]
+
+
+class TuneEntry(C.Structure):
+    _fields_ = [("order", C.c_uint16), ("reserved", C.c_uint16), ("snps_lo", C.c_uint32),
+                ("snps_hi", C.c_uint32), ("samples_lo", C.c_uint32), ("samples_hi", C.c_uint32),
+                ("algorithm", C.c_int32), ("variant", C.c_int32), ("ms", C.c_float)]
 
 
 FLAG_NO_THRESHOLD_SAMPLE = 1
@@ -55,7 +62,7 @@ FLAG_MFMA_INT8 = 2
 class SearchStats(C.Structure):
     _fields_ = [("combinations", C.c_uint64), ("launches", C.c_uint64),
                 ("redo_chunks", C.c_uint64), ("scan_ms", C.c_double),
-                ("total_ms", C.c_double), ("algorithm", C.c_int32), ("reserved", C.c_int32)]
+                ("total_ms", C.c_double), ("algorithm", C.c_int32), ("variant", C.c_int32)]
 
 
 class BatchConfig(C.Structure):
@@ -146,6 +153,8 @@ def lib():
     L.fiu_server_wait.argtypes = [vp, C.c_uint64, C.c_double, vp, vp, C.POINTER(u32), C.POINTER(RequestStats)]
     L.fiu_server_query.argtypes = [vp, C.POINTER(ServerInfo)]
     L.fiu_server_destroy.argtypes = [vp]
+    L.fiu_set_tuning.argtypes = [C.POINTER(TuneEntry), sz]
+    L.fiu_get_tuning.argtypes = [C.POINTER(TuneEntry), sz, C.POINTER(sz)]
     L.fiu_synth_spec_init.argtypes = [C.POINTER(SynthSpec)]
     L.fiu_synth_spec_init.restype = None
     L.fiu_synth_generate.argtypes = [C.POINTER(SynthSpec), vp, vp, C.POINTER(sz), C.POINTER(sz)]

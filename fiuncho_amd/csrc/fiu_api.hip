@@ -5,11 +5,13 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstddef>
 #include <chrono>
 #include <cmath>
 #include <cstring>
 #include <functional>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -338,6 +340,33 @@ void enqueue_upload(fiu_dataset *d, StagedUpload &up, hipStream_t s)
     up.staging_d = nullptr;
 }
 
+namespace {
+std::mutex g_tuning_mu;
+std::vector<fiu_tune_entry> g_tuning;
+}  // namespace
+
+bool valid_variant(int algo, int order, int variant)
+{
+    if (variant == 0) return true;
+    switch (algo) {
+    case FIU_ALGO_MARGINAL: return variant == 2 || variant == kMarginalPW || (order == 2 && variant == 8);
+    case FIU_ALGO_MFMA: return order == 3 ? (variant == 3 || variant == 4) : variant == 1;
+    default: return false;
+    }
+}
+
+bool tuned(int order, uint64_t snps, uint64_t samples, fiu_tune_entry &out)
+{
+    std::lock_guard<std::mutex> lk(g_tuning_mu);
+    for (const auto &e : g_tuning)
+        if (e.order == order && snps >= e.snps_lo && snps < e.snps_hi && samples >= e.samples_lo &&
+            samples < e.samples_hi) {
+            out = e;
+            return true;
+        }
+    return false;
+}
+
 void mi_constants(uint32_t cases, uint32_t ctrls, bool complete, ScanParams &P)
 {
     P.bounded = complete ? 1u : 0u;
@@ -503,12 +532,28 @@ void search(fiu_dataset *d, int order, int32_t step, int32_t offset, uint32_t ou
     if (order > kMaxOrder) fail(FIU_ERR_UNSUPPORTED, "order " + std::to_string(order) + " is above this build's limit");
     if (outputs > (uint32_t)kMaxOutputs)
         fail(FIU_ERR_UNSUPPORTED, "outputs above " + std::to_string(kMaxOutputs));
-    const int algo_req = cfg ? cfg->algorithm : FIU_ALGO_AUTO;
+    int algo_req = cfg ? cfg->algorithm : FIU_ALGO_AUTO;
+    int variant = cfg && cfg->struct_size >= offsetof(fiu_config, variant) + sizeof(int32_t) ? cfg->variant : 0;
     if (algo_req < FIU_ALGO_AUTO || algo_req > FIU_ALGO_MFMA) fail(FIU_ERR_INVALID, "unknown algorithm");
     if ((algo_req == FIU_ALGO_MARGINAL || algo_req == FIU_ALGO_MFMA) && !d->dev.uniform)
         fail(FIU_ERR_INVALID, "marginal algorithm needs every SNP to cover the same samples");
     if (algo_req == FIU_ALGO_MFMA && order != 2 && order != 3)
         fail(FIU_ERR_UNSUPPORTED, "the matrix-core scan covers orders 2 and 3");
+    if (algo_req != FIU_ALGO_AUTO && !valid_variant(algo_req, order, variant))
+        fail(FIU_ERR_INVALID, "unknown variant " + std::to_string(variant) + " of this algorithm");
+    // AUTO: the tuned choice of this shape class when the dataset allows it
+    // (order-3 matrix-core entries are timed on whole scans)
+    bool from_table = false;
+    fiu_tune_entry te{};
+    if (algo_req == FIU_ALGO_AUTO && tuned(order, d->snps, (uint64_t)d->cases + d->ctrls, te)) {
+        const bool allowed = te.algorithm == FIU_ALGO_DIRECT ||
+                             (d->dev.uniform && (te.algorithm != FIU_ALGO_MFMA || order == 2 || step == 1));
+        if (allowed) {
+            algo_req = te.algorithm;
+            if (variant == 0) variant = te.variant;
+            from_table = true;
+        }
+    }
     fiu_search_stats st{};
     st.algorithm = FIU_ALGO_DIRECT;
     const int n = (int)d->snps, M = order - 1;
@@ -541,14 +586,19 @@ void search(fiu_dataset *d, int order, int32_t step, int32_t offset, uint32_t ou
             (algo_req == FIU_ALGO_MFMA || (algo_req == FIU_ALGO_AUTO && step == 1)) && ensure_fp4_image(d, s))
             algo = FIU_ALGO_MFMA;
     }
+    if (algo != algo_req && from_table) variant = 0;   // the tuned algorithm did not apply
+    if (!valid_variant(algo, order, variant)) variant = 0;
     st.algorithm = algo;
+    st.variant = variant;
+    const int pw = algo == FIU_ALGO_MARGINAL && variant ? variant : kMarginalPW;
+    const int ring = algo == FIU_ALGO_MFMA && order == 3 && variant ? variant : 4;
     // matrix-core scans walk super-blocks of 8 x 8 tiles of 64 x 64 pairs;
     // fp4 operands accumulate in fp32, exact while every count is below 2^24;
     // they come from the dataset's fp4 image (int8 when it does not fit)
     const int T = d->dev.n_pad / 64;
     const int64_t n_super = mfma2_superblocks(T);
-    const bool mfma_fp4 = algo == FIU_ALGO_MFMA && !(cfg && (cfg->flags & FIU_FLAG_MFMA_INT8)) && small_classes &&
-                          ensure_fp4_image(d, s);
+    const bool int8 = (cfg && (cfg->flags & FIU_FLAG_MFMA_INT8)) || (order == 2 && variant == 1);
+    const bool mfma_fp4 = algo == FIU_ALGO_MFMA && !int8 && small_classes && ensure_fp4_image(d, s);
 
     const uint32_t cap = cfg && cfg->candidate_cap ? cfg->candidate_cap : (1u << 20);
     d->cand.alloc(cap);
@@ -627,10 +677,10 @@ void search(fiu_dataset *d, int order, int32_t step, int32_t offset, uint32_t ou
                                s),
                   "scan launch");
         } else if (algo == FIU_ALGO_MFMA) {
-            check(launch_mfma3(d->dev, P, d->state.p, d->fp4.p, (int)chunks[ci].j0, (int)chunks[ci].j1, T, s),
+            check(launch_mfma3(d->dev, P, d->state.p, d->fp4.p, (int)chunks[ci].j0, (int)chunks[ci].j1, T, ring, s),
                   "scan launch");
         } else if (algo == FIU_ALGO_MARGINAL) {
-            check(launch_scan_marginal(order, d->dev, P, d->state.p, s), "scan launch");
+            check(launch_scan_marginal(order, d->dev, P, d->state.p, s, pw), "scan launch");
         } else {
             check(launch_scan_direct(order, d->dev, P, d->state.p, s), "scan launch");
         }
@@ -719,6 +769,39 @@ fiu_status fiu_set_device(int device)
         check(hipGetDeviceCount(&count), "hipGetDeviceCount");
         if (device < 0 || device >= count) fail(FIU_ERR_INVALID, "no such device");
         current_device() = device;
+    });
+}
+
+fiu_status fiu_set_tuning(const fiu_tune_entry *entries, size_t n)
+{
+    return guard([&] {
+        if (n && !entries) fail(FIU_ERR_INVALID, "entries is NULL");
+        for (size_t i = 0; i < n; ++i) {
+            const fiu_tune_entry &e = entries[i];
+            if (e.order < 2 || e.order > kMaxOrder) fail(FIU_ERR_INVALID, "tuning entry with an unsupported order");
+            if (e.algorithm < FIU_ALGO_DIRECT || e.algorithm > FIU_ALGO_MFMA)
+                fail(FIU_ERR_INVALID, "tuning entry with an unknown algorithm");
+            if (e.algorithm == FIU_ALGO_MFMA && e.order != 2 && e.order != 3)
+                fail(FIU_ERR_INVALID, "matrix-core tuning entry above order 3");
+            if (e.algorithm == FIU_ALGO_MARGINAL && e.order > 3)
+                fail(FIU_ERR_INVALID, "marginal tuning entry above order 3");
+            if (!valid_variant(e.algorithm, e.order, e.variant)) fail(FIU_ERR_INVALID, "tuning entry with an unknown variant");
+            if (e.snps_lo >= e.snps_hi || e.samples_lo >= e.samples_hi)
+                fail(FIU_ERR_INVALID, "tuning entry with an empty range");
+        }
+        std::lock_guard<std::mutex> lk(g_tuning_mu);
+        g_tuning.assign(entries, entries + n);
+    });
+}
+
+fiu_status fiu_get_tuning(fiu_tune_entry *entries, size_t cap, size_t *n)
+{
+    return guard([&] {
+        if (!n) fail(FIU_ERR_INVALID, "n is NULL");
+        if (cap && !entries) fail(FIU_ERR_INVALID, "entries is NULL");
+        std::lock_guard<std::mutex> lk(g_tuning_mu);
+        *n = g_tuning.size();
+        std::copy(g_tuning.begin(), g_tuning.begin() + std::min(cap, g_tuning.size()), entries);
     });
 }
 

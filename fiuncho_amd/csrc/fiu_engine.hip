@@ -343,7 +343,12 @@ void Engine::plan_job(Job *j)
     // the matrix cores take order 2 (any distribution: the tiles mask other
     // shares) and whole order-3 scans, as the single search does by default
     const int T = d->dev.n_pad / 64;
-    if (mfma_dataset(d) && d->fp4_ready && (order == 2 || (order == 3 && d->pair_ready && j->step == 1))) {
+    // a tuning entry for this shape class picks the algorithm (the batched
+    // kernels have one variant each)
+    fiu_tune_entry te{};
+    const int tuned_algo = tuned(order, d->snps, (uint64_t)d->cases + d->ctrls, te) ? te.algorithm : FIU_ALGO_AUTO;
+    if ((tuned_algo == FIU_ALGO_AUTO || tuned_algo == FIU_ALGO_MFMA) && mfma_dataset(d) && d->fp4_ready &&
+        (order == 2 || (order == 3 && d->pair_ready && j->step == 1))) {
         j->kind = order == 2 ? kKindMfma2 : kKindMfma3;
         j->pw = 1;
         j->n_pref = order == 2 ? mfma2_superblocks(T) : std::max(0, n - 2);
@@ -354,7 +359,8 @@ void Engine::plan_job(Job *j)
         j->rs.algorithm = FIU_ALGO_MFMA;
         return;
     }
-    const bool marginal = d->dev.uniform && (order == 2 || (order == 3 && d->pair_ready));
+    const bool marginal =
+        tuned_algo != FIU_ALGO_DIRECT && d->dev.uniform && (order == 2 || (order == 3 && d->pair_ready));
     j->kind = marginal ? (order == 2 ? kKindScan2m : kKindScan3m)
                        : (order == 2 ? kKindDirect2 : order == 3 ? kKindDirect3 : kKindDirect4);
     j->pw = batch_group(j->kind);

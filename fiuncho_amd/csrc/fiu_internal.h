@@ -125,6 +125,12 @@ int bit_width(uint64_t x);
 // = offset (mod step).
 uint64_t count_combinations(uint64_t n, int m, uint64_t step, uint64_t offset);
 
+// Tuning table (fiu_set_tuning): the entry of the class holding (order,
+// snps, samples), if any.
+bool tuned(int order, uint64_t snps, uint64_t samples, fiu_tune_entry &out);
+// `variant` is one of `algo`'s kernel variants at `order` (0 = default).
+bool valid_variant(int algo, int order, int variant);
+
 // H(Y), 1/N and the screening margin of a dataset's class sizes.  `complete`:
 // every sample has a genotype in {0, 1, 2} at every SNP, so that a table's
 // cells sum to the class sizes and the MI upper bound applies.

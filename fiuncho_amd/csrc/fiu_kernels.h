@@ -6,7 +6,10 @@
 namespace fiu {
 
 hipError_t launch_scan_direct(int order, const DevData &D, const ScanParams &P, TopState *st, hipStream_t s);
-hipError_t launch_scan_marginal(int order, const DevData &D, const ScanParams &P, TopState *st, hipStream_t s);
+// pw: prefixes per wavefront (2, 4 or, order 2 only, 8; the threshold
+// sampling pass uses 4)
+hipError_t launch_scan_marginal(int order, const DevData &D, const ScanParams &P, TopState *st, hipStream_t s,
+                                int pw = kMarginalPW);
 hipError_t launch_tau_from_hist(const uint32_t *hist, uint32_t N, float delta, TopState *st, hipStream_t s);
 hipError_t launch_pair_table(const DevData &D, const ScanParams &P, uint32_t *pair_out, hipStream_t s);
 // Order 2 on the matrix cores: the pair triangle of T x T 64-SNP tiles is cut
@@ -20,9 +23,10 @@ int64_t mfma2_superblocks(int T);
 uint64_t fp4_image_units(const DevData &D, int cls);
 hipError_t launch_fp4_image(const DevData &D, uint4 *out, hipStream_t s);
 // Order 3 on the matrix cores (fp4): first SNPs [a0, a1), every (b, c) past
-// them, with the pair table in P.pair and the dataset's fp4 image.
+// them, with the pair table in P.pair and the dataset's fp4 image; `ring`
+// LDS stages (3 or 4).
 hipError_t launch_mfma3(const DevData &D, const ScanParams &P, TopState *st, const uint4 *fp4, int a0, int a1, int T,
-                        hipStream_t s);
+                        int ring, hipStream_t s);
 hipError_t launch_mfma2(const DevData &D, const ScanParams &P, TopState *st, int64_t s0, int64_t s1, int T,
                         const uint4 *fp4, hipStream_t s);
 hipError_t launch_merge(TopState *st, Cand *top, const Cand *cand, uint32_t N, uint32_t cap, uint32_t chunk_id,

@@ -1134,7 +1134,7 @@ template <int N, class F> __device__ __forceinline__ void static_for(F &&f)
 // ((r >> 1) ^ (r >> 3)) & 3.
 constexpr int kM3Group = 16 * 64;
 constexpr int kM3Stage = 13 * kM3Group;
-constexpr int kM3Ring = 4;   // stages resident: the one read + up to 3 loading
+constexpr int kM3Ring = 4;   // default stages resident: the one read + up to 3 loading
 __device__ __forceinline__ int m3_swz(int r) { return (r >> 1) & 3; }
 
 // Direct-to-LDS buffer load of 16 bytes per lane from byte offset voff of
@@ -1155,6 +1155,20 @@ template <int N> __device__ __forceinline__ void wait_vmcnt()
     __builtin_amdgcn_s_waitcnt(0x0f70 | N);
 }
 
+// wait_vmcnt<L * s>() for a runtime, wave-uniform s (clamped to 0 .. S).
+template <int S, int L> __device__ __forceinline__ void wait_stages(int s)
+{
+    if constexpr (S > 0) {
+        if (s >= S) {
+            wait_vmcnt<L * S>();
+            return;
+        }
+        wait_stages<S - 1, L>(s);
+    } else {
+        wait_vmcnt<0>();
+    }
+}
+
 // Workgroup barrier with no memory fence (a fence would drain every pending
 // direct-to-LDS load, including the ones meant to stay in flight); the
 // waits are explicit, the empty asm statements keep the compiler from moving
@@ -1168,7 +1182,8 @@ __device__ __forceinline__ void stage_barrier()
 
 // One workgroup's work item: first SNP a, tile u (see k_mfma3); returns at
 // once for items outside the triangle.  Every wave of the workgroup calls it
-// with the same arguments.
+// with the same arguments.  R: stages of the LDS ring (R - 1 in flight).
+template <int R>
 __device__ __forceinline__ void mfma3_tile(const DevData &D, const ScanParams &P, TopState *st, const Fp4Image &F,
                                            int a, int64_t u, int T, int8_t (*lds)[kM3Stage])
 {
@@ -1219,26 +1234,23 @@ __device__ __forceinline__ void mfma3_tile(const DevData &D, const ScanParams &P
             // resource over quad q of the class image (8 n_pad uint4)
             const __amdgpu_buffer_rsrc_t rq =
                 __builtin_amdgcn_make_buffer_rsrc((void *)(img + q * qstride), (short)0, 128 * np, 0x00020000);
-            int8_t *s = lds[q % kM3Ring];
+            int8_t *s = lds[q % R];
             dma16(rq, ub0, s + (2 * w) * kM3Group);
             dma16(rq, ub1, s + (2 * w + 1) * kM3Group);
             dma16(rq, uc, s + (8 + w) * kM3Group);
             if (a_lane) dma16(rq, ua, s + 12 * kM3Group);
         };
 #pragma unroll
-        for (int q = 0; q < kM3Ring - 1; ++q)
+        for (int q = 0; q < R - 1; ++q)
             if (q < iters) issue(q);
         for (int it = 0; it < iters; ++it) {
             // this wave's loads of stage it are complete once at most those of
-            // the stages after it (min(iters - 1 - it, kM3Ring - 2), 4 loads
-            // each) are pending; the barrier then publishes every wave's part
-            static_assert(kM3Ring == 4, "wait counts below");
-            const int ahead = min(iters - 1 - it, kM3Ring - 2);
-            if (ahead >= 2) wait_vmcnt<8>();
-            else if (ahead == 1) wait_vmcnt<4>();
-            else wait_vmcnt<0>();
+            // the stages after it (min(iters - 1 - it, R - 2), 4 loads each)
+            // are pending; the barrier then publishes every wave's part
+            static_assert(R >= 2 && 4 * (R - 2) < 16, "ring depth");
+            wait_stages<R - 2, 4>(iters - 1 - it);
             stage_barrier();
-            const int8_t *sb = lds[it % kM3Ring];
+            const int8_t *sb = lds[it % R];
             v4i ma[2], fb[2][2], fc[2];
 #pragma unroll
             for (int x = 0; x < 2; ++x) ma[x] = *(const v4i *)(sb + 12 * kM3Group + x * 64 + ao);
@@ -1248,8 +1260,8 @@ __device__ __forceinline__ void mfma3_tile(const DevData &D, const ScanParams &P
                 for (int bb = 0; bb < 2; ++bb) fb[y][bb] = *(const v4i *)(sb + (4 * y + 2 * wb + bb) * kM3Group + fo);
 #pragma unroll
             for (int z = 0; z < 2; ++z) fc[z] = *(const v4i *)(sb + (8 + 2 * z + wc) * kM3Group + fo);
-            // every wave is past stage it - 1, whose slot now takes stage it + 3
-            if (it + kM3Ring - 1 < iters) issue(it + kM3Ring - 1);
+            // every wave is past stage it - 1, whose slot now takes stage it + R - 1
+            if (it + R - 1 < iters) issue(it + R - 1);
             // the a mask goes on the narrower operand (a wave's 16 c rows per
             // z, against 32 b rows per y): 4 masked fragments per stage
 #pragma unroll
@@ -1348,16 +1360,17 @@ __device__ __forceinline__ void mfma3_tile(const DevData &D, const ScanParams &P
         });
 }
 
+template <int R>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 2)))   // 256 registers, no spills
 k_mfma3(const DevData D, const ScanParams P, TopState *st, const Fp4Image F, int a0, int na, int T)
 {
-    __shared__ __attribute__((aligned(1024))) int8_t lds[kM3Ring][kM3Stage];
+    __shared__ __attribute__((aligned(1024))) int8_t lds[R][kM3Stage];
     // block order: workgroup L runs on XCD L % 8; XCD k takes the tiles
     // u = 8 i + k, i = 0, 1, ..., and for each tile all na first SNPs in a
     // row, so that the workgroups resident on one XCD share their b and c
     // rows in its L2 (only the a rows differ)
     const int64_t L = blockIdx.x, s = L >> 3;
-    mfma3_tile(D, P, st, F, a0 + (int)(s % na), (s / na) * 8 + (L & 7), T, lds);
+    mfma3_tile<R>(D, P, st, F, a0 + (int)(s % na), (s / na) * 8 + (L & 7), T, lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -1555,7 +1568,7 @@ k_batch_mfma(const BatchJob *__restrict__ jobs, const BatchSeg *__restrict__ seg
         const int64_t j0 = segs[lo].j0, j1 = segs[lo].j1;
         if (J.kind == kKindMfma3) {
             const uint32_t na = (uint32_t)(j1 - j0);
-            mfma3_tile(D, P, st, F, (int)(j0 + l % na), (int64_t)(l / na) * 8 + x, J.T,
+            mfma3_tile<kM3Ring>(D, P, st, F, (int)(j0 + l % na), (int64_t)(l / na) * 8 + x, J.T,
                        (int8_t(*)[kM3Stage])lds);
         } else {
             const int64_t su = j0 + (int64_t)(l / 64) * 8 + x;
@@ -1748,26 +1761,36 @@ hipError_t launch_scan_direct(int order, const DevData &D, const ScanParams &P, 
     return P.hist ? scan_direct<true>(order, D, P, st, s) : scan_direct<false>(order, D, P, st, s);
 }
 
-template <bool HIST>
+template <int PW, bool HIST>
 static hipError_t scan_marginal(int order, const DevData &D, const ScanParams &P, TopState *st, hipStream_t s)
 {
     const int64_t prefixes = P.j1 - P.j0;
     if (prefixes <= 0) return hipSuccess;
-    const int64_t per_block = (kBlock / kWave) * kMarginalPW;
+    const int64_t per_block = (kBlock / kWave) * PW;
     const dim3 grid((unsigned)((prefixes + per_block - 1) / per_block)), block(kBlock);
     switch (order) {
-    case 2:
-        hipLaunchKernelGGL((k_scan2m<kMarginalPW, false, HIST>), grid, block, 0, s, D, P, st, (uint32_t *)nullptr);
-        break;
-    case 3: hipLaunchKernelGGL((k_scan3m<kMarginalPW, HIST>), grid, block, 0, s, D, P, st); break;
+    case 2: hipLaunchKernelGGL((k_scan2m<PW, false, HIST>), grid, block, 0, s, D, P, st, (uint32_t *)nullptr); break;
+    case 3:
+        if constexpr (PW <= kMarginalPW) {   // 8 prefixes per wave do not fit order 3's registers
+            hipLaunchKernelGGL((k_scan3m<PW, HIST>), grid, block, 0, s, D, P, st);
+            break;
+        }
+        return hipErrorInvalidValue;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
-hipError_t launch_scan_marginal(int order, const DevData &D, const ScanParams &P, TopState *st, hipStream_t s)
+hipError_t launch_scan_marginal(int order, const DevData &D, const ScanParams &P, TopState *st, hipStream_t s,
+                                int pw)
 {
-    return P.hist ? scan_marginal<true>(order, D, P, st, s) : scan_marginal<false>(order, D, P, st, s);
+    if (P.hist) return scan_marginal<kMarginalPW, true>(order, D, P, st, s);
+    switch (pw) {
+    case 2: return scan_marginal<2, false>(order, D, P, st, s);
+    case kMarginalPW: return scan_marginal<kMarginalPW, false>(order, D, P, st, s);
+    case 8: return scan_marginal<8, false>(order, D, P, st, s);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 int batch_blocks(int launch)
@@ -1836,7 +1859,7 @@ hipError_t launch_fp4_image(const DevData &D, uint4 *out, hipStream_t s)
 }
 
 hipError_t launch_mfma3(const DevData &D, const ScanParams &P, TopState *st, const uint4 *fp4, int a0, int a1, int T,
-                        hipStream_t s)
+                        int ring, hipStream_t s)
 {
     if (a1 <= a0) return hipSuccess;
     const int m = T - ((a0 + 1) >> 6);   // tiles from the first SNP's tile on (the most for a0)
@@ -1847,7 +1870,10 @@ hipError_t launch_mfma3(const DevData &D, const ScanParams &P, TopState *st, con
     F.class_off[1] = fp4_image_units(D, 0);
     const int64_t tiles8 = ((int64_t)m * (m + 1) + 7) / 8, na = a1 - a0;
     if (tiles8 * na * 8 > 0x7fffffffLL) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_mfma3, dim3((unsigned)(tiles8 * na * 8)), dim3(kBlock), 0, s, D, P, st, F, a0, (int)na, T);
+    const dim3 grid((unsigned)(tiles8 * na * 8)), block(kBlock);
+    if (ring == 3) hipLaunchKernelGGL((k_mfma3<3>), grid, block, 0, s, D, P, st, F, a0, (int)na, T);
+    else if (ring == kM3Ring) hipLaunchKernelGGL((k_mfma3<kM3Ring>), grid, block, 0, s, D, P, st, F, a0, (int)na, T);
+    else return hipErrorInvalidValue;
     return hipGetLastError();
 }
 

@@ -1,0 +1,166 @@
+"""Autotuner of the shape-class table (``fiu_set_tuning``).
+
+For each shape class (order, range of SNP counts, range of sample counts) the
+tuner builds a representative synthetic dataset on the device, times every
+candidate kernel -- algorithm and variant: the matrix-core scans with their
+LDS staging depth (order 3) or operand format (order 2), the popcount scans
+with their prefixes per wavefront -- on the whole scan, requires every
+candidate to return the identical top-N list, and keeps the fastest.  The
+table is loaded into the library, where AUTO searches and batched scans of
+that class use it.
+
+    entries = tune.autotune()          # times the default classes, loads the table
+    tune.save(entries, "tuning.json")  # later: tune.load("tuning.json")
+"""
+from __future__ import annotations
+
+import ctypes as C
+import json
+import time
+from dataclasses import asdict, dataclass
+from typing import Iterable, List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from . import _lib
+from ._lib import ALGO_DIRECT, ALGO_MARGINAL, ALGO_MFMA, FiuError, check, lib
+
+INF = 2 ** 31 - 1
+
+
+@dataclass
+class Entry:
+    """One class of the table: [snps_lo, snps_hi) x [samples_lo, samples_hi)."""
+    order: int
+    snps_lo: int
+    snps_hi: int
+    samples_lo: int
+    samples_hi: int
+    algorithm: int
+    variant: int = 0
+    ms: float = 0.0
+
+
+@dataclass
+class ShapeClass:
+    order: int
+    snps_lo: int
+    snps_hi: int
+    samples_lo: int
+    samples_hi: int
+    snps: int       # representative dataset timed for the class
+    samples: int
+
+
+def _classes(order, snp_edges, snp_reps, sample_edges, sample_reps):
+    out = []
+    for (slo, shi), s in zip(zip(snp_edges[:-1], snp_edges[1:]), snp_reps):
+        for (nlo, nhi), m in zip(zip(sample_edges[:-1], sample_edges[1:]), sample_reps):
+            out.append(ShapeClass(order, slo, shi, nlo, nhi, s, m))
+    return out
+
+
+# Classes covering every order-2 and order-3 shape; each representative is
+# timed on its whole scan (a few hundred milliseconds at most).
+DEFAULT_CLASSES: List[ShapeClass] = (
+    _classes(2, [0, 2048, 16384, INF], [1000, 6000, 20000], [0, 8192, INF], [4000, 16000])
+    + _classes(3, [0, 512, 2048, INF], [300, 1000, 2000], [0, 8192, INF], [4000, 16000]))
+
+
+def candidates(order: int) -> List[Tuple[int, int]]:
+    """(algorithm, variant) pairs the tuner times at `order`."""
+    if order == 2:
+        return [(ALGO_MFMA, 0), (ALGO_MFMA, 1), (ALGO_MARGINAL, 2), (ALGO_MARGINAL, 4), (ALGO_MARGINAL, 8)]
+    if order == 3:
+        return [(ALGO_MFMA, 3), (ALGO_MFMA, 4), (ALGO_MARGINAL, 2), (ALGO_MARGINAL, 4)]
+    return [(ALGO_DIRECT, 0)]
+
+
+def set_table(entries: Sequence[Entry]) -> None:
+    """Load `entries` as the library's table (an empty list clears it)."""
+    arr = (_lib.TuneEntry * max(len(entries), 1))()
+    for i, e in enumerate(entries):
+        t = arr[i]
+        t.order, t.snps_lo, t.snps_hi = e.order, e.snps_lo, e.snps_hi
+        t.samples_lo, t.samples_hi = e.samples_lo, e.samples_hi
+        t.algorithm, t.variant, t.ms = e.algorithm, e.variant, e.ms
+    check(lib().fiu_set_tuning(arr, len(entries)))
+
+
+def get_table() -> List[Entry]:
+    n = C.c_size_t()
+    check(lib().fiu_get_tuning(None, 0, C.byref(n)))
+    arr = (_lib.TuneEntry * max(n.value, 1))()
+    check(lib().fiu_get_tuning(arr, n.value, C.byref(n)))
+    return [Entry(t.order, t.snps_lo, t.snps_hi, t.samples_lo, t.samples_hi, t.algorithm, t.variant, t.ms)
+            for t in arr[:n.value]]
+
+
+def clear() -> None:
+    set_table([])
+
+
+def save(entries: Sequence[Entry], path: str) -> None:
+    with open(path, "w") as f:
+        json.dump([asdict(e) for e in entries], f, indent=1)
+
+
+def load(path: str, apply: bool = True) -> List[Entry]:
+    with open(path) as f:
+        entries = [Entry(**e) for e in json.load(f)]
+    if apply:
+        set_table(entries)
+    return entries
+
+
+def _time(search, d, order, dist, outputs, repeats):
+    best = float("inf")
+    res = None
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        res = search.run_arrays(d, order, dist, outputs)
+        best = min(best, time.perf_counter() - t0)
+    return best, res
+
+
+def autotune(classes: Iterable[ShapeClass] = DEFAULT_CLASSES, outputs: int = 50, repeats: int = 2,
+             seed: int = 7, apply: bool = True, log=None) -> List[Entry]:
+    """Time every candidate of every class; returns the table (and loads it
+    when `apply`).  Raises if two candidates disagree on a result."""
+    from . import Dataset, Distribution, GpuSearch, synth_spec
+    saved = get_table()
+    clear()   # candidates are timed as requested, not through an older table
+    entries: List[Entry] = []
+    try:
+        for sc in classes:
+            cases = sc.samples // 2
+            spec = synth_spec(sc.snps, cases, sc.samples - cases, seed=seed)
+            d = Dataset.synth(spec)
+            dist = Distribution(sc.snps, sc.order - 1)
+            best: Optional[Tuple[float, int, int]] = None
+            ref = None
+            for algo, var in candidates(sc.order):
+                s = GpuSearch(algorithm=algo, variant=var)
+                try:
+                    s.run_arrays(d, sc.order, dist, outputs)   # tables, code, caches
+                except FiuError:
+                    continue
+                t, (c, m) = _time(s, d, sc.order, dist, outputs, repeats)
+                if ref is None:
+                    ref = (c, m)
+                elif c.tolist() != ref[0].tolist() or not np.array_equal(m, ref[1]):
+                    raise RuntimeError(f"candidate {algo}/{var} disagrees on class {sc}")
+                if log:
+                    log(f"[tune] order {sc.order} {sc.snps} x {sc.samples}: algorithm {algo} variant {var} "
+                        f"{t * 1e3:.2f} ms")
+                if best is None or t < best[0]:
+                    best = (t, algo, var)
+            d.close()
+            if best is not None:
+                entries.append(Entry(sc.order, sc.snps_lo, sc.snps_hi, sc.samples_lo, sc.samples_hi,
+                                     best[1], best[2], best[0] * 1e3))
+    except BaseException:
+        set_table(saved)
+        raise
+    set_table(entries if apply else saved)
+    return entries

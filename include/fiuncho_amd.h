@@ -101,9 +101,11 @@ typedef enum {
     FIU_ALGO_MARGINAL = 2, /* 2^k cells + inclusion-exclusion from exact
                               lower-order counts; needs every SNP to cover the
                               same samples (checked at upload) */
-    FIU_ALGO_MFMA = 3      /* order 2 only: the marginal pair counts as int8
-                              indicator dot products on the matrix cores
-                              (int32 accumulation, exact); same requirement */
+    FIU_ALGO_MFMA = 3      /* orders 2 and 3: the 2^k marginal cells as
+                              indicator products on the matrix cores (fp4
+                              with fp32 accumulation, or int8 with int32;
+                              exact); same requirement, and for order 3 the
+                              pair table */
 } fiu_algorithm;
 
 /* fiu_config.flags */
@@ -122,6 +124,9 @@ typedef struct {
     uint32_t candidate_cap;  /* device candidate buffer entries, 0 = auto */
     uint32_t flags;          /* FIU_FLAG_* */
     void *stream;            /* hipStream_t to run on, NULL = library stream */
+    int32_t variant;         /* kernel variant of the algorithm, 0 = the
+                                tuned or default one (fiu_tune_entry) */
+    int32_t reserved;
 } fiu_config;
 
 /* Fill a fiu_config with defaults. */
@@ -134,7 +139,7 @@ typedef struct {
     double scan_ms;          /* device time of the scan launches (HIP events) */
     double total_ms;         /* wall time of the call */
     int32_t algorithm;       /* algorithm actually used */
-    int32_t reserved;
+    int32_t variant;         /* its kernel variant (fiu_tune_entry) */
 } fiu_search_stats;
 
 /* Exhaustive order-k search over the combinations whose (k-1)-prefix belongs
@@ -181,6 +186,36 @@ fiu_status fiu_contingency_batch(const fiu_dataset *ds, const int32_t *combos,
  * `ctrls` controls (MutualInformation.h:58-65, 83-84). */
 fiu_status fiu_mi_batch(const uint32_t *tables, size_t n, uint16_t order,
                         uint32_t cases, uint32_t ctrls, float *out_mi);
+
+/* ------------------------------------------------------------------------ */
+/* Tuning table                                                              */
+/* ------------------------------------------------------------------------ */
+/* No counterpart in the reference (one CPU kernel per build).  AUTO searches
+ * and batched scans whose (order, SNPs, cases + ctrls) fall in a class of the
+ * table use that class's algorithm and kernel variant, when the dataset
+ * allows them (else the default choice).  Variants:
+ *   FIU_ALGO_MARGINAL: prefixes per wavefront, 2, 4 (default) or, order 2
+ *                      only, 8;
+ *   FIU_ALGO_MFMA, order 3: LDS staging depth, 3 or 4 (default) stages;
+ *   FIU_ALGO_MFMA, order 2: 0 = fp4 operands (default), 1 = int8 operands.
+ * The table is written by an autotuner that times the candidates on the
+ * device (fiuncho_amd.tune); it is process-wide and starts empty.  Batched
+ * scans use its algorithm; their kernels have one variant each. */
+typedef struct {
+    uint16_t order;
+    uint16_t reserved;
+    uint32_t snps_lo, snps_hi;        /* SNPs in [lo, hi) */
+    uint32_t samples_lo, samples_hi;  /* cases + ctrls in [lo, hi) */
+    int32_t algorithm;                /* fiu_algorithm (not AUTO) */
+    int32_t variant;                  /* 0 = the algorithm's default */
+    float ms;                         /* time measured by the tuner (information) */
+} fiu_tune_entry;
+
+/* Replace the table (n = 0 clears it); FIU_ERR_INVALID for an unknown
+ * algorithm / variant or an empty range, and the table is left unchanged. */
+fiu_status fiu_set_tuning(const fiu_tune_entry *entries, size_t n);
+/* Copy up to cap entries; *n = entries in the table. */
+fiu_status fiu_get_tuning(fiu_tune_entry *entries, size_t cap, size_t *n);
 
 /* ------------------------------------------------------------------------ */
 /* Batched scans and the scan server                                         */

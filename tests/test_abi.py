@@ -99,3 +99,38 @@ def test_product_loader_errors(tmp_path):
     tped.write_text("1 s 0 1 A A C C\n")        # 2 samples' alleles for 1 sample
     with pytest.raises(fa.FiuError):
         fa.read_files([str(tped), str(tfam)])
+
+
+def test_tuning_table_roundtrip_and_validation():
+    from fiuncho_amd import tune
+    E = tune.Entry
+    good = [E(2, 0, 1000, 0, 4096, fa.ALGO_MARGINAL, 8, 1.5), E(3, 100, 200, 10, 20, fa.ALGO_MFMA, 3),
+            E(4, 0, 50, 0, 100, fa.ALGO_DIRECT, 0)]
+    try:
+        tune.set_table(good)
+        got = tune.get_table()
+        assert [(e.order, e.snps_lo, e.snps_hi, e.samples_lo, e.samples_hi, e.algorithm, e.variant) for e in got] == \
+               [(e.order, e.snps_lo, e.snps_hi, e.samples_lo, e.samples_hi, e.algorithm, e.variant) for e in good]
+        assert abs(got[0].ms - 1.5) < 1e-6
+        bad = [E(3, 0, 10, 0, 10, fa.ALGO_MARGINAL, 8),     # 8 prefixes per wave: order 2 only
+               E(2, 0, 10, 0, 10, fa.ALGO_MFMA, 3),         # ring depth: order 3 only
+               E(4, 0, 10, 0, 10, fa.ALGO_MFMA, 0),         # matrix cores: orders 2-3
+               E(2, 10, 10, 0, 10, fa.ALGO_DIRECT, 0),      # empty SNP range
+               E(2, 0, 10, 0, 10, fa.ALGO_AUTO, 0),         # AUTO is not a choice
+               E(1, 0, 10, 0, 10, fa.ALGO_DIRECT, 0)]
+        for b in bad:
+            try:
+                tune.set_table([b])
+            except fa.FiuError as e:
+                assert e.status == _lib.FIU_ERR_INVALID
+            else:
+                raise AssertionError(f"accepted {b}")
+        assert len(tune.get_table()) == len(good)   # a refused table leaves the old one
+    finally:
+        tune.clear()
+    assert tune.get_table() == []
+    # the autotuner's candidates are all valid table entries
+    for k in (2, 3):
+        for a, v in tune.candidates(k):
+            tune.set_table([E(k, 0, 10, 0, 10, a, v)])
+    tune.clear()
