@@ -21,6 +21,9 @@ Also reported on the same JSON line:
                 fp4 matrix-core scans (identical results required);
   batched       64 uneven datasets, one search per call vs one batched call;
   serving       a stream of scan requests, plain vs the scan server;
+  config1       the plain order-2 search of 1,000 SNPs x 4,000 samples;
+  tuning        the autotuner's table (timed before the batched and served
+                runs, which use it);
   cpu_baseline  the reference ThreadedSearch compiled from its own sources
                 (oracle/_ref, all host cores) or, when that library is absent,
                 the oracle's AVX-512 restatement, on a prefix sample of the
@@ -179,6 +182,43 @@ def bench_batch(count=64):
             "combo_samples": work, "plain_s": t_plain, "batched_s": t_batch,
             "speedup": t_plain / t_batch, "batched_combo_samples_per_s": work / t_batch,
             "steps": st.steps, "device_s": st.device_ms / 1e3, "results_identical": same}
+
+
+def bench_tune():
+    """The autotuner over its default shape classes (orders 2 and 3); the
+    table it loads is used by the searches that follow (batched, served)."""
+    from fiuncho_amd import tune
+    t0 = time.perf_counter()
+    entries = tune.autotune()
+    dt = time.perf_counter() - t0
+    print(f"[bench] tune: {len(entries)} classes in {dt:.1f} s", file=sys.stderr, flush=True)
+    names = {1: "direct", 2: "marginal", 3: "mfma"}
+    return {"seconds": dt, "classes": [
+        {"order": e.order, "snps": [e.snps_lo, e.snps_hi], "samples": [e.samples_lo, e.samples_hi],
+         "algorithm": names.get(e.algorithm, e.algorithm), "variant": e.variant, "ms": e.ms} for e in entries]}
+
+
+def bench_config1(repeats=20):
+    """The plain order-2 parity configuration: 1,000 SNPs x 4,000 samples
+    (2,000 / 2,000), top-10, one search per call."""
+    import fiuncho_amd as fa
+    spec = fa.synth_spec(1000, 2000, 2000, seed=1)
+    d = fa.Dataset.synth(spec)
+    s = fa.GpuSearch()
+    dist = fa.Distribution(1000, 1)
+    s.run_arrays(d, 2, dist, 10)
+    ts = []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        s.run_arrays(d, 2, dist, 10)
+        ts.append(time.perf_counter() - t0)
+    d.close()
+    ts.sort()
+    med = ts[len(ts) // 2]
+    print(f"[bench] config 1: {med * 1e3:.2f} ms per search (median of {repeats})", file=sys.stderr, flush=True)
+    return {"workload": "order-2 exhaustive scan, 1,000 SNPs x 4,000 samples (2,000/2,000), top-10",
+            "median_ms": med * 1e3, "min_ms": ts[0] * 1e3,
+            "combo_samples_per_s": 499500 * 4000 / med, "algorithm": s.last_stats.algorithm}
 
 
 MFMA_I8_PEAK = 5.0e15    # dense int8 MFMA ops per second (MI355X)
@@ -351,6 +391,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the uneven-batch comparison")
     ap.add_argument("--no-serve", action="store_true", help="skip the served-stream comparison")
     ap.add_argument("--no-order2", action="store_true", help="skip the order-2 popcount / matrix-core comparison")
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (batch / serve use the defaults)")
     ap.add_argument("--batch-datasets", type=int, default=64)
     ap.add_argument("--serve-requests", type=int, default=300)
     ap.add_argument("--only-extra", action="store_true",
@@ -384,7 +425,10 @@ def main():
     if args.only_extra:
         extra = {}
         if not args.no_order2:
+            extra["config1"] = bench_config1()
             extra["order2"] = bench_order2()
+        if not args.no_tune and not (args.no_batch and args.no_serve):
+            extra["tuning"] = bench_tune()
         if not args.no_batch:
             extra["batched"] = bench_batch(args.batch_datasets)
         if not args.no_serve:
@@ -468,7 +512,10 @@ def main():
     }
     ds.close()
     if not args.no_order2:
+        line["config1"] = bench_config1()
         line["order2"] = bench_order2()
+    if not args.no_tune and not (args.no_batch and args.no_serve):
+        line["tuning"] = bench_tune()
     if not args.no_batch:
         line["batched"] = bench_batch(args.batch_datasets)
     if not args.no_serve:
