@@ -853,6 +853,49 @@ template <bool FP4> struct MfmaFmt {
     static constexpr int substeps = FP4 ? 2 : 4;   // MFMAs per 128-sample stage and accumulator
 };
 
+// Direct-to-LDS buffer load of 16 bytes per lane from byte offset voff of
+// the resource: lane l's bytes land at lds_group + 16 l (lds_group
+// wave-uniform).  The per-lane offsets stay fixed while the resource base
+// moves (scalar arithmetic only).
+__device__ __forceinline__ void dma16(__amdgpu_buffer_rsrc_t r, uint32_t voff, int8_t *lds_group)
+{
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (__attribute__((address_space(3))) void *)lds_group, 16, voff, 0, 0,
+                                            0);
+}
+
+// Wait until at most N vector-memory operations of this wave are pending
+// (vmcnt N; lgkm and exp counters not waited on).
+template <int N> __device__ __forceinline__ void wait_vmcnt()
+{
+    static_assert(N >= 0 && N < 16, "vmcnt immediate");
+    __builtin_amdgcn_s_waitcnt(0x0f70 | N);
+}
+
+// wait_vmcnt<L * s>() for a runtime, wave-uniform s (clamped to 0 .. S).
+template <int S, int L> __device__ __forceinline__ void wait_stages(int s)
+{
+    if constexpr (S > 0) {
+        if (s >= S) {
+            wait_vmcnt<L * S>();
+            return;
+        }
+        wait_stages<S - 1, L>(s);
+    } else {
+        wait_vmcnt<0>();
+    }
+}
+
+// Workgroup barrier with no memory fence (a fence would drain every pending
+// direct-to-LDS load, including the ones meant to stay in flight); the
+// waits are explicit, the empty asm statements keep the compiler from moving
+// memory accesses across the barrier.
+__device__ __forceinline__ void stage_barrier()
+{
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+}
+
 // Eight bits (b < 256) as eight e2m1 nibbles, bit i -> nibble i = 0x2 (1.0)
 // or 0x0: the bits are spread 4 apart by three shift-or-mask steps, then
 // moved to bit 1 of each nibble.
@@ -934,8 +977,22 @@ __device__ __forceinline__ void tri_tile(int64_t u, int T, int &ta, int &tb)
 // genotypes 0/1, B likewise).  fp4: the rows are copied from the dataset's
 // fp4 image (four 16-byte units per thread); int8: each thread loads one
 // 128-bit quad of the bit tables and expands it to bytes in LDS.
-// LDS of one k_mfma2 workgroup: [buffer][A/B][row][k].
-template <bool FP4> using Mfma2Lds = int8_t[2][2][2 * kMfmaTile][MfmaFmt<FP4>::row];
+// LDS of one k_mfma2 workgroup, [buffer][A/B][row][k]: fp4, a ring of
+// kM2Ring stages of 64-byte rows filled by direct-to-LDS loads; int8, two
+// buffers of expanded rows.
+constexpr int kM2Ring = 4;
+template <bool FP4> struct Mfma2LdsT {
+    typedef int8_t type[2][2][2 * kMfmaTile][MfmaFmt<false>::row];
+};
+template <> struct Mfma2LdsT<true> {
+    typedef int8_t type[kM2Ring][2][2 * kMfmaTile][64];
+};
+template <bool FP4> using Mfma2Lds = typename Mfma2LdsT<FP4>::type;
+// Segment s (16 bytes) of LDS row r (fp4 ring) is stored at s ^ m2_swz(r):
+// the 32-row fragment reads (lane l: row l % 32, segment 2 k + l / 32) are
+// then free of bank conflicts (measured with a ds_read_b128 probe: 4.7
+// cycles per wave per CU, as a linear read).
+__device__ __forceinline__ int m2_swz(int r) { return (r >> 2) & 3; }
 
 // One workgroup's work item: tile `slot` (of 64) of super-block su; returns at
 // once for tiles outside the triangle.  Every wave of the workgroup calls it
@@ -955,14 +1012,6 @@ __device__ __forceinline__ void mfma2_tile(const DevData &D, const ScanParams &P
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6), wa = w >> 1, wb = w & 1;
     const int np = D.n_pad;
     const int a0 = ta * kMfmaTile, b0 = tb * kMfmaTile;
-    // int8 staging: this thread's row, operand m (A / B), genotype g, SNP
-    const int m_ld = tid >> 7, row_ld = tid & 127, g_ld = row_ld >> 6;
-    const int snp_ld = (m_ld ? b0 : a0) + (row_ld & 63);
-    // fp4 staging: segment seg of row q4 of each of the four 64-row panels;
-    // offsets in uint4 within a quad of the image (< 8 n_pad)
-    const int seg = tid & 3, q4 = tid >> 2;
-    const int ua0 = 4 * (a0 + q4) + seg, ua1 = 4 * (np + a0 + q4) + seg;
-    const int ub0 = 4 * (b0 + q4) + seg, ub1 = 4 * (np + b0 + q4) + seg;
     Acc acc[2][2][2];
 #pragma unroll
     for (int c = 0; c < 2; ++c)
@@ -970,81 +1019,110 @@ __device__ __forceinline__ void mfma2_tile(const DevData &D, const ScanParams &P
         for (int g = 0; g < 2; ++g)
 #pragma unroll
             for (int h = 0; h < 2; ++h) acc[c][g][h] = (Acc){0};
+    if constexpr (FP4) {
+        // staging: each wave issues 4 direct-to-LDS loads per stage, rows
+        // 16 w .. 16 w + 15 of the A genotype-0, A genotype-1, B genotype-0
+        // and B genotype-1 panels; lane l fills row 16 w + l / 4, stored
+        // segment l % 4, so it loads the row's segment (l % 4) ^ m2_swz(row).
+        // Byte offsets within a quad of the image (< 128 n_pad).
+        const int lr = lane >> 2, sg = (lane & 3) ^ m2_swz(lr);
+        const uint32_t oa0 = 16u * (uint32_t)(4 * (a0 + 16 * w + lr) + sg);
+        const uint32_t oa1 = 16u * (uint32_t)(4 * (np + a0 + 16 * w + lr) + sg);
+        const uint32_t ob0 = 16u * (uint32_t)(4 * (b0 + 16 * w + lr) + sg);
+        const uint32_t ob1 = 16u * (uint32_t)(4 * (np + b0 + 16 * w + lr) + sg);
+        const int r = lane & 31;
 #pragma unroll
-    for (int cls = 0; cls < 2; ++cls) {
-        const int iters = D.quads[cls];
-        const uint4 *src = D.bits + D.class_off[cls] + (uint64_t)g_ld * np + snp_ld;   // quad q at + q * 3 * np
-        const uint4 *img = F4.base + F4.class_off[cls];                                // quad q at + q * 8 * np
-        // int8: quads it+1 and it+2 wait in registers, quad it+3 is loaded
-        // during stage it; fp4: the units of stage it+1 are loaded during
-        // stage it
-        uint4 q1 = make_uint4(0, 0, 0, 0), q2 = q1, p0 = q1, p1 = q1, p2 = q1, p3 = q1;
-        auto gload = [&](int q) {
-            const uint4 *s = img + (int64_t)q * 8 * np;
-            p0 = s[ua0];
-            p1 = s[ua1];
-            p2 = s[ub0];
-            p3 = s[ub1];
-        };
-        auto stage = [&](int buf) {
-            if constexpr (FP4) {
-                *(uint4 *)&lds[buf][0][q4][seg * 16] = p0;
-                *(uint4 *)&lds[buf][0][64 + q4][seg * 16] = p1;
-                *(uint4 *)&lds[buf][1][q4][seg * 16] = p2;
-                *(uint4 *)&lds[buf][1][64 + q4][seg * 16] = p3;
-            } else {
-                expand_quad(q1, lds[buf][m_ld][row_ld]);
-            }
-        };
-        if constexpr (FP4) {
-            gload(0);
-        } else {
-            q1 = src[0];
-        }
-        stage(0);
-        if constexpr (!FP4) {
-            if (iters > 1) q1 = src[(uint64_t)3 * np];
-            if (iters > 2) q2 = src[(uint64_t)6 * np];
-        }
-        __syncthreads();
-        for (int it = 0; it < iters; ++it) {
-            const int buf = it & 1;
-            uint4 q3 = make_uint4(0, 0, 0, 0);
-            if constexpr (FP4) {
-                if (it + 1 < iters) gload(it + 1);
-            } else {
-                if (it + 3 < iters) q3 = src[(uint64_t)(it + 3) * 3 * np];
-            }
+        for (int cls = 0; cls < 2; ++cls) {
+            const int iters = D.quads[cls];
+            const uint4 *img = F4.base + F4.class_off[cls];   // quad q at + q * 8 * np
+            auto issue = [&](int q) {
+                const __amdgpu_buffer_rsrc_t rq =
+                    __builtin_amdgcn_make_buffer_rsrc((void *)(img + (int64_t)q * 8 * np), (short)0, 128 * np, 0x00020000);
+                auto &sl = lds[q % kM2Ring];
+                dma16(rq, oa0, &sl[0][16 * w][0]);
+                dma16(rq, oa1, &sl[0][kMfmaTile + 16 * w][0]);
+                dma16(rq, ob0, &sl[1][16 * w][0]);
+                dma16(rq, ob1, &sl[1][kMfmaTile + 16 * w][0]);
+            };
 #pragma unroll
-            for (int s = 0; s < F::substeps; ++s) {
-                // 16 operand bytes per lane: 16 int8 or 32 fp4 samples of row r
-                const int kk = s * 32 + (lane >> 5) * 16, r = lane & 31;
-                v4i fa[2], fb[2];
+            for (int q = 0; q < kM2Ring - 1; ++q)
+                if (q < iters) issue(q);
+            for (int it = 0; it < iters; ++it) {
+                // this wave's loads of stage it are complete once at most those
+                // of the later stages issued (4 each) are pending; the barrier
+                // publishes every wave's part
+                wait_stages<kM2Ring - 2, 4>(iters - 1 - it);
+                stage_barrier();
+                const auto &sl = lds[it % kM2Ring];
+                v4i fa[2][2], fb[2][2];   // [substep][genotype]: 32 fp4 samples of row r per lane
 #pragma unroll
-                for (int g = 0; g < 2; ++g) {
-                    fa[g] = *(const v4i *)&lds[buf][0][g * kMfmaTile + wa * 32 + r][kk];
-                    fb[g] = *(const v4i *)&lds[buf][1][g * kMfmaTile + wb * 32 + r][kk];
+                for (int s = 0; s < 2; ++s) {
+                    const int ko = ((2 * s + (lane >> 5)) ^ m2_swz(r)) << 4;
+#pragma unroll
+                    for (int g = 0; g < 2; ++g) {
+                        fa[s][g] = *(const v4i *)&sl[0][g * kMfmaTile + wa * 32 + r][ko];
+                        fb[s][g] = *(const v4i *)&sl[1][g * kMfmaTile + wb * 32 + r][ko];
+                    }
                 }
+                // every wave is past stage it - 1, whose slot takes stage it + kM2Ring - 1
+                if (it + kM2Ring - 1 < iters) issue(it + kM2Ring - 1);
 #pragma unroll
-                for (int g = 0; g < 2; ++g)
+                for (int s = 0; s < 2; ++s)
 #pragma unroll
-                    for (int h = 0; h < 2; ++h) {
-                        if constexpr (FP4) {
-                            const v8i a8 = {fa[g][0], fa[g][1], fa[g][2], fa[g][3], 0, 0, 0, 0};
-                            const v8i b8 = {fb[h][0], fb[h][1], fb[h][2], fb[h][3], 0, 0, 0, 0};
+                    for (int g = 0; g < 2; ++g)
+#pragma unroll
+                        for (int h = 0; h < 2; ++h) {
+                            const v8i a8 = {fa[s][g][0], fa[s][g][1], fa[s][g][2], fa[s][g][3], 0, 0, 0, 0};
+                            const v8i b8 = {fb[s][h][0], fb[s][h][1], fb[s][h][2], fb[s][h][3], 0, 0, 0, 0};
                             // formats 4/4 = e2m1; scale operands 0 select the unscaled product
                             acc[cls][g][h] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a8, b8, acc[cls][g][h],
                                                                                            4, 4, 0, 0, 0, 0);
-                        } else {
-                            acc[cls][g][h] =
-                                __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[g], fb[h], acc[cls][g][h], 0, 0, 0);
                         }
-                    }
             }
-            if (it + 1 < iters) stage(buf ^ 1);
+            // the next class's first loads reuse the ring
+            stage_barrier();
+        }
+    } else {
+        // int8 staging: this thread's row, operand m (A / B), genotype g, SNP;
+        // each stage loads one 128-bit quad per row and expands it to bytes
+        const int m_ld = tid >> 7, row_ld = tid & 127, g_ld = row_ld >> 6;
+        const int snp_ld = (m_ld ? b0 : a0) + (row_ld & 63);
+#pragma unroll
+        for (int cls = 0; cls < 2; ++cls) {
+            const int iters = D.quads[cls];
+            const uint4 *src = D.bits + D.class_off[cls] + (uint64_t)g_ld * np + snp_ld;   // quad q at + q * 3 * np
+            // quads it+1 and it+2 wait in registers, quad it+3 is loaded
+            // during stage it: two stages of latency hiding for every load
+            uint4 q1 = src[0], q2 = make_uint4(0, 0, 0, 0);
+            expand_quad(q1, lds[0][m_ld][row_ld]);
+            if (iters > 1) q1 = src[(uint64_t)3 * np];
+            if (iters > 2) q2 = src[(uint64_t)6 * np];
             __syncthreads();
-            q1 = q2;
-            q2 = q3;
+            for (int it = 0; it < iters; ++it) {
+                const int buf = it & 1;
+                uint4 q3 = make_uint4(0, 0, 0, 0);
+                if (it + 3 < iters) q3 = src[(uint64_t)(it + 3) * 3 * np];
+#pragma unroll
+                for (int s = 0; s < F::substeps; ++s) {
+                    // 16 operand bytes per lane: 16 int8 samples of row r
+                    const int kk = s * 32 + (lane >> 5) * 16, r = lane & 31;
+                    v4i fa[2], fb[2];
+#pragma unroll
+                    for (int g = 0; g < 2; ++g) {
+                        fa[g] = *(const v4i *)&lds[buf][0][g * kMfmaTile + wa * 32 + r][kk];
+                        fb[g] = *(const v4i *)&lds[buf][1][g * kMfmaTile + wb * 32 + r][kk];
+                    }
+#pragma unroll
+                    for (int g = 0; g < 2; ++g)
+#pragma unroll
+                        for (int h = 0; h < 2; ++h)
+                            acc[cls][g][h] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[g], fb[h], acc[cls][g][h], 0, 0, 0);
+                }
+                if (it + 1 < iters) expand_quad(q1, lds[buf ^ 1][m_ld][row_ld]);
+                __syncthreads();
+                q1 = q2;
+                q2 = q3;
+            }
         }
     }
     // epilogue: lane holds column b = b0 + wb*32 + lane%32 and rows
@@ -1136,49 +1214,6 @@ constexpr int kM3Group = 16 * 64;
 constexpr int kM3Stage = 13 * kM3Group;
 constexpr int kM3Ring = 4;   // default stages resident: the one read + up to 3 loading
 __device__ __forceinline__ int m3_swz(int r) { return (r >> 1) & 3; }
-
-// Direct-to-LDS buffer load of 16 bytes per lane from byte offset voff of
-// the resource: lane l's bytes land at lds_group + 16 l (lds_group
-// wave-uniform).  The per-lane offsets stay fixed while the resource base
-// moves (scalar arithmetic only).
-__device__ __forceinline__ void dma16(__amdgpu_buffer_rsrc_t r, uint32_t voff, int8_t *lds_group)
-{
-    __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (__attribute__((address_space(3))) void *)lds_group, 16, voff, 0, 0,
-                                            0);
-}
-
-// Wait until at most N vector-memory operations of this wave are pending
-// (vmcnt N; lgkm and exp counters not waited on).
-template <int N> __device__ __forceinline__ void wait_vmcnt()
-{
-    static_assert(N >= 0 && N < 16, "vmcnt immediate");
-    __builtin_amdgcn_s_waitcnt(0x0f70 | N);
-}
-
-// wait_vmcnt<L * s>() for a runtime, wave-uniform s (clamped to 0 .. S).
-template <int S, int L> __device__ __forceinline__ void wait_stages(int s)
-{
-    if constexpr (S > 0) {
-        if (s >= S) {
-            wait_vmcnt<L * S>();
-            return;
-        }
-        wait_stages<S - 1, L>(s);
-    } else {
-        wait_vmcnt<0>();
-    }
-}
-
-// Workgroup barrier with no memory fence (a fence would drain every pending
-// direct-to-LDS load, including the ones meant to stay in flight); the
-// waits are explicit, the empty asm statements keep the compiler from moving
-// memory accesses across the barrier.
-__device__ __forceinline__ void stage_barrier()
-{
-    asm volatile("" ::: "memory");
-    __builtin_amdgcn_s_barrier();
-    asm volatile("" ::: "memory");
-}
 
 // One workgroup's work item: first SNP a, tile u (see k_mfma3); returns at
 // once for items outside the triangle.  Every wave of the workgroup calls it
@@ -1542,9 +1577,10 @@ k_batch_scan(const BatchJob *__restrict__ jobs, const BatchSeg *__restrict__ seg
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 2)))
 k_batch_mfma(const BatchJob *__restrict__ jobs, const BatchSeg *__restrict__ segs, BatchHeader *hdr)
 {
-    __shared__ __attribute__((aligned(1024))) int8_t lds[kM3Ring * kM3Stage];
+    constexpr size_t kBytes = sizeof(Mfma2Lds<true>) > (size_t)kM3Ring * kM3Stage ? sizeof(Mfma2Lds<true>)
+                                                                                  : (size_t)kM3Ring * kM3Stage;
+    __shared__ __attribute__((aligned(1024))) int8_t lds[kBytes];   // the order-2 or the order-3 tiles
     __shared__ uint32_t unit_sh;
-    static_assert(sizeof(Mfma2Lds<true>) <= sizeof(lds), "LDS of the order-2 tiles");
     const int x = blockIdx.x & 7;
     const uint32_t nseg = hdr->mseg, units = hdr->munits;
     for (;;) {

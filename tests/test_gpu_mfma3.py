@@ -25,6 +25,10 @@ pytestmark = pytest.mark.gpu
     (130, 300, 420, 200, 61),    # three tiles, partial last tile
     (23, 64, 64, 1771, 62),      # every triple
     (90, 1000, 900, 100, 63),    # 8 + 8 quads per class
+    (3, 50, 60, 5, 64),          # the single triple
+    (5, 64, 64, 10, 65),         # every triple of a tiny set
+    (64, 200, 200, 300, 66),     # exactly one full tile
+    (65, 40, 2000, 500, 67),     # one SNP in the second tile; 1 quad of cases, 16 of controls
 ])
 def test_mfma3_vs_oracle(snps, cases, ctrls, outputs, seed):
     print(f"case: mfma3 {snps}x{cases}+{ctrls}", flush=True)
