@@ -75,18 +75,18 @@ def slice_combos(n, order, step, offset):
     return total
 
 
-def cpu_baseline(words, geno, cw, tw, threads, sample_step):
-    """Reference ThreadedSearch (or the oracle port) on prefix sample 0."""
+def cpu_baseline(words, raw_path, cw, tw, threads, sample_step):
+    """Reference ThreadedSearch (or the oracle port) on prefix sample 0; with
+    the reference build, also the reference loader's time on the .raw."""
     import oracle
     combos = slice_combos(N_SNPS, ORDER, sample_step, 0)
     ref = oracle.ref("avx512")
+    read_s = None
     if ref is not None and oracle.has_avx512():
-        with tempfile.TemporaryDirectory(dir="/dev/shm" if os.path.isdir("/dev/shm") else None) as tmp:
-            path = os.path.join(tmp, "bench.raw")
-            write_raw(path, geno, N_CASES)
-            d = ref.load(path)
-            del geno
-            _, _, sec = d.search(ORDER, OUTPUTS, step=sample_step, offset=0, threads=threads)
+        t0 = time.perf_counter()
+        d = ref.load(raw_path)
+        read_s = time.perf_counter() - t0
+        _, _, sec = d.search(ORDER, OUTPUTS, step=sample_step, offset=0, threads=threads)
         kind = "reference"
     else:
         d = oracle.Dataset.__new__(oracle.Dataset)
@@ -97,7 +97,7 @@ def cpu_baseline(words, geno, cw, tw, threads, sample_step):
         sec = time.perf_counter() - t0
         kind = "port"
     return {"value": combos * (N_CASES + N_CTRLS) / sec, "unit": "combo*samples/s", "cores": threads,
-            "kind": kind,
+            "kind": kind, "reference_read_s": read_s,
             "sample": f"Distribution({N_SNPS}, {ORDER - 1}, {sample_step}, 0) of the same dataset "
                       f"(read from a PLINK .raw by the reference loader): "
                       f"{combos} of {math.comb(N_SNPS, ORDER)} triples, {sec:.1f} s, "
@@ -359,6 +359,40 @@ def bench_serve(count=300, load=0.7):
             "results_identical": same}
 
 
+def bench_ingest(path, geno, words):
+    """Start-up: the headline dataset's PLINK .raw (10,000 SNPs x 16,000
+    samples) read by the library -- host parse, device pack -- against host
+    parse + host pack, and the device pack alone from in-memory genotypes."""
+    import fiuncho_amd as fa
+    out = {"file": f"PLINK .raw, {N_SNPS} SNPs x {N_CASES + N_CTRLS} samples", "file_bytes": os.path.getsize(path)}
+    t0 = time.perf_counter()
+    d = fa.Dataset.read([path])
+    out["read_device_pack_s"] = time.perf_counter() - t0
+    dev_words = d.words()
+    d.close()
+    t0 = time.perf_counter()
+    host_words = fa.read_files([path])[0]
+    out["read_host_pack_s"] = time.perf_counter() - t0
+    # the device-packed tables equal the host-packed ones; both differ from
+    # the generator's words only in the reference readers' re-flushed word
+    # (the 8,000th case is followed by a control: case word 124 is zero)
+    out["identical_words"] = bool(np.array_equal(dev_words, host_words))
+    out["differs_from_generated_words"] = int(np.count_nonzero(dev_words != words))
+    del dev_words, host_words
+    is_ctrl = np.r_[np.zeros(N_CASES, np.uint8), np.ones(N_CTRLS, np.uint8)]
+    ts = []
+    for _ in range(4):
+        t0 = time.perf_counter()
+        fa.Dataset.from_genotypes(geno, is_ctrl).close()
+        ts.append(time.perf_counter() - t0)
+    out["from_genotypes_ms"] = min(ts[1:]) * 1e3
+    out["genotype_bytes"] = int(geno.nbytes)
+    print(f"[bench] ingest: read {out['read_device_pack_s']:.2f} s (device pack) vs "
+          f"{out['read_host_pack_s']:.2f} s (host pack); from genotypes {out['from_genotypes_ms']:.1f} ms",
+          file=sys.stderr, flush=True)
+    return out
+
+
 def write_raw(path, geno, cases):
     """PLINK .raw (FID IID PAT MAT SEX PHENOTYPE + one column per SNP)."""
     snps, n = geno.shape
@@ -392,6 +426,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the served-stream comparison")
     ap.add_argument("--no-order2", action="store_true", help="skip the order-2 popcount / matrix-core comparison")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (batch / serve use the defaults)")
+    ap.add_argument("--no-ingest", action="store_true", help="skip the .raw read / device packing timing")
     ap.add_argument("--batch-datasets", type=int, default=64)
     ap.add_argument("--serve-requests", type=int, default=300)
     ap.add_argument("--only-extra", action="store_true",
@@ -433,13 +468,20 @@ def main():
             extra["batched"] = bench_batch(args.batch_datasets)
         if not args.no_serve:
             extra["serving"] = bench_serve(args.serve_requests)
+        if not args.no_ingest:
+            spec = fa.synth_spec(N_SNPS, N_CASES, N_CTRLS, seed=1, planted=PLANTED, penetrance_seed=2)
+            words, geno, _, _ = fa.synth_generate(spec, words=True, geno=True)
+            with tempfile.TemporaryDirectory(dir="/dev/shm" if os.path.isdir("/dev/shm") else None) as tmp:
+                path = os.path.join(tmp, "bench.raw")
+                write_raw(path, geno, N_CASES)
+                extra["ingest"] = bench_ingest(path, geno, words)
         print(json.dumps(extra), flush=True)
         return
     algo = {"auto": fa.ALGO_AUTO, "direct": fa.ALGO_DIRECT, "marginal": fa.ALGO_MARGINAL,
             "mfma": fa.ALGO_MFMA}[args.algorithm]
     t0 = time.perf_counter()
     spec = fa.synth_spec(N_SNPS, N_CASES, N_CTRLS, seed=1, planted=PLANTED, penetrance_seed=2)
-    words, geno, cw, tw = fa.synth_generate(spec, words=True, geno=not args.no_cpu)
+    words, geno, cw, tw = fa.synth_generate(spec, words=True, geno=not (args.no_cpu and args.no_ingest))
     ds = fa.Dataset.from_words(words, N_SNPS, cw, tw, N_CASES, N_CTRLS)
     print(f"[bench] dataset generated and uploaded in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     search = fa.GpuSearch(algorithm=algo)
@@ -520,10 +562,17 @@ def main():
         line["batched"] = bench_batch(args.batch_datasets)
     if not args.no_serve:
         line["serving"] = bench_serve(args.serve_requests)
-    if not args.no_cpu:
-        threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-        print(f"[bench] CPU baseline on {threads} threads ...", file=sys.stderr, flush=True)
-        line["cpu_baseline"] = cpu_baseline(words, geno, cw, tw, threads, args.cpu_sample_step)
+    if not (args.no_cpu and args.no_ingest):
+        with tempfile.TemporaryDirectory(dir="/dev/shm" if os.path.isdir("/dev/shm") else None) as tmp:
+            path = os.path.join(tmp, "bench.raw")
+            write_raw(path, geno, N_CASES)
+            if not args.no_ingest:
+                line["ingest"] = bench_ingest(path, geno, words)
+            del geno
+            if not args.no_cpu:
+                threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+                print(f"[bench] CPU baseline on {threads} threads ...", file=sys.stderr, flush=True)
+                line["cpu_baseline"] = cpu_baseline(words, path, cw, tw, threads, args.cpu_sample_step)
     print(json.dumps(line), flush=True)
 
 

@@ -115,6 +115,21 @@ class Dataset:
         return cls(h.value)
 
     @classmethod
+    def from_genotypes(cls, geno, is_ctrl, reader_flush: bool = False) -> "Dataset":
+        """Pack a SNP-major genotype matrix (snps x samples; 0/1/2, other
+        values uncalled) on the device; `is_ctrl` per sample.  `reader_flush`
+        zeroes the words the reference file readers re-flush, as
+        ``Dataset.read`` does (RAWFile.hpp:226-259)."""
+        geno = np.ascontiguousarray(geno, dtype=np.int8)
+        is_ctrl = np.ascontiguousarray(is_ctrl, dtype=np.uint8)
+        if geno.ndim != 2 or is_ctrl.shape != (geno.shape[1],):
+            raise ValueError("geno must be snps x samples and is_ctrl one flag per sample")
+        h = C.c_void_p()
+        check(lib().fiu_dataset_from_genotypes(geno.ctypes.data, geno.shape[0], geno.shape[1], is_ctrl.ctypes.data,
+                                               _lib.FIU_GENO_READER_FLUSH if reader_flush else 0, C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
     def from_words(cls, words, snps, cases_words, ctrls_words, cases, ctrls) -> "Dataset":
         words = np.ascontiguousarray(words, dtype=np.uint64)
         if words.size != snps * 3 * (cases_words + ctrls_words):

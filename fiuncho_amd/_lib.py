@@ -27,7 +27,7 @@ ALGO_AUTO, ALGO_DIRECT, ALGO_MARGINAL, ALGO_MFMA = 0, 1, 2, 3
 # Every symbol include/fiuncho_amd.h declares (checked by the CPU tests).
 EXPORTS = (
     "fiu_last_error", "fiu_version", "fiu_set_device",
-    "fiu_dataset_create", "fiu_dataset_read", "fiu_read_files", "fiu_dataset_info", "fiu_dataset_words",
+    "fiu_dataset_create", "fiu_dataset_read", "fiu_dataset_from_genotypes", "fiu_read_files", "fiu_dataset_info", "fiu_dataset_words",
     "fiu_dataset_destroy", "fiu_config_init", "fiu_search", "fiu_ct_size",
     "fiu_combine_batch", "fiu_contingency_batch", "fiu_mi_batch",
     "fiu_batch_config_init", "fiu_search_batch", "fiu_server_create", "fiu_server_submit",
@@ -56,6 +56,7 @@ class TuneEntry(C.Structure):
 
 
 FLAG_NO_THRESHOLD_SAMPLE = 1
+FIU_GENO_READER_FLUSH = 1
 FLAG_MFMA_INT8 = 2
 
 
@@ -130,6 +131,7 @@ def lib():
     L.fiu_set_device.argtypes = [C.c_int]
     L.fiu_dataset_create.argtypes = [vp, sz, sz, sz, u32, u32, C.POINTER(vp)]
     L.fiu_dataset_read.argtypes = [C.POINTER(C.c_char_p), sz, C.POINTER(vp)]
+    L.fiu_dataset_from_genotypes.argtypes = [vp, sz, sz, vp, u32, C.POINTER(vp)]
     L.fiu_read_files.argtypes = [C.POINTER(C.c_char_p), sz, vp, sz, C.POINTER(sz), C.POINTER(u32),
                                  C.POINTER(u32), C.POINTER(sz), C.POINTER(sz)]
     L.fiu_dataset_info.argtypes = [vp, C.POINTER(sz), C.POINTER(u32), C.POINTER(u32),

@@ -85,7 +85,7 @@ Shape shape_of(size_t snps, size_t cw, size_t tw)
     return s;
 }
 
-void upload(fiu_dataset *d, const uint64_t *words, hipStream_t stream)
+void bind_stream(fiu_dataset *d, hipStream_t stream)
 {
     if (d->snps > (size_t)INT32_MAX - 64) fail(FIU_ERR_UNSUPPORTED, "more SNPs than INT_MAX");
     check(hipSetDevice(d->device), "hipSetDevice");
@@ -96,23 +96,20 @@ void upload(fiu_dataset *d, const uint64_t *words, hipStream_t stream)
         check(hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking), "hipStreamCreate");
         d->own_stream = true;
     }
+}
+
+// Device image from reference-layout words already on the device: the
+// transposes into the SNP-minor quads, then row counts and coverage flags.
+// Returns once the device no longer reads `words`.
+void finish_upload(fiu_dataset *d, const uint64_t *words)
+{
     const Shape sh = shape_of(d->snps, d->cw, d->tw);
     const int n = sh.n, n_pad = sh.n_pad;
     d->bits.alloc(sh.total);
-    const size_t nwords = d->snps * 3 * (d->cw + d->tw);
-    {
-        DBuf<uint64_t> staging;
-        staging.arena = d->arena;
-        staging.alloc(nwords);
-        if (nwords)
-            check(hipMemcpyAsync(staging.p, words, nwords * 8, hipMemcpyHostToDevice, d->stream), "upload");
-        check(launch_transpose(staging.p, d->bits.p, n, n_pad, 0, d->cw, d->tw, sh.q0, 0, d->stream), "transpose");
-        check(launch_transpose(staging.p, d->bits.p, n, n_pad, 1, d->cw, d->tw, sh.q1, sh.off1, d->stream),
-              "transpose");
-        // the staging buffer is released (and may be reused) once the
-        // transposes have read it
-        check(hipStreamSynchronize(d->stream), "transpose sync");
-    }
+    check(launch_transpose(words, d->bits.p, n, n_pad, 0, d->cw, d->tw, sh.q0, 0, d->stream), "transpose");
+    check(launch_transpose(words, d->bits.p, n, n_pad, 1, d->cw, d->tw, sh.q1, sh.off1, d->stream), "transpose");
+    // the caller releases (and may reuse) `words` once the transposes read it
+    check(hipStreamSynchronize(d->stream), "transpose sync");
     DevData &D = d->dev;
     D.bits = d->bits.p;
     D.class_off[0] = 0;
@@ -135,6 +132,17 @@ void upload(fiu_dataset *d, const uint64_t *words, hipStream_t stream)
     D.uniform = (n > 0 && fl[0] == 0) ? 1 : 0;
     D.total[0] = fl[1];
     D.total[1] = fl[2];
+}
+
+void upload(fiu_dataset *d, const uint64_t *words, hipStream_t stream)
+{
+    bind_stream(d, stream);
+    const size_t nwords = d->snps * 3 * (d->cw + d->tw);
+    DBuf<uint64_t> staging;
+    staging.arena = d->arena;
+    staging.alloc(nwords);
+    if (nwords) check(hipMemcpyAsync(staging.p, words, nwords * 8, hipMemcpyHostToDevice, d->stream), "upload");
+    finish_upload(d, staging.p);
 }
 
 }  // namespace
@@ -172,6 +180,79 @@ fiu_dataset *create_dataset(const uint64_t *words, size_t snps, size_t cw, size_
     } else {
         upload(d.get(), words, stream);
     }
+    return d.release();
+}
+
+fiu_dataset *dataset_from_genotypes(const int8_t *geno, size_t snps, size_t samples, const uint8_t *is_ctrl,
+                                    bool reader_flush, Arena *arena, hipStream_t stream)
+{
+    if (!samples) fail(FIU_ERR_INVALID, "no samples");
+    if (!is_ctrl) fail(FIU_ERR_INVALID, "is_ctrl is NULL");
+    if (!geno && snps) fail(FIU_ERR_INVALID, "geno is NULL");
+    if (samples > (size_t)INT32_MAX) fail(FIU_ERR_UNSUPPORTED, "more samples than INT_MAX");
+    // per-class sample lists (file order within a class), cases first; not
+    // needed when each class is one run of samples
+    std::vector<int32_t> idx(samples);
+    size_t ctrls = 0, changes = 0;
+    for (size_t j = 0; j < samples; ++j) {
+        ctrls += is_ctrl[j] != 0;
+        changes += j && (is_ctrl[j] != 0) != (is_ctrl[j - 1] != 0);
+    }
+    const size_t cases = samples - ctrls;
+    {
+        size_t a = 0, b = cases;
+        for (size_t j = 0; j < samples; ++j) idx[is_ctrl[j] ? b++ : a++] = (int32_t)j;
+    }
+    const bool runs = changes <= 1;
+    const uint32_t n_class[2] = {(uint32_t)cases, (uint32_t)ctrls};
+    const uint32_t start[2] = {runs ? (uint32_t)idx[0] : 0u, runs ? (uint32_t)idx[cases] : 0u};
+    const size_t cw = packed_words(cases), tw = packed_words(ctrls);
+    if (!cases || !ctrls) fail(FIU_ERR_INVALID, "zero words per row: one class has no samples");
+    if (!snps) return create_dataset(nullptr, 0, cw, tw, (uint32_t)cases, (uint32_t)ctrls, arena, stream);
+    std::vector<uint8_t> zero;
+    if (reader_flush) {
+        for (const auto &q : flush_quirk_words(is_ctrl, samples)) {
+            if (zero.empty()) zero.assign(cw + tw, 0);
+            zero[q.first ? cw + q.second : q.second] = 1;
+        }
+    }
+    auto d = std::make_unique<fiu_dataset>();
+    d->device = current_device();
+    d->use_arena(arena);
+    d->snps = snps;
+    d->cw = cw;
+    d->tw = tw;
+    d->cases = (uint32_t)cases;
+    d->ctrls = (uint32_t)ctrls;
+    bind_stream(d.get(), stream);
+    const size_t nwords = snps * 3 * (cw + tw);
+    DBuf<uint64_t> words;
+    words.arena = d->arena;
+    words.alloc(nwords);
+    {
+        DBuf<int8_t> g;
+        DBuf<int32_t> di;
+        DBuf<uint8_t> dz;
+        g.arena = di.arena = dz.arena = d->arena;
+        g.alloc((snps * samples + 15) / 16 * 16);   // the kernel reads whole 16-byte blocks
+        check(hipMemcpyAsync(g.p, geno, snps * samples, hipMemcpyHostToDevice, d->stream), "genotype upload");
+        if (!runs) {
+            di.alloc(samples);
+            check(hipMemcpyAsync(di.p, idx.data(), samples * 4, hipMemcpyHostToDevice, d->stream), "sample lists");
+        }
+        if (!zero.empty()) {
+            dz.alloc(zero.size());
+            check(hipMemcpyAsync(dz.p, zero.data(), zero.size(), hipMemcpyHostToDevice, d->stream), "flush words");
+        }
+        check(launch_pack_words(g.p, snps, samples, runs ? nullptr : di.p, n_class, start, cw, tw,
+                                zero.empty() ? nullptr : dz.p, words.p, d->stream),
+              "pack");
+        d->host_words.resize(nwords);
+        check(hipMemcpyAsync(d->host_words.data(), words.p, nwords * 8, hipMemcpyDeviceToHost, d->stream),
+              "words download");
+        check(hipStreamSynchronize(d->stream), "pack sync");
+    }
+    finish_upload(d.get(), words.p);
     return d.release();
 }
 
@@ -828,10 +909,19 @@ fiu_status fiu_dataset_read(const char *const *paths, size_t npaths, fiu_dataset
         if (!out || (!paths && npaths)) fail(FIU_ERR_INVALID, "NULL argument");
         std::vector<std::string> p;
         for (size_t i = 0; i < npaths; ++i) p.emplace_back(paths[i] ? paths[i] : "");
-        PackedDataset pk;
-        read_files(p, pk);
-        *out = create_dataset(pk.words.data(), pk.snps, pk.cases_words, pk.ctrls_words, (uint32_t)pk.cases,
-                              (uint32_t)pk.ctrls);
+        Genotypes g;
+        parse_files(p, g);
+        *out = dataset_from_genotypes(g.geno.data(), g.snps, g.samples, g.is_ctrl.data(), true);
+    });
+}
+
+fiu_status fiu_dataset_from_genotypes(const int8_t *geno, size_t snps, size_t samples, const uint8_t *is_ctrl,
+                                      uint32_t flags, fiu_dataset **out)
+{
+    return guard([&] {
+        if (!out) fail(FIU_ERR_INVALID, "out is NULL");
+        if (flags & ~FIU_GENO_READER_FLUSH) fail(FIU_ERR_INVALID, "unknown flags");
+        *out = dataset_from_genotypes(geno, snps, samples, is_ctrl, (flags & FIU_GENO_READER_FLUSH) != 0);
     });
 }
 

@@ -188,6 +188,12 @@ fiu_dataset *create_dataset(const uint64_t *words, size_t snps, size_t cw, size_
                             uint32_t ctrls, Arena *arena = nullptr, hipStream_t stream = nullptr,
                             bool keep_host_words = true);
 
+// Pack a SNP-major genotype matrix on the device into the reference layout
+// (as RawFile::fill does; `reader_flush`: with the readers' re-flushed words
+// zeroed) and build the device image from it.
+fiu_dataset *dataset_from_genotypes(const int8_t *geno, size_t snps, size_t samples, const uint8_t *is_ctrl,
+                                    bool reader_flush, Arena *arena = nullptr, hipStream_t stream = nullptr);
+
 // Device bytes a dataset of this shape takes (bits, row counts, flags) and
 // the transient staging its upload needs.
 size_t dataset_bytes(size_t snps, size_t cw, size_t tw);

@@ -31,6 +31,18 @@ hipError_t launch_mfma2(const DevData &D, const ScanParams &P, TopState *st, int
                         const uint4 *fp4, hipStream_t s);
 hipError_t launch_merge(TopState *st, Cand *top, const Cand *cand, uint32_t N, uint32_t cap, uint32_t chunk_id,
                         uint32_t *redo, hipStream_t s);
+// Genotypes (SNP-major int8; the buffer allocated to a multiple of 16 bytes)
+// -> reference-layout words of snps x 3(cw+tw).  `idx` null: class c is the
+// run of n_class[c] samples from start[c]; else the per-class sample lists.
+// `zero`: per word slot, 1 = written as 0 (or null).
+constexpr size_t kPackLdsMax = 64 * 1024;
+inline size_t pack_lds_bytes(uint64_t samples, uint64_t cw, uint64_t tw)
+{
+    return (size_t)(cw + tw) * 24 + (size_t)((samples + 31) / 16 * 16);
+}
+hipError_t launch_pack_words(const int8_t *geno, uint64_t snps, uint64_t samples, const int32_t *idx,
+                             const uint32_t n_class[2], const uint32_t start[2], uint64_t cw, uint64_t tw,
+                             const uint8_t *zero, uint64_t *words, hipStream_t s);
 hipError_t launch_transpose(const uint64_t *src, uint4 *dst, int n, int n_pad, int cls, uint64_t cw, uint64_t tw,
                             int quads, uint64_t class_off, hipStream_t s);
 hipError_t launch_singles(const uint4 *bits, const DevData &D, uint32_t *singles, uint32_t *flags, hipStream_t s);

@@ -20,6 +20,7 @@
 
 #include <math.h>
 
+#include <algorithm>
 #include <type_traits>
 #include <utility>
 
@@ -1639,6 +1640,85 @@ __global__ void k_batch_reset(BatchHeader *hdr)
 }
 
 // ---------------------------------------------------------------------------
+// Dataset ingest: genotype matrix -> reference-layout words.
+// ---------------------------------------------------------------------------
+// A workgroup packs whole SNPs (grid-stride).  With `Stage`, the SNP's
+// genotype row is first copied into LDS with aligned 16-byte loads and its
+// 3(cw+tw) output words are assembled in LDS and written out with 16-byte
+// stores; without it (rows too long for LDS) both go straight to memory.
+// One wavefront per word slot (class c, word w), all three rows: lane i takes
+// the genotype of the class's sample 64w+i -- sample start_c + 64w + i of the
+// file when each class is one contiguous run of samples, else the one the
+// per-class sample list `idx` names (cases then controls, each in file
+// order) -- a ballot per genotype value puts lane i at bit i, and the bit
+// reversal puts sample 64w at the most significant bit (RAWFile.hpp:233-260:
+// buff = (buff<<1) + bit).  A last partial word of r samples is right-aligned
+// (RAWFile.hpp:263-283): the reversed ballot shifted right by 64 - r.
+// Padding words, and the slots flagged in `zero` (the readers' second flush
+// of an emptied buffer), are 0.
+struct PackArgs {
+    const int8_t *geno;      // snps x samples, allocated to a multiple of 16 bytes
+    const int32_t *idx;      // per-class sample lists, or null: contiguous classes
+    const uint8_t *zero;     // per word slot (cw + tw), or null
+    uint64_t *words;         // snps x 3(cw + tw)
+    uint64_t samples;
+    uint32_t n_class[2], start[2];
+    uint32_t cw, tw, snps;
+};
+
+template <bool Stage>
+__global__ __launch_bounds__(256) void k_pack_words(PackArgs A)
+{
+    extern __shared__ __align__(16) uint8_t pack_lds[];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t per = A.cw + A.tw;
+    uint64_t *out_lds = (uint64_t *)pack_lds;   // 3 * per words, then the row
+    for (uint32_t s = blockIdx.x; s < A.snps; s += gridDim.x) {
+        const uint64_t r0 = (uint64_t)s * A.samples;
+        const int8_t *row;
+        uint64_t *out;
+        if constexpr (Stage) {
+            const uint64_t a0 = r0 & ~(uint64_t)15, a1 = (r0 + A.samples + 15) & ~(uint64_t)15;
+            uint4 *buf = (uint4 *)(pack_lds + (size_t)per * 24);
+            const uint4 *src = (const uint4 *)(A.geno + a0);
+            __syncthreads();   // the previous SNP's words are out
+            for (uint32_t i = threadIdx.x; i < (uint32_t)((a1 - a0) >> 4); i += blockDim.x) buf[i] = src[i];
+            __syncthreads();
+            row = (const int8_t *)buf + (r0 - a0);
+            out = out_lds;
+        } else {
+            row = A.geno + r0;
+            out = A.words + (uint64_t)s * 3 * per;
+        }
+        for (uint32_t slot = wave; slot < per; slot += blockDim.x >> 6) {
+            const int c = slot >= A.cw;
+            const uint32_t w = c ? slot - A.cw : slot, W = c ? A.tw : A.cw;
+            const int64_t have = (int64_t)A.n_class[c] - 64 * (int64_t)w;   // samples in this word
+            int g = -1;
+            if (lane < have) {
+                const uint32_t pos = 64 * w + lane;
+                const uint64_t j = A.idx ? (uint64_t)A.idx[(c ? A.n_class[0] : 0) + pos] : (uint64_t)A.start[c] + pos;
+                g = row[j];
+            }
+            uint64_t v[3];
+#pragma unroll
+            for (int z = 0; z < 3; ++z) {
+                const uint64_t m = __builtin_bitreverse64(__ballot(g == z));
+                v[z] = have >= 64 ? m : have > 0 ? m >> (64 - have) : 0;
+            }
+            if (A.zero && A.zero[slot]) v[0] = v[1] = v[2] = 0;
+            if (lane < 3) out[(c ? 3 * A.cw : 0) + lane * W + w] = lane == 0 ? v[0] : lane == 1 ? v[1] : v[2];
+        }
+        if constexpr (Stage) {
+            __syncthreads();
+            const uint4 *o = (const uint4 *)out_lds;
+            uint4 *dst = (uint4 *)(A.words + (uint64_t)s * 3 * per);   // 3 * per * 8 bytes: a multiple of 16
+            for (uint32_t i = threadIdx.x; i < per * 3 / 2; i += blockDim.x) dst[i] = o[i];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Dataset image: transpose + per-row counts + coverage check.
 // ---------------------------------------------------------------------------
 __global__ void k_transpose(const uint64_t *__restrict__ src, uint4 *__restrict__ dst, int n, int n_pad,
@@ -1927,6 +2007,30 @@ hipError_t launch_merge(TopState *st, Cand *top, const Cand *cand, uint32_t N, u
                         uint32_t *redo, hipStream_t s)
 {
     hipLaunchKernelGGL(k_merge, dim3(1), dim3(1024), 0, s, st, top, cand, N, cap, chunk_id, redo);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack_words(const int8_t *geno, uint64_t snps, uint64_t samples, const int32_t *idx,
+                             const uint32_t n_class[2], const uint32_t start[2], uint64_t cw, uint64_t tw,
+                             const uint8_t *zero, uint64_t *words, hipStream_t s)
+{
+    if (snps == 0 || cw + tw == 0) return hipSuccess;
+    PackArgs A;
+    A.geno = geno;
+    A.idx = idx;
+    A.zero = zero;
+    A.words = words;
+    A.samples = samples;
+    for (int c = 0; c < 2; ++c) A.n_class[c] = n_class[c], A.start[c] = start[c];
+    A.cw = (uint32_t)cw;
+    A.tw = (uint32_t)tw;
+    A.snps = (uint32_t)snps;
+    const unsigned blocks = (unsigned)std::min<uint64_t>(snps, 8192);
+    const size_t lds = pack_lds_bytes(samples, cw, tw);
+    if (lds <= kPackLdsMax)
+        hipLaunchKernelGGL(k_pack_words<true>, dim3(blocks), dim3(256), lds, s, A);
+    else
+        hipLaunchKernelGGL(k_pack_words<false>, dim3(blocks), dim3(256), 0, s, A);
     return hipGetLastError();
 }
 

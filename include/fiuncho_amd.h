@@ -64,11 +64,31 @@ fiu_status fiu_dataset_create(const uint64_t *words, size_t snps,
                               uint32_t cases, uint32_t ctrls,
                               fiu_dataset **out);
 
-/* Read .raw, or .tped + .tfam (in either order), pack the bit tables with the
- * layout of RawFile::fill / TPEDFile::fill (ALIGN = 64 bytes) and upload.
+/* Read .raw, or .tped + .tfam (in either order), parse them on the host and
+ * pack the bit tables on the device with the layout of RawFile::fill /
+ * TPEDFile::fill (ALIGN = 64 bytes), as fiu_dataset_from_genotypes with
+ * FIU_GENO_READER_FLUSH.
  * Replaces: Dataset<uint64_t>::read (Dataset.h:48-94). */
 fiu_status fiu_dataset_read(const char *const *paths, size_t npaths,
                             fiu_dataset **out);
+
+/* Pack a genotype matrix on the device and keep it there.  `geno` is
+ * SNP-major, snps x samples int8: 0, 1, 2 = minor-allele count, any other
+ * value = uncalled (sets no bit).  `is_ctrl[j]` non-zero makes sample j a
+ * control, zero a case.  The tables are the ones RawFile::fill and
+ * TPEDFile::fill build from these genotypes (RAWFile.hpp:209-290,
+ * TPEDFile.hpp:175-269, ALIGN = 64 bytes): samples keep their order within
+ * their class, full words hold them most-significant bit first, the last
+ * partial word is right-aligned, padding bits and words are zero.  With
+ * FIU_GENO_READER_FLUSH the words that the reference readers write a second
+ * time from an emptied buffer (RAWFile.hpp:226-259, TPEDFile.hpp:213-245)
+ * are zero too, which makes the tables equal to what Dataset<uint64_t>::read
+ * builds from a file holding these genotypes; fiu_dataset_read packs this way.
+ * Replaces: the packing half of Dataset<uint64_t>::read (Dataset.h:48-94). */
+#define FIU_GENO_READER_FLUSH 1u
+fiu_status fiu_dataset_from_genotypes(const int8_t *geno, size_t snps,
+                                      size_t samples, const uint8_t *is_ctrl,
+                                      uint32_t flags, fiu_dataset **out);
 
 /* Host-only read of the same files into the reference host layout, without a
  * device: call with words = NULL to learn the shape, then with a buffer of

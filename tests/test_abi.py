@@ -134,3 +134,48 @@ def test_tuning_table_roundtrip_and_validation():
         for a, v in tune.candidates(k):
             tune.set_table([E(k, 0, 10, 0, 10, a, v)])
     tune.clear()
+
+
+def test_genotype_ingest_validation_before_any_device_call():
+    import pytest
+    g = np.zeros((3, 4), np.int8)
+    for geno, is_ctrl, flags in ((g, np.array([0, 1, 0, 1], np.uint8), 2),        # unknown flag
+                                 (np.zeros((3, 0), np.int8), np.zeros(0, np.uint8), 0),   # no samples
+                                 (g, np.zeros(4, np.uint8), 0),                   # no controls
+                                 (g, np.ones(4, np.uint8), 0)):                   # no cases
+        h = C.c_void_p()
+        with pytest.raises(fa.FiuError) as e:
+            _lib.check(_lib.lib().fiu_dataset_from_genotypes(geno.ctypes.data, geno.shape[0], geno.shape[1],
+                                                             is_ctrl.ctypes.data, flags, C.byref(h)))
+        assert e.value.status == _lib.FIU_ERR_INVALID
+    with pytest.raises(ValueError):
+        fa.Dataset.from_genotypes(g, np.zeros(3, np.uint8))
+
+
+def test_parallel_parse_reports_the_first_bad_line(tmp_path):
+    # files large enough to be parsed on several threads: the error names the
+    # first bad line of the file, as a line-by-line reader would
+    import pytest
+    rng = np.random.default_rng(4)
+    g = rng.integers(0, 3, size=(3000, 5))
+    rows = [" ".join(["f", "i", "0", "0", "1", str(1 + (j % 2))] + [str(v) for v in g[j]]) for j in range(3000)]
+    rows[2700] = rows[2700].replace(" 1 ", " NA ", 1) if " 1 " in rows[2700] else rows[2700] + " x"
+    rows[1700] = rows[1700].rsplit(" ", 1)[0] + " NA"
+    p = tmp_path / "big.raw"
+    p.write_text("FID IID PAT MAT SEX PHENOTYPE " + " ".join(f"S{i}_A" for i in range(5)) + "\n" + "\n".join(rows) + "\n")
+    with pytest.raises(fa.FiuError, match="at line 1702$"):
+        fa.read_files([str(p)])
+    rows[1700] = rows[1700].rsplit(" ", 1)[0] + " 2"
+    rows[2700] = " ".join(["f", "i", "0", "0", "1", "1"] + ["0"] * 5)
+    p.write_text("FID IID PAT MAT SEX PHENOTYPE " + " ".join(f"S{i}_A" for i in range(5)) + "\n" + "\n".join(rows) + "\n")
+    words, snps, cases, ctrls, _, _ = fa.read_files([str(p)])
+    assert (snps, cases, ctrls) == (5, 1500, 1500)
+    tfam = tmp_path / "b.tfam"
+    tfam.write_text("".join(f"f{k} i{k} 0 0 1 {1 + k % 2}\n" for k in range(4)))
+    lines = [f"1 rs{s} 0 {s} A C A C A A C C" for s in range(1200)]
+    lines[1000] = "1 rs 0 1 A C A C A A C"       # odd number of nucleotides
+    lines[600] = "1 rs 0 1 A C A C A A C N"      # invalid nucleotide
+    tped = tmp_path / "b.tped"
+    tped.write_text("\n".join(lines) + "\n")
+    with pytest.raises(fa.FiuError, match=r"b\.tped:601: invalid nucleotide value 'N'"):
+        fa.read_files([str(tped), str(tfam)])
