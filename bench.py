@@ -420,6 +420,7 @@ def main():
     ap.add_argument("--slices", type=int, default=1,
                     help="a step scans Distribution(n, 2, SLICES, i): 1 = the whole order-3 scan")
     ap.add_argument("--algorithm", default="auto", choices=["auto", "direct", "marginal", "mfma"])
+    ap.add_argument("--variant", type=int, default=0, help="kernel variant of the algorithm (0 = default)")
     ap.add_argument("--cpu-sample-step", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the uneven-batch comparison")
@@ -484,7 +485,7 @@ def main():
     words, geno, cw, tw = fa.synth_generate(spec, words=True, geno=not (args.no_cpu and args.no_ingest))
     ds = fa.Dataset.from_words(words, N_SNPS, cw, tw, N_CASES, N_CTRLS)
     print(f"[bench] dataset generated and uploaded in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
-    search = fa.GpuSearch(algorithm=algo)
+    search = fa.GpuSearch(algorithm=algo, variant=args.variant)
     S = args.slices
     # warmup (never timed): small shares of the scan build the pair table and
     # load the code; then the timed steps
@@ -548,7 +549,7 @@ def main():
         "config": {"workload": "order-3 exhaustive scan, 10,000 SNPs x 16,000 samples (8,000/8,000), top-100",
                    "step": "one full scan" if S == 1 else f"one Distribution(10000, 2, {S}, i) share",
                    "combinations": combos,
-                   "algorithm": algo_name},
+                   "algorithm": algo_name, "variant": search.last_stats.variant},
         "roofline": roofline,
         "top1": {"combination": list(top[0][0]), "mi": top[0][1]} if top else None,
     }

@@ -1276,45 +1276,57 @@ __device__ __forceinline__ void mfma3_tile(const DevData &D, const ScanParams &P
             dma16(rq, uc, s + (8 + w) * kM3Group);
             if (a_lane) dma16(rq, ua, s + 12 * kM3Group);
         };
-#pragma unroll
-        for (int q = 0; q < R - 1; ++q)
-            if (q < iters) issue(q);
-        for (int it = 0; it < iters; ++it) {
-            // this wave's loads of stage it are complete once at most those of
-            // the stages after it (min(iters - 1 - it, R - 2), 4 loads each)
-            // are pending; the barrier then publishes every wave's part
-            static_assert(R >= 2 && 4 * (R - 2) < 16, "ring depth");
-            wait_stages<R - 2, 4>(iters - 1 - it);
-            stage_barrier();
-            const int8_t *sb = lds[it % R];
+        struct Frag {
             v4i ma[2], fb[2][2], fc[2];
+        };
+        auto read = [&](Frag &f, int it) __attribute__((always_inline)) {
+            const int8_t *sb = lds[it % R];
 #pragma unroll
-            for (int x = 0; x < 2; ++x) ma[x] = *(const v4i *)(sb + 12 * kM3Group + x * 64 + ao);
+            for (int x = 0; x < 2; ++x) f.ma[x] = *(const v4i *)(sb + 12 * kM3Group + x * 64 + ao);
 #pragma unroll
             for (int y = 0; y < 2; ++y)
 #pragma unroll
-                for (int bb = 0; bb < 2; ++bb) fb[y][bb] = *(const v4i *)(sb + (4 * y + 2 * wb + bb) * kM3Group + fo);
+                for (int bb = 0; bb < 2; ++bb)
+                    f.fb[y][bb] = *(const v4i *)(sb + (4 * y + 2 * wb + bb) * kM3Group + fo);
 #pragma unroll
-            for (int z = 0; z < 2; ++z) fc[z] = *(const v4i *)(sb + (8 + 2 * z + wc) * kM3Group + fo);
-            // every wave is past stage it - 1, whose slot now takes stage it + R - 1
-            if (it + R - 1 < iters) issue(it + R - 1);
-            // the a mask goes on the narrower operand (a wave's 16 c rows per
-            // z, against 32 b rows per y): 4 masked fragments per stage
+            for (int z = 0; z < 2; ++z) f.fc[z] = *(const v4i *)(sb + (8 + 2 * z + wc) * kM3Group + fo);
+        };
+        // the a mask goes on the narrower operand (a wave's 16 c rows per
+        // z, against 32 b rows per y): 4 masked fragments per stage
+        auto multiply = [&](const Frag &f) __attribute__((always_inline)) {
 #pragma unroll
             for (int x = 0; x < 2; ++x)
 #pragma unroll
                 for (int z = 0; z < 2; ++z) {
-                    const v4i mc = fc[z] & ma[x];   // nibble 0x2 & 0x2 = 1.0, else 0
+                    const v4i mc = f.fc[z] & f.ma[x];   // nibble 0x2 & 0x2 = 1.0, else 0
                     const v8i c8 = {mc[0], mc[1], mc[2], mc[3], 0, 0, 0, 0};
 #pragma unroll
                     for (int y = 0; y < 2; ++y)
 #pragma unroll
                         for (int bb = 0; bb < 2; ++bb) {
-                            const v8i b8 = {fb[y][bb][0], fb[y][bb][1], fb[y][bb][2], fb[y][bb][3], 0, 0, 0, 0};
+                            const v8i b8 = {f.fb[y][bb][0], f.fb[y][bb][1], f.fb[y][bb][2], f.fb[y][bb][3],
+                                            0, 0, 0, 0};
                             acc[cls][x][y][bb][z] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(
                                 b8, c8, acc[cls][x][y][bb][z], 4, 4, 0, 0, 0, 0);
                         }
                 }
+        };
+#pragma unroll
+        for (int q = 0; q < R - 1; ++q)
+            if (q < iters) issue(q);
+        for (int it = 0; it < iters; ++it) {
+            // this wave's loads of stage it are complete once at most
+            // those of the stages after it (min(iters - 1 - it, R - 2),
+            // 4 loads each) are pending; the barrier then publishes every
+            // wave's part
+            static_assert(R >= 2 && 4 * (R - 2) < 16, "ring depth");
+            wait_stages<R - 2, 4>(iters - 1 - it);
+            stage_barrier();
+            Frag f;
+            read(f, it);
+            // every wave is past stage it - 1, whose slot now takes stage it + R - 1
+            if (it + R - 1 < iters) issue(it + R - 1);
+            multiply(f);
         }
         // the next class's first loads reuse the ring: every wave must be
         // done reading this class's last stages

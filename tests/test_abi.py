@@ -114,6 +114,7 @@ def test_tuning_table_roundtrip_and_validation():
         assert abs(got[0].ms - 1.5) < 1e-6
         bad = [E(3, 0, 10, 0, 10, fa.ALGO_MARGINAL, 8),     # 8 prefixes per wave: order 2 only
                E(2, 0, 10, 0, 10, fa.ALGO_MFMA, 3),         # ring depth: order 3 only
+               E(3, 0, 10, 0, 10, fa.ALGO_MFMA, 5),         # no such staging variant
                E(4, 0, 10, 0, 10, fa.ALGO_MFMA, 0),         # matrix cores: orders 2-3
                E(2, 10, 10, 0, 10, fa.ALGO_DIRECT, 0),      # empty SNP range
                E(2, 0, 10, 0, 10, fa.ALGO_AUTO, 0),         # AUTO is not a choice

@@ -34,11 +34,12 @@ def test_mfma3_vs_oracle(snps, cases, ctrls, outputs, seed):
     print(f"case: mfma3 {snps}x{cases}+{ctrls}", flush=True)
     d, od = synth(snps, cases, ctrls, seed)
     oc, om, _ = O.search(od, 3, outputs, threads=4, mi_mode=2)
-    s = fa.GpuSearch(algorithm=fa.ALGO_MFMA)
-    c, m = s.run_arrays(d, 3, fa.Distribution(snps, 2), outputs)
-    assert s.last_stats.algorithm == fa.ALGO_MFMA
-    assert c.tolist() == oc.tolist() and np.array_equal(m, om)
-    assert s.last_stats.combinations == math.comb(snps, 3)
+    for variant in (0, 3):   # default (4-stage ring), 3-stage ring
+        s = fa.GpuSearch(algorithm=fa.ALGO_MFMA, variant=variant)
+        c, m = s.run_arrays(d, 3, fa.Distribution(snps, 2), outputs)
+        assert s.last_stats.algorithm == fa.ALGO_MFMA
+        assert c.tolist() == oc.tolist() and np.array_equal(m, om), variant
+        assert s.last_stats.combinations == math.comb(snps, 3)
 
 
 def test_mfma3_default_and_planted():
