@@ -430,7 +430,8 @@ bool valid_variant(int algo, int order, int variant)
 {
     if (variant == 0) return true;
     switch (algo) {
-    case FIU_ALGO_MARGINAL: return variant == 2 || variant == kMarginalPW || (order == 2 && variant == 8);
+    case FIU_ALGO_MARGINAL:
+        return order <= 3 && (variant == 2 || variant == kMarginalPW || (order == 2 && variant == 8));
     case FIU_ALGO_MFMA: return order == 3 ? (variant == 3 || variant == 4) : variant == 1;
     default: return false;
     }
@@ -620,6 +621,9 @@ void search(fiu_dataset *d, int order, int32_t step, int32_t offset, uint32_t ou
         fail(FIU_ERR_INVALID, "marginal algorithm needs every SNP to cover the same samples");
     if (algo_req == FIU_ALGO_MFMA && order != 2 && order != 3)
         fail(FIU_ERR_UNSUPPORTED, "the matrix-core scan covers orders 2 and 3");
+    if (order > kMaxDirectOrder && (algo_req == FIU_ALGO_DIRECT || !d->dev.uniform))
+        fail(FIU_ERR_UNSUPPORTED, "order " + std::to_string(order) +
+                                      " is scanned marginally only: every SNP must cover the same samples");
     if (algo_req != FIU_ALGO_AUTO && !valid_variant(algo_req, order, variant))
         fail(FIU_ERR_INVALID, "unknown variant " + std::to_string(variant) + " of this algorithm");
     // AUTO: the tuned choice of this shape class when the dataset allows it
@@ -654,6 +658,7 @@ void search(fiu_dataset *d, int order, int32_t step, int32_t offset, uint32_t ou
     ensure_binom(d, M, s);
     int algo = FIU_ALGO_DIRECT;
     const bool small_classes = d->cases < (1u << 24) && d->ctrls < (1u << 24);   // fp32-exact counts
+    if (algo_req != FIU_ALGO_DIRECT && d->dev.uniform && order >= 4) algo = FIU_ALGO_MARGINAL;
     if (algo_req != FIU_ALGO_DIRECT && d->dev.uniform && (order == 2 || order == 3)) {
         algo = FIU_ALGO_MARGINAL;
         if (order == 3 && !ensure_pair_table(d, s)) algo = FIU_ALGO_DIRECT;  // table does not fit
@@ -760,6 +765,8 @@ void search(fiu_dataset *d, int order, int32_t step, int32_t offset, uint32_t ou
         } else if (algo == FIU_ALGO_MFMA) {
             check(launch_mfma3(d->dev, P, d->state.p, d->fp4.p, (int)chunks[ci].j0, (int)chunks[ci].j1, T, ring, s),
                   "scan launch");
+        } else if (algo == FIU_ALGO_MARGINAL && order >= 4) {
+            check(launch_scan_km(order, d->dev, P, d->state.p, s), "scan launch");
         } else if (algo == FIU_ALGO_MARGINAL) {
             check(launch_scan_marginal(order, d->dev, P, d->state.p, s, pw), "scan launch");
         } else {
@@ -777,7 +784,9 @@ void search(fiu_dataset *d, int order, int32_t step, int32_t offset, uint32_t ou
         H.j0 = (int64_t)n_pref - sample_prefixes;
         H.j1 = (int64_t)n_pref;
         H.hist = d->hist.p;
-        if (algo == FIU_ALGO_MARGINAL || algo == FIU_ALGO_MFMA)
+        if (algo == FIU_ALGO_MARGINAL && order >= 4)
+            check(launch_scan_km(order, d->dev, H, d->state.p, s), "sample launch");
+        else if (algo == FIU_ALGO_MARGINAL || algo == FIU_ALGO_MFMA)
             check(launch_scan_marginal(order, d->dev, H, d->state.p, s), "sample launch");
         else
             check(launch_scan_direct(order, d->dev, H, d->state.p, s), "sample launch");
@@ -864,8 +873,8 @@ fiu_status fiu_set_tuning(const fiu_tune_entry *entries, size_t n)
                 fail(FIU_ERR_INVALID, "tuning entry with an unknown algorithm");
             if (e.algorithm == FIU_ALGO_MFMA && e.order != 2 && e.order != 3)
                 fail(FIU_ERR_INVALID, "matrix-core tuning entry above order 3");
-            if (e.algorithm == FIU_ALGO_MARGINAL && e.order > 3)
-                fail(FIU_ERR_INVALID, "marginal tuning entry above order 3");
+            if (e.algorithm == FIU_ALGO_DIRECT && e.order > kMaxDirectOrder)
+                fail(FIU_ERR_INVALID, "direct tuning entry above order 4");
             if (!valid_variant(e.algorithm, e.order, e.variant)) fail(FIU_ERR_INVALID, "tuning entry with an unknown variant");
             if (e.snps_lo >= e.snps_hi || e.samples_lo >= e.samples_hi)
                 fail(FIU_ERR_INVALID, "tuning entry with an empty range");

@@ -322,7 +322,8 @@ void Engine::plan_job(Job *j)
     fiu_dataset *d = j->ds;
     const int order = j->order, M = order - 1;
     if (order < 2) fail(FIU_ERR_INVALID, "order must be >= 2");
-    if (order > kMaxOrder) fail(FIU_ERR_UNSUPPORTED, "order " + std::to_string(order) + " is above this build's limit");
+    if (order > kMaxBatchOrder)
+        fail(FIU_ERR_UNSUPPORTED, "order " + std::to_string(order) + " is above the batched scans' limit (4)");
     if (j->step < 1) fail(FIU_ERR_INVALID, "distribution step must be >= 1");
     if (j->offset < 0) fail(FIU_ERR_INVALID, "distribution offset must be >= 0");
     if (j->outputs > (uint32_t)kMaxOutputs) fail(FIU_ERR_UNSUPPORTED, "outputs above " + std::to_string(kMaxOutputs));
@@ -872,7 +873,7 @@ uint64_t Server::submit(const fiu_request *req)
     j->offset = req->dist_offset;
     j->outputs = req->outputs;
     if (req->order < 2) fail(FIU_ERR_INVALID, "order must be >= 2");
-    if (req->order > kMaxOrder) fail(FIU_ERR_UNSUPPORTED, "order above this build's limit");
+    if (req->order > kMaxBatchOrder) fail(FIU_ERR_UNSUPPORTED, "order above the batched scans' limit (4)");
     if (req->dist_step < 1 || req->dist_offset < 0) fail(FIU_ERR_INVALID, "bad distribution");
     if (req->outputs > (uint32_t)kMaxOutputs) fail(FIU_ERR_UNSUPPORTED, "outputs above " + std::to_string(kMaxOutputs));
     if (req->dataset) {

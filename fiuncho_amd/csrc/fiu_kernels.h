@@ -5,6 +5,9 @@
 
 namespace fiu {
 
+// Orders 4 and 5, marginal in the last SNP (uniform datasets); P.hist set:
+// the threshold sampling pass.
+hipError_t launch_scan_km(int order, const DevData &D, const ScanParams &P, TopState *st, hipStream_t s);
 hipError_t launch_scan_direct(int order, const DevData &D, const ScanParams &P, TopState *st, hipStream_t s);
 // pw: prefixes per wavefront (2, 4 or, order 2 only, 8; the threshold
 // sampling pass uses 4)

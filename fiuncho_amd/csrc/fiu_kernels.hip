@@ -128,22 +128,41 @@ __device__ inline void unrank(uint64_t r, int n, int m, const uint64_t *binom, i
 // all have one sign (no cancellation inside the sum).  Error < 2e-6 nats for
 // N < 2^24; only used to decide which combinations get the exact value.
 // Counts as uint32_t or as FP32 holding the same integers (exact below 2^24).
-template <int CELLS, class T>
-__device__ __forceinline__ float mi_screen(const T *a0, const T *a1, float hy, float inv_n)
+// The sum over cells, in log2 units (the screen of a table split over
+// several lanes adds their sums).
+// Cells come from cell(x, fc, ft) (counts as FP32), x < CELLS.
+template <int CELLS, class F>
+__device__ __forceinline__ float screen_sum_fn(F &&cell)
 {
     // hardware v_rcp_f32 / v_log_f32 (log2) on normal arguments: every ratio
     // is clamped to >= 2^-100, and a zero count multiplies its log by zero
     float s = 0.0f;
 #pragma unroll
     for (int x = 0; x < CELLS; ++x) {
-        const float fc = (float)a0[x], ft = (float)a1[x];
+        float fc, ft;
+        cell(x, fc, ft);
         const float r = __builtin_amdgcn_rcpf(fmaxf(fc + ft, 1.0f));
         const float lc = __builtin_amdgcn_logf(fmaxf(fc * r, 7.9e-31f));
         const float lt = __builtin_amdgcn_logf(fmaxf(ft * r, 7.9e-31f));
         s = fmaf(fc, lc, s);
         s = fmaf(ft, lt, s);
     }
-    return fmaf(s, inv_n * 0.69314718f, hy);
+    return s;
+}
+
+template <int CELLS, class T>
+__device__ __forceinline__ float screen_sum(const T *a0, const T *a1)
+{
+    return screen_sum_fn<CELLS>([&](int x, float &fc, float &ft) __attribute__((always_inline)) {
+        fc = (float)a0[x];
+        ft = (float)a1[x];
+    });
+}
+
+template <int CELLS, class T>
+__device__ __forceinline__ float mi_screen(const T *a0, const T *a1, float hy, float inv_n)
+{
+    return fmaf(screen_sum<CELLS>(a0, a1), inv_n * 0.69314718f, hy);
 }
 
 // Upper bound on the MI in nats, one reciprocal per cell and no logarithm:
@@ -152,17 +171,30 @@ __device__ __forceinline__ float mi_screen(const T *a0, const T *a1, float hy, f
 //   chi2/N = N/(n_c n_t) sum_x d_x^2 / n_x,  d_x = n_cx - n_x n_c/N.
 // FP32 error: below 2e-6 relative plus 5e-7 absolute (each |d_x| error is
 // under 1.2e-7 (n_x n_c/N + |d_x|), and sum_x |d_x| <= 2 min(n_c, n_t)).
-template <int CELLS, class T>
-__device__ __forceinline__ float mi_bound(const T *a0, const T *a1, float frac_c, float chi_scale)
+// Cells from cell(x, fc, fx): case count and case + control count, FP32.
+template <int CELLS, class F>
+__device__ __forceinline__ float mi_bound_fn(F &&cell, float frac_c, float chi_scale)
 {
     float s = 0.0f;
 #pragma unroll
     for (int x = 0; x < CELLS; ++x) {
-        const float fx = (float)(a0[x] + a1[x]);
-        const float d = fmaf(-fx, frac_c, (float)a0[x]);
+        float fc, fx;
+        cell(x, fc, fx);
+        const float d = fmaf(-fx, frac_c, fc);
         s = fmaf(d * d, __builtin_amdgcn_rcpf(fmaxf(fx, 1.0f)), s);
     }
     return s * chi_scale;
+}
+
+template <int CELLS, class T>
+__device__ __forceinline__ float mi_bound(const T *a0, const T *a1, float frac_c, float chi_scale)
+{
+    return mi_bound_fn<CELLS>(
+        [&](int x, float &fc, float &fx) __attribute__((always_inline)) {
+            fx = (float)(a0[x] + a1[x]);
+            fc = (float)a0[x];
+        },
+        frac_c, chi_scale);
 }
 
 // The bound rejects a combination when even its bound, raised by far more
@@ -185,18 +217,42 @@ __device__ __noinline__ double plogp(double p)
 // Exact MI: H(X) + H(Y) - H(X,Y) in FP64 from the integer counts, summed in
 // cell order with no fused multiply-add, so that the value depends only on
 // the counts (not on tiling, batching or launch shape).
+// The running sums continue over CELLS consecutive cells (a table split over
+// lanes in cell order is one chain through them).
+// Cells from cell(x, n0, n1) (integer counts).
+template <int CELLS, class F>
+__device__ __forceinline__ void mi_exact_part_fn(F &&cell, double n, double &hx, double &hall)
+{
+#pragma clang fp contract(off)
+#pragma unroll
+    for (int x = 0; x < CELLS; ++x) {
+        uint32_t n0, n1;
+        cell(x, n0, n1);
+        const double pc = (double)n0 / n, pt = (double)n1 / n, pa = pc + pt;
+        hall -= plogp(pc);
+        hall -= plogp(pt);
+        hx -= plogp(pa);
+    }
+}
+
+template <int CELLS>
+__device__ __forceinline__ void mi_exact_part(const uint32_t *a0, const uint32_t *a1, double n, double &hx,
+                                              double &hall)
+{
+    mi_exact_part_fn<CELLS>(
+        [&](int x, uint32_t &n0, uint32_t &n1) __attribute__((always_inline)) {
+            n0 = a0[x];
+            n1 = a1[x];
+        },
+        n, hx, hall);
+}
+
 template <int CELLS>
 __device__ __forceinline__ double mi_exact(const uint32_t *a0, const uint32_t *a1, double n, double hy)
 {
 #pragma clang fp contract(off)
     double hx = 0.0, hall = 0.0;
-#pragma unroll
-    for (int x = 0; x < CELLS; ++x) {
-        const double pc = (double)a0[x] / n, pt = (double)a1[x] / n, pa = pc + pt;
-        hall -= plogp(pc);
-        hall -= plogp(pt);
-        hx -= plogp(pa);
-    }
+    mi_exact_part<CELLS>(a0, a1, n, hx, hall);
     return hx + hy - hall;
 }
 
@@ -475,6 +531,218 @@ k_scan_direct(const DevData D, const ScanParams P, TopState *st)
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     direct_unit<K, HIST>(D, P, st, P.j0 + (int64_t)blockIdx.x * (kBlock / kWave) + wid, P.j1, INT32_MAX, 0,
                          lds_hist);
+    hist_end<HIST>(lds_hist, P.hist);  // every wave of the block reaches the barrier
+}
+
+// ---------------------------------------------------------------------------
+// Orders 4 and 5, marginal in the last SNP (FIU_ALGO_MARGINAL).
+// ---------------------------------------------------------------------------
+// A wavefront takes one prefix (s_0 < ... < s_{k-2}) of the distribution and
+// the last SNPs c after it.  The prefix's 3^(k-1) genotype rows (ANDs of one
+// row per prefix SNP, as GenotypeTable::combine chains them,
+// gt_combine.cpp:3-27) are split over L = 3^(k-3) lanes per c by the
+// genotypes of the first k-3 prefix SNPs (the lane's part p); the 9 rows of
+// the last two prefix SNPs are wave-uniform scalars built per 32-bit word
+// from scalar loads.  A lane ANDs each of them with its part's row and with
+// c's rows 0 and 1, and counts (AND + popcount, gt_popcnt.cpp:10-18).
+// Genotype 2 of c follows from the prefix row's own count, taken once per
+// prefix: c's rows partition the samples that every SNP covers (checked at
+// upload).  Cells counted per word: 2 * 3^(k-1) instead of 3^k.  Cell
+// x = 3 g + z with g = 9 p + r, first SNP most significant
+// (ContingencyTable.h:67-90): the L lanes of a c hold consecutive cell
+// ranges, so the bound and the screen are sums over them and the exact MI is
+// one chain through them in cell order.  Lanes: order 4, 21 c x 3 parts;
+// order 5, 7 c x 9 parts (lane 63 idles).
+template <int K> struct Km {
+    static_assert(K == 4 || K == 5, "orders 4 and 5");
+    static constexpr int LS = K - 3;            // prefix SNPs split over lanes
+    static constexpr int L = Pow3<K - 3>::v;     // lanes per c
+    static constexpr int CPW = 64 / L;           // c per wavefront
+    static constexpr int NR = 9;                 // uniform rows (last two prefix SNPs)
+    static constexpr int LC = 3 * NR;            // cells per lane
+};
+
+// One class: acc[r][z] = sum over words of popcount(row r & f & c_z), z < Z
+// (Z = 2: c's rows 0 and 1; Z = 1: no c, the prefix rows' own counts).
+// f = AND of rows digit_i(p) of the lane-split SNPs pre[i] (lane offsets
+// voff_f[i]); row r = AND of rows r / 3 and r % 3 of the last two prefix
+// SNPs (scalar loads: their indices are wave-uniform); c at voff_c.
+template <int K, int Z>
+__device__ __forceinline__ void km_count(const uint4 *__restrict__ base, int np, int nq, const int (&pre)[K - 1],
+                                         const int (&voff_f)[Km<K>::LS], int voff_c, uint32_t (&acc)[9][Z])
+{
+    constexpr int LS = Km<K>::LS, NR = Km<K>::NR;
+#pragma unroll
+    for (int r = 0; r < NR; ++r)
+#pragma unroll
+        for (int z = 0; z < Z; ++z) acc[r][z] = 0;
+    if (nq <= 0) return;
+    const int u0 = pre[K - 3], u1 = pre[K - 2];
+    const uint4 *row = base;
+    __amdgpu_buffer_rsrc_t rs = quad_rsrc(row, np);
+    uint4 f[LS], cz[Z], pz[2][3];
+    auto load = [&](const uint4 *rw, __amdgpu_buffer_rsrc_t r, uint4(&lf)[LS], uint4(&lc)[Z], uint4(&lp)[2][3])
+        __attribute__((always_inline)) {
+#pragma unroll
+        for (int i = 0; i < LS; ++i) lf[i] = ld_lane(r, np, 0, voff_f[i]);
+        if constexpr (Z == 2) {
+#pragma unroll
+            for (int z = 0; z < Z; ++z) lc[z] = ld_lane(r, np, z, voff_c);
+        }
+#pragma unroll
+        for (int z = 0; z < 3; ++z) {
+            lp[0][z] = rw[z * np + u0];
+            lp[1][z] = rw[z * np + u1];
+        }
+    };
+    load(row, rs, f, cz, pz);
+    for (int q = 0; q < nq; ++q) {
+        const uint4 *nrow = q + 1 < nq ? row + (size_t)3 * np : row;
+        const __amdgpu_buffer_rsrc_t nr = quad_rsrc(nrow, np);
+        uint4 nf[LS], ncz[Z], npz[2][3];
+        load(nrow, nr, nf, ncz, npz);
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+            uint32_t rows[NR];
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int b = 0; b < 3; ++b) rows[a * 3 + b] = comp(pz[0][a], w) & comp(pz[1][b], w);
+            uint32_t fw = comp(f[0], w);
+#pragma unroll
+            for (int i = 1; i < LS; ++i) fw &= comp(f[i], w);
+            if constexpr (Z == 2) {
+                const uint32_t x0 = comp(cz[0], w) & fw, x1 = comp(cz[1], w) & fw;
+#pragma unroll
+                for (int r = 0; r < NR; ++r) {
+                    acc[r][0] += __builtin_popcount(rows[r] & x0);
+                    acc[r][1] += __builtin_popcount(rows[r] & x1);
+                }
+            } else {
+#pragma unroll
+                for (int r = 0; r < NR; ++r) acc[r][0] += __builtin_popcount(rows[r] & fw);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < LS; ++i) f[i] = nf[i];
+#pragma unroll
+        for (int z = 0; z < Z; ++z) cz[z] = ncz[z];
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int z = 0; z < 3; ++z) pz[a][z] = npz[a][z];
+        rs = nr;
+        row = nrow;
+    }
+}
+
+template <int K, bool HIST>
+__device__ __forceinline__ void km_unit(const DevData &D, const ScanParams &P, TopState *st, int64_t j, int64_t j1,
+                                        uint32_t *lds_hist)
+{
+    constexpr int M = K - 1, LS = Km<K>::LS, L = Km<K>::L, CPW = Km<K>::CPW, NR = Km<K>::NR, LC = Km<K>::LC;
+    int pre[M];
+    if (!prefix_at<M>(P, D.n, j, j1, pre)) return;
+#pragma unroll
+    for (int i = 0; i < M; ++i) pre[i] = __builtin_amdgcn_readfirstlane(pre[i]);
+    uint64_t pre_packed = 0;
+#pragma unroll
+    for (int i = 0; i < M; ++i) pre_packed = (pre_packed << P.combo_bits) | (uint64_t)pre[i];
+    pre_packed <<= P.combo_bits;
+    const int last = pre[M - 1], n = D.n, np = D.n_pad;
+    const int lane = threadIdx.x & 63, part = lane % L, first_lane = lane - part;
+    const bool lane_ok = lane < CPW * L;
+    int voff_f[LS];
+#pragma unroll
+    for (int i = 0; i < LS; ++i) {
+        int digit = part;
+#pragma unroll
+        for (int e = i + 1; e < LS; ++e) digit /= 3;
+        voff_f[i] = 16 * ((digit % 3) * np + pre[i]);
+    }
+    const uint4 *b0 = D.bits + D.class_off[0];
+    const uint4 *b1 = D.bits + D.class_off[1];
+    // the prefix rows' own counts (genotype "any" of c)
+    uint32_t pr0[NR][1], pr1[NR][1];
+    km_count<K, 1>(b0, np, D.quads[0], pre, voff_f, 0, pr0);
+    km_count<K, 1>(b1, np, D.quads[1], pre, voff_f, 0, pr1);
+    const float tau = st->tau;
+    const uint64_t tau_combo = st->tau_combo;
+    // sums over the L lanes of this c, in the same order in each
+    auto sum_parts = [&](float v) __attribute__((always_inline)) {
+        float t = __shfl(v, first_lane);
+#pragma unroll
+        for (int i = 1; i < L; ++i) t += __shfl(v, first_lane + i);
+        return t;
+    };
+    for (int c0 = last + 1; c0 < n; c0 += CPW) {
+        const int c = c0 + lane / L;
+        const bool valid = lane_ok && c < n;
+        const int voff_c = 16 * min(c, np - 1);
+        uint32_t k0[NR][2], k1[NR][2];
+        km_count<K, 2>(b0, np, D.quads[0], pre, voff_f, voff_c, k0);
+        km_count<K, 2>(b1, np, D.quads[1], pre, voff_f, voff_c, k1);
+        // this lane's cell x = 3 r + z (z = 2: the prefix row's count minus
+        // genotypes 0 and 1), computed where it is used
+        auto cnt = [&](int x, uint32_t &n0, uint32_t &n1) __attribute__((always_inline)) {
+            const int r = x / 3, z = x % 3;
+            n0 = z < 2 ? k0[r][z] : pr0[r][0] - k0[r][0] - k0[r][1];
+            n1 = z < 2 ? k1[r][z] : pr1[r][0] - k1[r][0] - k1[r][1];
+        };
+        auto cnt_screen = [&](int x, float &fc, float &ft) __attribute__((always_inline)) {
+            uint32_t n0, n1;
+            cnt(x, n0, n1);
+            fc = (float)n0;
+            ft = (float)n1;
+        };
+        auto cnt_bound = [&](int x, float &fc, float &fx) __attribute__((always_inline)) {
+            uint32_t n0, n1;
+            cnt(x, n0, n1);
+            fc = (float)n0;
+            fx = (float)(n0 + n1);
+        };
+        if constexpr (HIST) {
+            const float ms = fmaf(sum_parts(screen_sum_fn<LC>(cnt_screen)), P.inv_n * 0.69314718f, P.hy);
+            if (valid && part == 0) atomicAdd(&lds_hist[hist_bin(ms)], 1u);
+        } else {
+            bool pass = valid;
+            if (P.bounded)
+                pass = pass && bound_passes(sum_parts(mi_bound_fn<LC>(cnt_bound, P.frac_c, P.chi_scale)), tau, P.delta);
+            float ms = 0.0f;
+            if (__ballot(pass)) ms = fmaf(sum_parts(screen_sum_fn<LC>(cnt_screen)), P.inv_n * 0.69314718f, P.hy);
+            const bool exact = pass && ms >= tau - P.delta;
+            bool want = false;
+            float mf = 0.0f;
+            if (__ballot(exact)) {
+                // one FP64 chain through the L lanes, in cell order
+                double hx = 0.0, hall = 0.0;
+                for (int p = 0; p < L; ++p) {
+                    if (exact && part == p) mi_exact_part_fn<LC>(cnt, P.n64, hx, hall);
+                    const double nhx = __shfl(hx, lane > 0 ? lane - 1 : 0);
+                    const double nhall = __shfl(hall, lane > 0 ? lane - 1 : 0);
+                    if (part == p + 1) {
+                        hx = nhx;
+                        hall = nhall;
+                    }
+                }
+                if (exact && part == L - 1) {
+#pragma clang fp contract(off)
+                    mf = (float)(hx + P.hy64 - hall);
+                    want = better(mf, pre_packed | (uint64_t)c, tau, tau_combo);
+                }
+            }
+            push_candidates(want, mf, pre_packed | (uint64_t)c, P, st);
+        }
+    }
+}
+
+template <int K, bool HIST>
+__global__ void __launch_bounds__(kBlock) k_scan_km(const DevData D, const ScanParams P, TopState *st)
+{
+    __shared__ uint32_t lds_hist[HIST ? kHistBins : 1];
+    hist_begin<HIST>(lds_hist);
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    km_unit<K, HIST>(D, P, st, P.j0 + (int64_t)blockIdx.x * (kBlock / kWave) + wid, P.j1, lds_hist);
     hist_end<HIST>(lds_hist, P.hist);  // every wave of the block reaches the barrier
 }
 
@@ -1879,6 +2147,27 @@ static hipError_t scan_direct(int order, const DevData &D, const ScanParams &P, 
     case 2: hipLaunchKernelGGL((k_scan_direct<2, HIST>), grid, block, 0, s, D, P, st); break;
     case 3: hipLaunchKernelGGL((k_scan_direct<3, HIST>), grid, block, 0, s, D, P, st); break;
     case 4: hipLaunchKernelGGL((k_scan_direct<4, HIST>), grid, block, 0, s, D, P, st); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_scan_km(int order, const DevData &D, const ScanParams &P, TopState *st, hipStream_t s)
+{
+    const int64_t prefixes = P.j1 - P.j0;
+    if (prefixes <= 0) return hipSuccess;
+    const int64_t per_block = kBlock / kWave;
+    const dim3 grid((unsigned)((prefixes + per_block - 1) / per_block)), block(kBlock);
+    const bool hist = P.hist != nullptr;
+    switch (order) {
+    case 4:
+        if (hist) hipLaunchKernelGGL((k_scan_km<4, true>), grid, block, 0, s, D, P, st);
+        else hipLaunchKernelGGL((k_scan_km<4, false>), grid, block, 0, s, D, P, st);
+        break;
+    case 5:
+        if (hist) hipLaunchKernelGGL((k_scan_km<5, true>), grid, block, 0, s, D, P, st);
+        else hipLaunchKernelGGL((k_scan_km<5, false>), grid, block, 0, s, D, P, st);
+        break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

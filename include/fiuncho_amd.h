@@ -117,10 +117,14 @@ fiu_status fiu_dataset_destroy(fiu_dataset *ds);
 
 typedef enum {
     FIU_ALGO_AUTO = 0,     /* fastest exact path the dataset allows */
-    FIU_ALGO_DIRECT = 1,   /* AND+popcount of all 3^k cells (the reference's) */
-    FIU_ALGO_MARGINAL = 2, /* 2^k cells + inclusion-exclusion from exact
-                              lower-order counts; needs every SNP to cover the
-                              same samples (checked at upload) */
+    FIU_ALGO_DIRECT = 1,   /* AND+popcount of all 3^k cells (the reference's);
+                              orders 2-4 */
+    FIU_ALGO_MARGINAL = 2, /* orders 2-3: 2^k cells + inclusion-exclusion
+                              from exact lower-order counts; orders 4-5: the
+                              prefix's 3^(k-1) rows against genotypes 0 and 1
+                              of the last SNP, genotype 2 from the prefix
+                              rows' counts; needs every SNP to cover the same
+                              samples (checked at upload) */
     FIU_ALGO_MFMA = 3      /* orders 2 and 3: the 2^k marginal cells as
                               indicator products on the matrix cores (fp4
                               with fp32 accumulation, or int8 with int32;
@@ -162,7 +166,8 @@ typedef struct {
     int32_t variant;         /* its kernel variant (fiu_tune_entry) */
 } fiu_search_stats;
 
-/* Exhaustive order-k search over the combinations whose (k-1)-prefix belongs
+/* Exhaustive order-k search (k = 2..5; order 5 needs every SNP to cover the
+ * same samples) over the combinations whose (k-1)-prefix belongs
  * to Distribution<int>(snps, order-1, dist_step, dist_offset)
  * (Distribution.h:147-176), keeping the `outputs` best by mutual information.
  * Results are ordered by MI descending, ties by combination ascending;
@@ -242,7 +247,7 @@ fiu_status fiu_get_tuning(fiu_tune_entry *entries, size_t cap, size_t *n);
 /* ------------------------------------------------------------------------ */
 /* No counterpart in the reference, which runs one search per process
  * (MPIEngine.h:135-137).  Many searches over different datasets, sizes and
- * orders advance together through batched device steps: persistent launches
+ * orders (2-4) advance together through batched device steps: persistent launches
  * over a slice of every active search (popcount kinds, order 4, matrix-core
  * kinds for order 2 and whole order-3 scans), then one top-N merge per
  * search, replayed from a HIP graph.  Each search returns exactly what

@@ -1,0 +1,99 @@
+"""GPU: orders 4 and 5, marginal in the last SNP (k_scan_km).
+
+The prefix's 3^(k-1) genotype rows against genotypes 0 and 1 of the last
+SNP, genotype 2 from the prefix rows' own counts, the rows split over 3
+(order 4) or 9 (order 5) lanes per last SNP.  Bar: the same top-N
+(combinations, order and float MI bit for bit) as the oracle's FP64 ranking,
+as the direct all-cells kernel at order 4, and the reference's own top-90 at
+order 5 (tests/data, in test_gpu_parity)."""
+import math
+
+import numpy as np
+import pytest
+
+import fiuncho_amd as fa
+import oracle as O
+
+from .test_gpu_parity import synth
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("order,snps,cases,ctrls,outputs,seed", [
+    (4, 23, 333, 190, 100, 31),     # ragged classes, several c tiles of 21
+    (4, 40, 64, 64, 500, 32),       # whole words only
+    (4, 5, 50, 70, 5, 33),          # every combination of a tiny set
+    (4, 4, 20, 30, 1, 34),          # the single combination
+    (5, 15, 300, 260, 200, 35),     # tiles of 7 last SNPs
+    (5, 12, 1000, 900, 792, 36),    # every combination (C(12, 5) = 792), 8 + 8 quads
+    (5, 6, 40, 2000, 6, 37),        # 1 quad of cases, 16 of controls
+    (5, 5, 70, 60, 1, 38),          # the single combination
+])
+def test_marginal_orders_vs_oracle(order, snps, cases, ctrls, outputs, seed):
+    print(f"case: order {order} {snps}x{cases}+{ctrls}", flush=True)
+    d, od = synth(snps, cases, ctrls, seed)
+    oc, om, _ = O.search(od, order, outputs, threads=4, mi_mode=2)
+    s = fa.GpuSearch(algorithm=fa.ALGO_MARGINAL)
+    c, m = s.run_arrays(d, order, fa.Distribution(snps, order - 1), outputs)
+    assert s.last_stats.algorithm == fa.ALGO_MARGINAL
+    assert c.tolist() == oc.tolist() and np.array_equal(m, om)
+    assert s.last_stats.combinations == math.comb(snps, order)
+    auto = fa.GpuSearch()
+    ca, ma = auto.run_arrays(d, order, fa.Distribution(snps, order - 1), outputs)
+    assert auto.last_stats.algorithm == fa.ALGO_MARGINAL
+    assert ca.tolist() == oc.tolist() and np.array_equal(ma, om)
+    if order == 4:
+        cd, md = fa.GpuSearch(algorithm=fa.ALGO_DIRECT).run_arrays(d, 4, fa.Distribution(snps, 3), outputs)
+        assert cd.tolist() == oc.tolist() and np.array_equal(md, om)
+
+
+@pytest.mark.parametrize("order", [4, 5])
+def test_marginal_orders_shares_and_chunks(order):
+    snps = 30 if order == 4 else 18
+    planted = (3, 11, 20, 27) if order == 4 else (1, 6, 9, 13, 17)
+    print(f"case: order {order} shares, chunks", flush=True)
+    d, od = synth(snps, 700, 650, 90 + order, planted=planted)
+    full = fa.GpuSearch().run_arrays(d, order, fa.Distribution(snps, order - 1), 300)
+    oc, om, _ = O.search(od, order, 300, threads=4, mi_mode=2)
+    assert full[0].tolist() == oc.tolist() and np.array_equal(full[1], om)
+    # round-robin shares of the prefixes (MPI ranks) merge into the full list.
+    # A share is the distribution's own prefixes, as ThreadedSearch(1) scans
+    # them: with T threads the reference refines it to layer(T, t), another
+    # set for step > 1 (ThreadedSearch.h:219-239, Distribution.h:172-176)
+    merged = {}
+    for off in range(3):
+        dist = fa.Distribution(snps, order - 1).layer(3, off)
+        c, m = fa.GpuSearch().run_arrays(d, order, dist, 300)
+        sc, sm, _ = O.search(od, order, 300, step=3, offset=off, threads=1, mi_mode=2)
+        assert c.tolist() == sc.tolist() and np.array_equal(m, sm)
+        merged.update({tuple(x): v for x, v in zip(c.tolist(), m)})
+    best = sorted(merged.items(), key=lambda t: (-t[1], t[0]))[:300]
+    assert [c for c, _ in best] == [tuple(c) for c in full[0].tolist()]
+    # tiny chunks and a small candidate buffer: overflowing chunks are re-scanned
+    s = fa.GpuSearch(chunk_combos=500, candidate_cap=128, sample_threshold=False)
+    c, m = s.run_arrays(d, order, fa.Distribution(snps, order - 1), 60)
+    oc, om, _ = O.search(od, order, 60, threads=4, mi_mode=2)
+    assert c.tolist() == oc.tolist() and np.array_equal(m, om)
+    assert s.last_stats.redo_chunks > 0
+
+
+def test_incomplete_data():
+    # one uncalled genotype: order 4 falls back to the direct kernel, order 5
+    # (marginal only) is refused
+    rng = np.random.default_rng(7)
+    geno = rng.integers(0, 3, size=(12, 300)).astype(np.int8)
+    geno[4, 17] = 3
+    is_ctrl = (np.arange(300) >= 140).astype(np.uint8)
+    od = O.pack(geno, is_ctrl)
+    d = fa.Dataset.from_words(od.words, 12, od.cases_words, od.ctrls_words, 140, 160)
+    s = fa.GpuSearch()
+    c, m = s.run_arrays(d, 4, fa.Distribution(12, 3), 50)
+    assert s.last_stats.algorithm == fa.ALGO_DIRECT
+    oc, om, _ = O.search(od, 4, 50, threads=2, mi_mode=2)
+    assert c.tolist() == oc.tolist() and np.array_equal(m, om)
+    with pytest.raises(fa.FiuError):
+        fa.GpuSearch().run_arrays(d, 5, fa.Distribution(12, 4), 10)
+    with pytest.raises(fa.FiuError):
+        fa.GpuSearch(algorithm=fa.ALGO_DIRECT).run_arrays(d, 5, fa.Distribution(12, 4), 10)
+    with pytest.raises(fa.FiuError):
+        fa.GpuSearch().run_arrays(d, 6, fa.Distribution(12, 5), 10)

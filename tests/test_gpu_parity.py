@@ -81,8 +81,8 @@ def test_dataset_read_errors():
 
 
 def test_contingency_all_combinations(ds, tables):
-    print("case: contingency_batch all 375 combinations", flush=True)
-    for k in (2, 3, 4):
+    print("case: contingency_batch all 627 combinations", flush=True)
+    for k in (2, 3, 4, 5):
         combos = tables[f"k{k}_combos"]
         counts, mi = fa.contingency_batch(ds, combos)
         cells = 3 ** k
@@ -129,7 +129,7 @@ def test_mi_batch_known_answers():
 def test_search_testdata(ds, search_golden, tables, algorithm):
     # tests/threadedsearch.cpp:17-31 plus the reference's own top-90
     print(f"case: search tests/data algorithm={algorithm}", flush=True)
-    for o in (2, 3, 4):
+    for o in ((2, 3, 4, 5) if algorithm == fa.ALGO_AUTO else (2, 3, 4)):   # order 5: marginal only
         res = fa.GpuSearch(algorithm=algorithm).run(ds, o, fa.Distribution(10, o - 1), 90)
         combos = np.array([r.combination for r in res])
         mi = np.array([r.val for r in res], np.float32)
@@ -216,8 +216,10 @@ def test_search_vs_oracle_fp64(snps, cases, ctrls, order, outputs, seed):
         s = fa.GpuSearch(algorithm=algorithm)
         combos, mi = s.run_arrays(d, order, fa.Distribution(snps, order - 1), outputs)
         used = s.last_stats.algorithm
-        if algorithm == fa.ALGO_DIRECT or order > 3:
+        if algorithm == fa.ALGO_DIRECT:
             assert used == fa.ALGO_DIRECT
+        elif order > 3:
+            assert used == fa.ALGO_MARGINAL   # orders 4-5: marginal in the last SNP
         elif algorithm != fa.ALGO_MARGINAL:
             assert used == fa.ALGO_MFMA   # the default of whole order-2 and order-3 scans
         else:
@@ -305,10 +307,10 @@ def test_uncalled_sample_keeps_exact_mi():
     is_ctrl = (np.arange(300) >= 160).astype(np.uint8)
     od = O.pack(geno, is_ctrl)
     d = fa.Dataset.from_words(od.words, 40, od.cases_words, od.ctrls_words, 160, 140)
-    for order in (2, 3):
+    for order in (2, 3, 4, 5):
         print(f"case: uncalled sample k={order}", flush=True)
         oc, om, _ = O.search(od, order, 60, threads=2, mi_mode=2)
-        for algorithm in (fa.ALGO_AUTO, fa.ALGO_MARGINAL, fa.ALGO_DIRECT):
+        for algorithm in (fa.ALGO_AUTO, fa.ALGO_MARGINAL) + ((fa.ALGO_DIRECT,) if order < 5 else ()):
             combos, mi = fa.GpuSearch(algorithm=algorithm).run_arrays(d, order, fa.Distribution(40, order - 1), 60)
             assert combos.tolist() == oc.tolist() and np.array_equal(mi, om), (order, algorithm)
 
