@@ -221,6 +221,49 @@ def bench_config1(repeats=20):
             "combo_samples_per_s": 499500 * 4000 / med, "algorithm": s.last_stats.algorithm}
 
 
+def bench_high_orders():
+    """Orders 4 and 5 (the last-SNP-marginal popcount scan, k_scan_km) on
+    seeded HWE datasets of 16,000 samples (8,000 / 8,000), top-100; order 4
+    also with the all-cells direct kernel, the reference's algorithm."""
+    import fiuncho_amd as fa
+    out = {}
+    for order, snps in ((4, 400), (5, 100)):
+        spec = fa.synth_spec(snps, 8000, 8000, seed=5)
+        d = fa.Dataset.synth(spec)
+        dist = fa.Distribution(snps, order - 1)
+        res = {}
+        for name, algo in (("marginal", fa.ALGO_MARGINAL), ("direct", fa.ALGO_DIRECT)):
+            if order == 5 and algo == fa.ALGO_DIRECT:
+                continue
+            s = fa.GpuSearch(algorithm=algo)
+            s.run_arrays(d, order, fa.Distribution(snps, order - 1, 64, 63), 100)   # code and tables
+            t0 = time.perf_counter()
+            c, m = s.run_arrays(d, order, dist, 100)
+            wall = time.perf_counter() - t0
+            st = s.last_stats
+            cs = st.combinations * 16000 / (st.scan_ms / 1e3)
+            # the reference's AND + popcount ops (3^k cells x 2 per 32 samples)
+            ref_ops = cs * 2 * 3 ** order / 32
+            res[name] = {"wall_s": wall, "scan_ms": st.scan_ms, "combinations": st.combinations,
+                         "combo_samples_per_s": cs, "reference_algorithm_valu_frac": ref_ops / VALU_PEAK}
+            if name == "marginal":
+                # executed: 2 * 3^(k-1) cells x (AND + popcount) per 32 samples
+                res[name]["executed_valu_frac"] = cs * 2 * 2 * 3 ** (order - 1) / 32 / VALU_PEAK
+            res[name + "_result"] = (c, m)
+            print(f"[bench] order {order} {snps} x 16000 {name}: {st.scan_ms:.1f} ms scan, {cs:.3e} combo*samples/s",
+                  file=sys.stderr, flush=True)
+        if "direct" in res:
+            res["identical"] = bool(res["marginal_result"][0].tolist() == res["direct_result"][0].tolist() and
+                                    np.array_equal(res["marginal_result"][1], res["direct_result"][1]))
+            res["speedup"] = res["direct"]["scan_ms"] / res["marginal"]["scan_ms"]
+        res.pop("marginal_result", None)
+        res.pop("direct_result", None)
+        res["workload"] = f"order-{order} exhaustive scan, {snps} SNPs x 16,000 samples (8,000/8,000), top-100"
+        out[f"order{order}"] = res
+        d.close()
+    return out
+
+
 MFMA_I8_PEAK = 5.0e15    # dense int8 MFMA ops per second (MI355X)
 MFMA_FP4_PEAK = 10.1e15  # dense fp4 MFMA flops per second (MI355X)
 
@@ -428,6 +471,7 @@ def main():
     ap.add_argument("--no-order2", action="store_true", help="skip the order-2 popcount / matrix-core comparison")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (batch / serve use the defaults)")
     ap.add_argument("--no-ingest", action="store_true", help="skip the .raw read / device packing timing")
+    ap.add_argument("--no-high", action="store_true", help="skip the order-4 / order-5 scans")
     ap.add_argument("--batch-datasets", type=int, default=64)
     ap.add_argument("--serve-requests", type=int, default=300)
     ap.add_argument("--only-extra", action="store_true",
@@ -463,6 +507,8 @@ def main():
         if not args.no_order2:
             extra["config1"] = bench_config1()
             extra["order2"] = bench_order2()
+        if not args.no_high:
+            extra["high_orders"] = bench_high_orders()
         if not args.no_tune and not (args.no_batch and args.no_serve):
             extra["tuning"] = bench_tune()
         if not args.no_batch:
@@ -557,6 +603,8 @@ def main():
     if not args.no_order2:
         line["config1"] = bench_config1()
         line["order2"] = bench_order2()
+    if not args.no_high:
+        line["high_orders"] = bench_high_orders()
     if not args.no_tune and not (args.no_batch and args.no_serve):
         line["tuning"] = bench_tune()
     if not args.no_batch:

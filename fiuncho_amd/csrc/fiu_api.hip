@@ -725,11 +725,14 @@ void search(fiu_dataset *d, int order, int32_t step, int32_t offset, uint32_t ou
         }
     }
     const bool sample = !(cfg && (cfg->flags & FIU_FLAG_NO_THRESHOLD_SAMPLE));
+    // at most about 1/16 of the scan's combinations (the pass screens every
+    // combination it samples)
+    const uint64_t sample_cap = std::max<uint64_t>(65536, (uint64_t)((double)n_pref * per_avg / 16.0));
     const int64_t sample_prefixes =
         chunks.empty() || !sample
             ? 0
             : tail_prefixes(n, M, (uint64_t)step, (uint64_t)offset, total_prefixes, (int64_t)n_pref,
-                            std::min<uint64_t>(1ull << 24, target));
+                            std::min<uint64_t>({1ull << 24, target, sample_cap}));
     d->redo.alloc(std::max<size_t>(chunks.size(), 1));
     TopState init{};
     init.tau = -INFINITY;
