@@ -726,8 +726,10 @@ void search(fiu_dataset *d, int order, int32_t step, int32_t offset, uint32_t ou
     }
     const bool sample = !(cfg && (cfg->flags & FIU_FLAG_NO_THRESHOLD_SAMPLE));
     // at most about 1/16 of the scan's combinations (the pass screens every
-    // combination it samples)
-    const uint64_t sample_cap = std::max<uint64_t>(65536, (uint64_t)((double)n_pref * per_avg / 16.0));
+    // combination it samples with a popcount kernel), 1/256 ahead of a
+    // matrix-core scan, which costs 5-12x less per combination
+    const double sample_frac = algo == FIU_ALGO_MFMA ? 1.0 / 256.0 : 1.0 / 16.0;
+    const uint64_t sample_cap = std::max<uint64_t>(65536, (uint64_t)((double)n_pref * per_avg * sample_frac));
     const int64_t sample_prefixes =
         chunks.empty() || !sample
             ? 0

@@ -10,7 +10,7 @@ namespace fiu {
 constexpr int kWave = 64;            // CDNA wavefront width
 constexpr int kBlock = 256;          // threads per scan workgroup (4 waves)
 constexpr int kMaxOrder = 5;         // orders with a scan kernel
-constexpr int kMaxBatchOrder = 4;    // orders the batched engine schedules
+constexpr int kMaxBatchOrder = 5;    // orders the batched engine schedules
 constexpr int kMaxDirectOrder = 4;   // orders with an all-cells (direct) kernel
 constexpr int kMaxPrefix = 7;        // longest prefix the unranker handles
 constexpr int kMarginalPW = 4;       // prefixes per wavefront in marginal scans
@@ -106,6 +106,8 @@ enum BatchKind : int32_t {
     kKindDirect4 = 4,  // order 4, all cells (its own launch: register budget)
     kKindMfma2 = 5,    // order 2 on the matrix cores (fp4): units are 64 x 64 tiles of super-blocks
     kKindMfma3 = 6,    // order 3 on the matrix cores (fp4): units are (first SNP, tile) pairs
+    kKindKm4 = 7,      // order 4, marginal in the last SNP (launch 3)
+    kKindKm5 = 8,      // order 5, marginal in the last SNP (launch 3)
 };
 
 // fp4 image of a dataset's genotype rows 0 and 1 (operands of the matrix-core
@@ -146,14 +148,14 @@ struct BatchSeg {
 
 // Per-launch header in device memory (the launch itself is a fixed graph).
 struct BatchHeader {
-    uint32_t nseg[2];        // segments of the main launch, of the order-4 launch
-    uint32_t units[2];       // units of each launch
+    uint32_t nseg[3];        // segments of the main launch, the direct order-4
+                             // launch, the marginal order-4/5 launch
+    uint32_t units[3];       // units of each launch
     uint32_t nmerge;         // jobs to merge after the launches
-    uint32_t head[2];        // work counters (cleared by the graph)
+    uint32_t head[3];        // work counters (cleared by the graph)
     uint32_t mseg;           // segments of the matrix-core launch
     uint32_t munits;         // units of each XCD's queue in it
     uint32_t mhead[8];       // its per-XCD work counters (cleared by the graph)
-    uint32_t pad_[3];
 };
 
 __host__ __device__ inline uint64_t pair_index(int64_t b, int64_t c, int64_t n)

@@ -233,7 +233,7 @@ class Engine {
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t exec_ = nullptr;
     size_t max_segs_ = 0, max_msegs_ = 0, off_merge_ = 0, off_segs0_ = 0, off_segs1_ = 0, off_segs2_ = 0,
-           desc_bytes_ = 0;
+           off_segs3_ = 0, desc_bytes_ = 0;
     std::vector<Job *> slots_;
     std::vector<Job *> active_;
     std::mutex ready_mu_;
@@ -263,7 +263,8 @@ Engine::Engine(int device, const EngineConfig &cfg, Clock::time_point epoch) : d
     // matrix-core segments: order-3 ranges are cut at every 64th first SNP
     max_msegs_ = (size_t)A * 64 + 1024;
     off_segs2_ = off_segs1_ + max_segs_ * sizeof(BatchSeg);
-    desc_bytes_ = off_segs2_ + max_msegs_ * sizeof(BatchSeg);
+    off_segs3_ = off_segs2_ + max_msegs_ * sizeof(BatchSeg);
+    desc_bytes_ = off_segs3_ + max_segs_ * sizeof(BatchSeg);
     jobs_d_.alloc(A);
     states_d_.alloc(A);
     cand_d_.alloc((size_t)A * cfg_.cand_cap);
@@ -283,14 +284,15 @@ Engine::Engine(int device, const EngineConfig &cfg, Clock::time_point epoch) : d
         check(hipEventCreateWithFlags(&ev_done_[p], hipEventDisableTiming), "hipEventCreate");
     }
     slots_.assign(A, nullptr);
-    const int b0 = batch_blocks(0), b1 = batch_blocks(1), b2 = batch_blocks(2);
-    if (b0 <= 0 || b1 <= 0 || b2 <= 0) fail(FIU_ERR_DEVICE, "cannot size the persistent scan launches");
-    // the step graph: counter reset, the three persistent scans, the merges
+    const int b0 = batch_blocks(0), b1 = batch_blocks(1), b2 = batch_blocks(2), b3 = batch_blocks(3);
+    if (b0 <= 0 || b1 <= 0 || b2 <= 0 || b3 <= 0) fail(FIU_ERR_DEVICE, "cannot size the persistent scan launches");
+    // the step graph: counter reset, the four persistent scans, the merges
     char *d = desc_d_.p;
     check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "graph capture");
-    const hipError_t le = launch_batch(jobs_d_.p, (const BatchSeg *)(d + off_segs0_), (const BatchSeg *)(d + off_segs1_),
-                                       (const BatchSeg *)(d + off_segs2_), (const int32_t *)(d + off_merge_),
-                                       (BatchHeader *)d, b0, b1, b2, A, stream_);
+    const hipError_t le =
+        launch_batch(jobs_d_.p, (const BatchSeg *)(d + off_segs0_), (const BatchSeg *)(d + off_segs1_),
+                     (const BatchSeg *)(d + off_segs2_), (const BatchSeg *)(d + off_segs3_),
+                     (const int32_t *)(d + off_merge_), (BatchHeader *)d, b0, b1, b2, b3, A, stream_);
     const hipError_t ce = hipStreamEndCapture(stream_, &graph_);
     check(le, "batched step capture");
     check(ce, "graph capture end");
@@ -323,7 +325,7 @@ void Engine::plan_job(Job *j)
     const int order = j->order, M = order - 1;
     if (order < 2) fail(FIU_ERR_INVALID, "order must be >= 2");
     if (order > kMaxBatchOrder)
-        fail(FIU_ERR_UNSUPPORTED, "order " + std::to_string(order) + " is above the batched scans' limit (4)");
+        fail(FIU_ERR_UNSUPPORTED, "order " + std::to_string(order) + " is above the batched scans' limit (5)");
     if (j->step < 1) fail(FIU_ERR_INVALID, "distribution step must be >= 1");
     if (j->offset < 0) fail(FIU_ERR_INVALID, "distribution offset must be >= 0");
     if (j->outputs > (uint32_t)kMaxOutputs) fail(FIU_ERR_UNSUPPORTED, "outputs above " + std::to_string(kMaxOutputs));
@@ -360,19 +362,28 @@ void Engine::plan_job(Job *j)
         j->rs.algorithm = FIU_ALGO_MFMA;
         return;
     }
-    const bool marginal =
-        tuned_algo != FIU_ALGO_DIRECT && d->dev.uniform && (order == 2 || (order == 3 && d->pair_ready));
-    j->kind = marginal ? (order == 2 ? kKindScan2m : kKindScan3m)
+    if (order > kMaxDirectOrder && !d->dev.uniform)
+        fail(FIU_ERR_UNSUPPORTED, "order " + std::to_string(order) +
+                                      " is scanned marginally only: every SNP must cover the same samples");
+    const bool marginal = tuned_algo != FIU_ALGO_DIRECT && d->dev.uniform &&
+                          (order == 2 || (order == 3 && d->pair_ready) || order >= 4);
+    j->kind = marginal ? (order == 2 ? kKindScan2m : order == 3 ? kKindScan3m : order == 4 ? kKindKm4 : kKindKm5)
                        : (order == 2 ? kKindDirect2 : order == 3 ? kKindDirect3 : kKindDirect4);
     j->pw = batch_group(j->kind);
+    // cells counted per word: 2^k (marginal orders 2-3), 2 * 3^(k-1)
+    // (marginal orders 4-5), 3^k (direct)
     int cells = 1;
-    for (int i = 0; i < order; ++i) cells *= marginal ? 2 : 3;
+    for (int i = 0; i < order; ++i) cells *= marginal && order <= 3 ? 2 : 3;
+    if (marginal && order >= 4) cells = cells / 3 * 2;
     const double quads = (double)d->dev.quads[0] + (double)d->dev.quads[1];
-    const double epi = marginal ? (order == 3 ? 600.0 : 200.0) : 300.0;
+    const double epi = marginal ? (order == 3 ? 600.0 : order >= 4 ? 400.0 : 200.0) : 300.0;
     const double per_tile = (double)cells * 8.0 * quads + epi;   // wave instructions per prefix and tile
     const int n_tiles = std::max(1, d->dev.n_pad / 64);
     // about 40k wave instructions (tens of microseconds) per unit
     j->tiles = (int)std::clamp(std::lround(40000.0 / (j->pw * per_tile)), 1L, (long)n_tiles);
+    // marginal orders 4-5 count the prefix rows once per unit: at least 4
+    // tiles (256 last SNPs) per unit keep that pass a few per cent
+    if (marginal && order >= 4) j->tiles = std::min(n_tiles, std::max(j->tiles, 4));
     j->chunks = (n_tiles + j->tiles - 1) / j->tiles;
     const double per_pref = (double)(n - M) / (double)(M + 1);   // mean combinations per prefix
     j->instr_per_prefix = (per_pref / 64.0 + 0.5) * per_tile;
@@ -515,10 +526,11 @@ bool Engine::launch_step()
     char *desc = desc_h_[parity];
     BatchHeader *H = (BatchHeader *)desc;
     int32_t *merge = (int32_t *)(desc + off_merge_);
-    BatchSeg *segs[2] = {(BatchSeg *)(desc + off_segs0_), (BatchSeg *)(desc + off_segs1_)};
+    BatchSeg *segs[3] = {(BatchSeg *)(desc + off_segs0_), (BatchSeg *)(desc + off_segs1_),
+                         (BatchSeg *)(desc + off_segs3_)};
     BatchSeg *msegs = (BatchSeg *)(desc + off_segs2_);
-    uint32_t nseg[2] = {0, 0}, nmseg = 0;
-    uint64_t units[2] = {0, 0}, munits = 0;
+    uint32_t nseg[3] = {0, 0, 0}, nmseg = 0;
+    uint64_t units[3] = {0, 0, 0}, munits = 0;
     uint32_t nmerge = 0;
     // matrix-core ranges: per-XCD queue units (see k_batch_mfma); order-3
     // ranges are cut where the first SNP's tile changes (a constant tile
@@ -570,7 +582,7 @@ bool Engine::launch_step()
     };
     auto add_seg = [&](Job *j, int64_t a, int64_t b) {
         if (batch_mfma(j->kind)) return add_mseg(j, a, b);
-        const int L = j->kind == kKindDirect4 ? 1 : 0;
+        const int L = j->kind == kKindDirect4 ? 1 : (j->kind == kKindKm4 || j->kind == kKindKm5) ? 2 : 0;
         if (nseg[L] >= max_segs_) return false;
         const uint64_t u = (uint64_t)((b - a + j->pw - 1) / j->pw) * (uint64_t)j->chunks;
         if (units[L] + u > (1ull << 31)) return false;
@@ -619,10 +631,10 @@ bool Engine::launch_step()
     }
     if (R.jobs.empty()) return false;
     std::memset(H, 0, sizeof(BatchHeader));
-    H->nseg[0] = nseg[0];
-    H->nseg[1] = nseg[1];
-    H->units[0] = (uint32_t)units[0];
-    H->units[1] = (uint32_t)units[1];
+    for (int L = 0; L < 3; ++L) {
+        H->nseg[L] = nseg[L];
+        H->units[L] = (uint32_t)units[L];
+    }
     H->nmerge = nmerge;
     H->mseg = nmseg;
     H->munits = (uint32_t)munits;
@@ -638,6 +650,10 @@ bool Engine::launch_step()
         check(hipMemcpyAsync(desc_d_.p + off_segs2_, desc + off_segs2_, nmseg * sizeof(BatchSeg),
                              hipMemcpyHostToDevice, stream_),
               "step descriptor");
+    if (nseg[2])
+        check(hipMemcpyAsync(desc_d_.p + off_segs3_, desc + off_segs3_, nseg[2] * sizeof(BatchSeg),
+                             hipMemcpyHostToDevice, stream_),
+              "step descriptor");
     check(hipEventRecord(ev_start_[parity], stream_), "event");
     check(hipGraphLaunch(exec_, stream_), "step graph launch");
     check(hipEventRecord(ev_end_[parity], stream_), "event");
@@ -648,7 +664,7 @@ bool Engine::launch_step()
     R.valid = true;
     ++step_no_;
     ++stats_.steps;
-    stats_.units += units[0] + units[1] + 8 * munits;
+    stats_.units += units[0] + units[1] + units[2] + 8 * munits;
     return true;
 }
 
@@ -873,7 +889,7 @@ uint64_t Server::submit(const fiu_request *req)
     j->offset = req->dist_offset;
     j->outputs = req->outputs;
     if (req->order < 2) fail(FIU_ERR_INVALID, "order must be >= 2");
-    if (req->order > kMaxBatchOrder) fail(FIU_ERR_UNSUPPORTED, "order above the batched scans' limit (4)");
+    if (req->order > kMaxBatchOrder) fail(FIU_ERR_UNSUPPORTED, "order above the batched scans' limit (5)");
     if (req->dist_step < 1 || req->dist_offset < 0) fail(FIU_ERR_INVALID, "bad distribution");
     if (req->outputs > (uint32_t)kMaxOutputs) fail(FIU_ERR_UNSUPPORTED, "outputs above " + std::to_string(kMaxOutputs));
     if (req->dataset) {
@@ -1074,9 +1090,15 @@ void Server::info(fiu_server_info *out)
     out->device_ms = s.device_ms;
 }
 
-EngineConfig engine_config(const fiu_batch_config *cfg)
+// step_ms = 0 takes `default_step_ms`: 24 ms for a batch of known searches
+// (the drain at the end of each persistent launch costs about 0.8 ms, so
+// longer steps waste less; measured 1.55 s at 6 ms, 1.19 s at 24 ms on 64
+// uneven searches), 6 ms for the server, whose requests join at step
+// boundaries.
+EngineConfig engine_config(const fiu_batch_config *cfg, double default_step_ms)
 {
     EngineConfig ec;
+    ec.step_ms = default_step_ms;
     if (cfg) {
         if (cfg->struct_size != sizeof(fiu_batch_config)) fail(FIU_ERR_INVALID, "fiu_batch_config.struct_size mismatch");
         if (cfg->max_active) ec.max_active = (int)std::min<uint32_t>(cfg->max_active, 1024);
@@ -1100,7 +1122,7 @@ void fiu_batch_config_init(fiu_batch_config *cfg)
     std::memset(cfg, 0, sizeof(*cfg));
     cfg->struct_size = sizeof(fiu_batch_config);
     cfg->max_active = 128;
-    cfg->step_ms = 6.0;
+    cfg->step_ms = 0.0;   // the default of each use (engine_config)
     cfg->candidate_cap = 1u << 16;
 }
 
@@ -1109,7 +1131,7 @@ fiu_status fiu_search_batch(fiu_scan *scans, size_t n, const fiu_batch_config *c
     return guard([&] {
         if (!scans && n) fail(FIU_ERR_INVALID, "scans is NULL");
         const auto t0 = Clock::now();
-        const EngineConfig ec = engine_config(cfg);
+        const EngineConfig ec = engine_config(cfg, 24.0);
         std::vector<std::unique_ptr<Job>> jobs;
         int device = -1;
         for (size_t i = 0; i < n; ++i) {
@@ -1177,7 +1199,7 @@ fiu_status fiu_server_create(const fiu_batch_config *cfg, fiu_server **out)
 {
     return guard([&] {
         if (!out) fail(FIU_ERR_INVALID, "out is NULL");
-        const EngineConfig ec = engine_config(cfg);
+        const EngineConfig ec = engine_config(cfg, 6.0);
         auto s = std::make_unique<fiu_server>();
         s->impl = std::make_unique<Server>(current_device(), ec, cfg ? cfg->memory_budget : 0);
         *out = s.release();

@@ -57,12 +57,13 @@ hipError_t launch_mi_batch(const uint32_t *tables, uint64_t n, uint64_t ct_size,
                            hipStream_t s);
 
 // Batched scans: resident workgroups of the persistent scan kernel of launch
-// 0 (popcount orders 2-3), 1 (order 4) or 2 (matrix-core kinds, a multiple
-// of 8) on the current device, and the whole batched step: counter reset,
-// the three persistent scans, the per-job merges.
+// 0 (popcount orders 2-3), 1 (direct order 4), 2 (matrix-core kinds, a
+// multiple of 8) or 3 (marginal orders 4-5) on the current device, and the
+// whole batched step: counter reset, the four persistent scans, the per-job
+// merges.
 int batch_blocks(int launch);
 hipError_t launch_batch(const BatchJob *jobs, const BatchSeg *segs0, const BatchSeg *segs1, const BatchSeg *segs2,
-                        const int32_t *merge_list, BatchHeader *hdr, int blocks0, int blocks1, int blocks2,
-                        int merge_blocks, hipStream_t s);
+                        const BatchSeg *segs3, const int32_t *merge_list, BatchHeader *hdr, int blocks0, int blocks1,
+                        int blocks2, int blocks3, int merge_blocks, hipStream_t s);
 
 }  // namespace fiu

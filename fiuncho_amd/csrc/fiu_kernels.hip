@@ -636,9 +636,10 @@ __device__ __forceinline__ void km_count(const uint4 *__restrict__ base, int np,
     }
 }
 
+// Last SNPs c in tiles t_lo .. t_hi (64 SNPs each) only.
 template <int K, bool HIST>
 __device__ __forceinline__ void km_unit(const DevData &D, const ScanParams &P, TopState *st, int64_t j, int64_t j1,
-                                        uint32_t *lds_hist)
+                                        int t_hi, int t_lo, uint32_t *lds_hist)
 {
     constexpr int M = K - 1, LS = Km<K>::LS, L = Km<K>::L, CPW = Km<K>::CPW, NR = Km<K>::NR, LC = Km<K>::LC;
     int pre[M];
@@ -650,6 +651,8 @@ __device__ __forceinline__ void km_unit(const DevData &D, const ScanParams &P, T
     for (int i = 0; i < M; ++i) pre_packed = (pre_packed << P.combo_bits) | (uint64_t)pre[i];
     pre_packed <<= P.combo_bits;
     const int last = pre[M - 1], n = D.n, np = D.n_pad;
+    const int c_begin = max(last + 1, 64 * t_lo), c_end = min(n, 64 * min(t_hi, (np >> 6) - 1) + 64);
+    if (c_begin >= c_end) return;
     const int lane = threadIdx.x & 63, part = lane % L, first_lane = lane - part;
     const bool lane_ok = lane < CPW * L;
     int voff_f[LS];
@@ -675,9 +678,9 @@ __device__ __forceinline__ void km_unit(const DevData &D, const ScanParams &P, T
         for (int i = 1; i < L; ++i) t += __shfl(v, first_lane + i);
         return t;
     };
-    for (int c0 = last + 1; c0 < n; c0 += CPW) {
+    for (int c0 = c_begin; c0 < c_end; c0 += CPW) {
         const int c = c0 + lane / L;
-        const bool valid = lane_ok && c < n;
+        const bool valid = lane_ok && c < c_end;
         const int voff_c = 16 * min(c, np - 1);
         uint32_t k0[NR][2], k1[NR][2];
         km_count<K, 2>(b0, np, D.quads[0], pre, voff_f, voff_c, k0);
@@ -742,7 +745,7 @@ __global__ void __launch_bounds__(kBlock) k_scan_km(const DevData D, const ScanP
     __shared__ uint32_t lds_hist[HIST ? kHistBins : 1];
     hist_begin<HIST>(lds_hist);
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    km_unit<K, HIST>(D, P, st, P.j0 + (int64_t)blockIdx.x * (kBlock / kWave) + wid, P.j1, lds_hist);
+    km_unit<K, HIST>(D, P, st, P.j0 + (int64_t)blockIdx.x * (kBlock / kWave) + wid, P.j1, INT32_MAX, 0, lds_hist);
     hist_end<HIST>(lds_hist, P.hist);  // every wave of the block reaches the barrier
 }
 
@@ -1791,8 +1794,9 @@ k_merge(TopState *st, Cand *top, const Cand *cand, uint32_t N, uint32_t cap, uin
 // Persistent scan over the segments of one launch: every wavefront takes the
 // next unit from a counter until the units run out (every wave reaches the
 // exit, so the grid drains).  LAUNCH 0 runs the order-2/3 kinds in one
-// kernel, LAUNCH 1 the order-4 kind, whose register budget would otherwise
-// set the occupancy of the others.
+// kernel, LAUNCH 1 the direct order-4 kind and LAUNCH 2 the marginal
+// order-4/5 kinds, whose register budgets would otherwise set the occupancy
+// of the others.
 #if defined(__HIP_DEVICE_COMPILE__)
 typedef const __attribute__((address_space(4))) BatchJob ConstJob;
 #else
@@ -1800,7 +1804,7 @@ typedef const BatchJob ConstJob;  // host pass: kernel bodies are only parsed
 #endif
 
 template <int LAUNCH>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(LAUNCH == 0 ? 3 : 1)))
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(LAUNCH == 0 ? 3 : LAUNCH == 2 ? 2 : 1)))
 k_batch_scan(const BatchJob *__restrict__ jobs, const BatchSeg *__restrict__ segs, BatchHeader *hdr)
 {
     const int lane = threadIdx.x & 63;
@@ -1842,8 +1846,11 @@ k_batch_scan(const BatchJob *__restrict__ jobs, const BatchSeg *__restrict__ seg
             case kKindDirect3: direct_unit<3, false>(D, P, st, j0 + g, j1, t_hi, t_lo, nullptr); break;
             default: break;
             }
-        } else {
+        } else if constexpr (LAUNCH == 1) {
             direct_unit<4, false>(D, P, st, j0 + g, j1, t_hi, t_lo, nullptr);
+        } else {
+            if (J.kind == kKindKm4) km_unit<4, false>(D, P, st, j0 + g, j1, t_hi, t_lo, nullptr);
+            else km_unit<5, false>(D, P, st, j0 + g, j1, t_hi, t_lo, nullptr);
         }
     }
 }
@@ -1915,6 +1922,7 @@ __global__ void k_batch_reset(BatchHeader *hdr)
     if (threadIdx.x == 0) {
         hdr->head[0] = 0;
         hdr->head[1] = 0;
+        hdr->head[2] = 0;
     }
     if (threadIdx.x < 8) hdr->mhead[threadIdx.x] = 0;
 }
@@ -2218,6 +2226,7 @@ int batch_blocks(int launch)
     const hipError_t e =
         launch == 0   ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_scan<0>, kBlock, 0)
         : launch == 1 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_scan<1>, kBlock, 0)
+        : launch == 3 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_scan<2>, kBlock, 0)
                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_mfma, kBlock, 0);
     if (e != hipSuccess) return 0;
     const int blocks = std::max(1, per_cu) * cus;
@@ -2225,12 +2234,13 @@ int batch_blocks(int launch)
 }
 
 hipError_t launch_batch(const BatchJob *jobs, const BatchSeg *segs0, const BatchSeg *segs1, const BatchSeg *segs2,
-                        const int32_t *merge_list, BatchHeader *hdr, int blocks0, int blocks1, int blocks2,
-                        int merge_blocks, hipStream_t s)
+                        const BatchSeg *segs3, const int32_t *merge_list, BatchHeader *hdr, int blocks0, int blocks1,
+                        int blocks2, int blocks3, int merge_blocks, hipStream_t s)
 {
     hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, hdr);
     hipLaunchKernelGGL((k_batch_scan<0>), dim3(blocks0), dim3(kBlock), 0, s, jobs, segs0, hdr);
     if (blocks1 > 0) hipLaunchKernelGGL((k_batch_scan<1>), dim3(blocks1), dim3(kBlock), 0, s, jobs, segs1, hdr);
+    if (blocks3 > 0) hipLaunchKernelGGL((k_batch_scan<2>), dim3(blocks3), dim3(kBlock), 0, s, jobs, segs3, hdr);
     if (blocks2 > 0) hipLaunchKernelGGL(k_batch_mfma, dim3(blocks2), dim3(kBlock), 0, s, jobs, segs2, hdr);
     hipLaunchKernelGGL(k_batch_merge, dim3(merge_blocks), dim3(1024), 0, s, jobs, merge_list, hdr);
     return hipGetLastError();

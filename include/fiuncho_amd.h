@@ -260,7 +260,9 @@ typedef struct {
     uint64_t memory_budget;  /* server: device bytes for the tables it
                                 uploads, 0 = a quarter of the free
                                 memory, at most 8 GiB */
-    double step_ms;          /* target device time of one batched step */
+    double step_ms;          /* target device time of one batched step;
+                                0 = 24 ms for fiu_search_batch, 6 ms for
+                                the server (requests join between steps) */
     uint32_t candidate_cap;  /* candidate buffer entries per search */
     uint32_t flags;          /* reserved, 0 */
 } fiu_batch_config;

@@ -54,33 +54,35 @@ def mixed():
     sets = [
         (dataset(300, 700, 500, 101), 2, 50),        # order-2 marginal
         (dataset(160, 1500, 900, 102), 3, 40),       # order-3 marginal (pair table)
-        (dataset(45, 600, 650, 103), 4, 30),         # order-4 direct (second launch)
+        (dataset(45, 600, 650, 103), 4, 30),         # order-4 marginal (fourth launch)
+        (dataset(20, 400, 380, 107), 5, 30),         # order-5 marginal (fourth launch)
         (dataset(500, 130, 370, 104), 2, 100),       # ragged classes
         (dataset(90, 2100, 1900, 105, planted=(3, 50, 77)), 3, 20),
     ]
     d_inc, od_inc = incomplete(60, 300, 260, 106)
     sets.append(((d_inc), 3, 25))                   # order-3 direct (incomplete genotypes)
     sets.append((d_inc, 2, 25))                     # order-2 direct
+    sets.append((d_inc, 4, 25))                     # order-4 direct (second launch)
     return sets, od_inc
 
 
 def test_batch_matches_plain(mixed):
     sets, od_inc = mixed
     scans = [(d, k, fa.Distribution(d.snps, k - 1), n) for d, k, n in sets]
-    print("case: batch of 7 mixed searches", flush=True)
+    print("case: batch of 9 mixed searches", flush=True)
     res, st = fa.search_batch(scans)
     assert st.steps > 0 and st.device_ms > 0
     for (d, k, n), r in zip(sets, res):
         assert same(r, plain(d, k, fa.Distribution(d.snps, k - 1), n)), (d.snps, k)
     # the incomplete-genotype searches against the oracle as well
-    for k, r in ((3, res[5]), (2, res[6])):
+    for k, r in ((3, res[6]), (2, res[7]), (4, res[8])):
         oc, om, _ = O.search(od_inc, k, 25, threads=2, mi_mode=2)
         assert r[0].tolist() == oc.tolist() and np.array_equal(r[1], om)
 
 
 def test_batch_planted_is_first(mixed):
     sets, _ = mixed
-    d, k, n = sets[4]
+    d, k, n = sets[5]
     res, _ = fa.search_batch([(d, k, fa.Distribution(d.snps, k - 1), n)])
     assert res[0][0][0].tolist() == [3, 50, 77]
 
@@ -117,7 +119,8 @@ def test_batch_candidate_overflow_rescans(mixed):
 def test_batch_small_cases_vs_oracle():
     print("case: batch small cases vs oracle", flush=True)
     cases = []
-    for i, (snps, ca, ct, k, n) in enumerate([(40, 90, 70, 2, 30), (25, 64, 200, 3, 40), (14, 100, 33, 4, 1001)]):
+    for i, (snps, ca, ct, k, n) in enumerate([(40, 90, 70, 2, 30), (25, 64, 200, 3, 40), (14, 100, 33, 4, 1001),
+                                              (12, 120, 90, 5, 800)]):
         words, geno, cw, tw = synth(snps, ca, ct, 200 + i, geno=True)
         d = fa.Dataset.from_words(words, snps, cw, tw, ca, ct)
         od = O.pack(geno, np.r_[np.zeros(ca, np.uint8), np.ones(ct, np.uint8)])
@@ -140,7 +143,10 @@ def test_batch_edge_cases():
     assert [len(r[1]) for r in res] == [0, 0, 0, math.comb(12, 2)]
     assert same(res[3], plain(d, 2, fa.Distribution(12, 1), 1000))
     with pytest.raises(fa.FiuError):
-        fa.search_batch([(d, 5, fa.Distribution(12, 4), 10)])
+        fa.search_batch([(d, 6, fa.Distribution(12, 5), 10)])   # above order 5
+    d_inc, _ = incomplete(12, 60, 60, 43)
+    with pytest.raises(fa.FiuError):
+        fa.search_batch([(d_inc, 5, fa.Distribution(12, 4), 10)])   # order 5 needs complete genotypes
     assert fa.search_batch([])[0] == []
 
 
@@ -150,7 +156,7 @@ def test_server_alone_and_in_the_mix():
     print("case: server alone vs mixed", flush=True)
     reqs = []
     for i, (snps, ca, ct, k, n) in enumerate([(400, 900, 1100, 2, 50), (150, 2000, 1000, 3, 30),
-                                              (40, 500, 500, 4, 20), (700, 300, 300, 2, 80),
+                                              (40, 500, 500, 4, 20), (700, 300, 300, 2, 80), (16, 700, 600, 5, 25),
                                               (120, 800, 1600, 3, 60), (64, 4000, 4000, 3, 10)]):
         words, _, cw, tw = synth(snps, ca, ct, 300 + i)
         reqs.append(dict(order=k, outputs=n, words=words, snps=snps, cases_words=cw, ctrls_words=tw,
