@@ -247,8 +247,8 @@ fiu_status fiu_get_tuning(fiu_tune_entry *entries, size_t cap, size_t *n);
 /* ------------------------------------------------------------------------ */
 /* No counterpart in the reference, which runs one search per process
  * (MPIEngine.h:135-137).  Many searches over different datasets, sizes and
- * orders (2-4) advance together through batched device steps: persistent launches
- * over a slice of every active search (popcount kinds, order 4, matrix-core
+ * orders (2-5) advance together through batched device steps: persistent launches
+ * over a slice of every active search (popcount kinds, orders 4-5, matrix-core
  * kinds for order 2 and whole order-3 scans), then one top-N merge per
  * search, replayed from a HIP graph.  Each search returns exactly what
  * fiu_search returns for it alone. */
