@@ -1695,14 +1695,147 @@ k_mfma3(const DevData D, const ScanParams P, TopState *st, const Fp4Image F, int
 // ---------------------------------------------------------------------------
 // Top-N merge (one workgroup): current list + candidates -> best N, unique.
 // ---------------------------------------------------------------------------
+// Sorts buf[0, L) (L a power of two, padded with empty entries) by key,
+// then writes the first N unique non-empty entries to top; returns how many
+// unique entries there were (all threads).
+__device__ __forceinline__ uint32_t sort_unique(Cand *buf, uint32_t L, Cand *top, uint32_t N, uint32_t *psum)
+{
+    const int tid = threadIdx.x;
+    for (uint32_t k = 2; k <= L; k <<= 1) {
+        for (uint32_t jj = k >> 1; jj > 0; jj >>= 1) {
+            for (uint32_t i = tid; i < L; i += 1024) {
+                const uint32_t ixj = i ^ jj;
+                if (ixj > i) {
+                    const Cand a = buf[i], b = buf[ixj];
+                    const bool desc = (i & k) == 0;
+                    const bool sw = desc ? better(b.mi, b.combo, a.mi, a.combo) : better(a.mi, a.combo, b.mi, b.combo);
+                    if (sw) { buf[i] = b; buf[ixj] = a; }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    // unique, non-empty entries in order; contiguous segments per thread
+    const uint32_t seg = (L + 1023) / 1024;
+    const uint32_t s0 = min(L, tid * seg), s1 = min(L, s0 + seg);
+    auto keep = [&](uint32_t i) {
+        const bool empty = buf[i].combo == ~0ull && buf[i].mi == -INFINITY;
+        const bool dup = i > 0 && buf[i].combo == buf[i - 1].combo && buf[i].mi == buf[i - 1].mi &&
+                         buf[i].tag == buf[i - 1].tag;
+        return !empty && !dup;
+    };
+    uint32_t local = 0;
+    for (uint32_t i = s0; i < s1; ++i) local += keep(i);
+    psum[tid] = local;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const uint32_t v = tid >= off ? psum[tid - off] : 0;
+        __syncthreads();
+        psum[tid] += v;
+        __syncthreads();
+    }
+    uint32_t pos = psum[tid] - local;
+    for (uint32_t i = s0; i < s1; ++i) {
+        if (keep(i)) {
+            if (pos < N) top[pos] = buf[i];
+            ++pos;
+        }
+    }
+    const uint32_t unique = psum[1023];
+    __syncthreads();
+    return unique;
+}
+
 // Body shared by the single-scan and the batched merge (1024 threads).
+// Many candidates (a first slice scanned with no threshold pushes every
+// combination) are filtered first: a histogram of MI (hist_bin) over the
+// current list and the candidates gives the highest bin b with at least 2N
+// entries at or above it.  At most N entries are duplicates of others (a
+// re-scanned range repeats entries of the list, at most one copy each), so
+// at least N unique entries lie in bins >= b and every entry below b ranks
+// after them: only the entries in bins >= b are sorted.
 __device__ __forceinline__ void merge_topn(TopState *st, Cand *top, const Cand *cand, uint32_t N, uint32_t cap,
                                            Cand *buf, uint32_t *psum)
 {
     const int tid = threadIdx.x;
     const uint32_t count = min(st->cand_count, cap);
     uint32_t n_top = st->n_top;
-    for (uint32_t done = 0; done < count;) {
+    bool merged = false;
+    if (count > (uint32_t)kMergeSlots / 2 && N > 0) {
+        static_assert(kHistBins * sizeof(uint32_t) <= kMergeSlots * sizeof(Cand), "histogram in the merge buffer");
+        static_assert(kHistBins == 4 * 1024, "four bins per thread");
+        uint32_t *hist = (uint32_t *)buf;
+        for (int b = tid; b < kHistBins; b += 1024) hist[b] = 0;
+        __syncthreads();
+        for (uint32_t i = tid; i < count; i += 1024) atomicAdd(&hist[hist_bin(cand[i].mi)], 1u);
+        for (uint32_t i = tid; i < n_top; i += 1024) atomicAdd(&hist[hist_bin(top[i].mi)], 1u);
+        __syncthreads();
+        // suffix sums over the threads' four-bin groups (thread t: bins 4t .. 4t + 3)
+        uint32_t grp[4], local = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) local += grp[q] = hist[4 * tid + q];
+        psum[tid] = local;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+            const uint32_t v = tid + off < 1024 ? psum[tid + off] : 0;
+            __syncthreads();
+            psum[tid] += v;
+            __syncthreads();
+        }
+        const uint32_t need = 2 * N;
+        const uint32_t above = tid + 1 < 1024 ? psum[tid + 1] : 0;   // entries in the bins of higher threads
+        int sel = -1;                                                   // this thread's pick, if it holds b
+        if (psum[tid] >= need && above < need) {
+            uint32_t acc = above;
+            for (int q = 3; q >= 0 && sel < 0; --q) {
+                acc += grp[q];
+                if (acc >= need) sel = 4 * tid + q;
+            }
+        }
+        __syncthreads();
+        if (tid == 0) psum[0] = 0;   // no pick: keep every bin
+        __syncthreads();
+        if (sel >= 0) psum[0] = (uint32_t)sel;
+        __syncthreads();
+        const uint32_t bmin = psum[0];
+        __syncthreads();
+        // compact the entries in bins >= bmin into buf (order is irrelevant:
+        // they are sorted next); counts first, so that the total is known
+        uint32_t mine = 0;
+        for (uint32_t i = tid; i < count; i += 1024) mine += hist_bin(cand[i].mi) >= bmin;
+        for (uint32_t i = tid; i < n_top; i += 1024) mine += hist_bin(top[i].mi) >= bmin;
+        psum[tid] = mine;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+            const uint32_t v = tid >= off ? psum[tid - off] : 0;
+            __syncthreads();
+            psum[tid] += v;
+            __syncthreads();
+        }
+        const uint32_t total = psum[1023];
+        if (total <= (uint32_t)kMergeSlots) {
+            uint32_t pos = psum[tid] - mine;
+            __syncthreads();   // the histogram (the start of buf) is read no more
+            for (uint32_t i = tid; i < n_top; i += 1024)
+                if (hist_bin(top[i].mi) >= bmin) buf[pos++] = top[i];
+            for (uint32_t i = tid; i < count; i += 1024)
+                if (hist_bin(cand[i].mi) >= bmin) buf[pos++] = cand[i];
+            uint32_t L = 2;
+            while (L < total) L <<= 1;
+            for (uint32_t i = total + tid; i < L; i += 1024) {
+                Cand e;
+                e.mi = -INFINITY;
+                e.tag = 0;
+                e.combo = ~0ull;
+                buf[i] = e;
+            }
+            __syncthreads();
+            n_top = min(N, sort_unique(buf, L, top, N, psum));
+            merged = true;
+        }
+        __syncthreads();
+    }
+    for (uint32_t done = 0; !merged && done < count;) {
         const uint32_t take = min(count - done, (uint32_t)kMergeSlots - n_top);
         const uint32_t total = n_top + take;
         uint32_t L = 2;
@@ -1715,51 +1848,7 @@ __device__ __forceinline__ void merge_topn(TopState *st, Cand *top, const Cand *
             buf[i] = e;
         }
         __syncthreads();
-        for (uint32_t k = 2; k <= L; k <<= 1) {
-            for (uint32_t jj = k >> 1; jj > 0; jj >>= 1) {
-                for (uint32_t i = tid; i < L; i += 1024) {
-                    const uint32_t ixj = i ^ jj;
-                    if (ixj > i) {
-                        const Cand a = buf[i], b = buf[ixj];
-                        const bool desc = (i & k) == 0;
-                        const bool sw = desc ? better(b.mi, b.combo, a.mi, a.combo)
-                                             : better(a.mi, a.combo, b.mi, b.combo);
-                        if (sw) { buf[i] = b; buf[ixj] = a; }
-                    }
-                }
-                __syncthreads();
-            }
-        }
-        // unique, non-empty entries in order; contiguous segments per thread
-        const uint32_t seg = (L + 1023) / 1024;
-        const uint32_t s0 = min(L, tid * seg), s1 = min(L, s0 + seg);
-        uint32_t local = 0;
-        for (uint32_t i = s0; i < s1; ++i) {
-            const bool empty = buf[i].combo == ~0ull && buf[i].mi == -INFINITY;
-            const bool dup = i > 0 && buf[i].combo == buf[i - 1].combo && buf[i].mi == buf[i - 1].mi &&
-                             buf[i].tag == buf[i - 1].tag;
-            local += (!empty && !dup);
-        }
-        psum[tid] = local;
-        __syncthreads();
-        for (int off = 1; off < 1024; off <<= 1) {
-            const uint32_t v = tid >= off ? psum[tid - off] : 0;
-            __syncthreads();
-            psum[tid] += v;
-            __syncthreads();
-        }
-        uint32_t pos = psum[tid] - local;
-        for (uint32_t i = s0; i < s1; ++i) {
-            const bool empty = buf[i].combo == ~0ull && buf[i].mi == -INFINITY;
-            const bool dup = i > 0 && buf[i].combo == buf[i - 1].combo && buf[i].mi == buf[i - 1].mi &&
-                             buf[i].tag == buf[i - 1].tag;
-            if (!empty && !dup) {
-                if (pos < N) top[pos] = buf[i];
-                ++pos;
-            }
-        }
-        n_top = min(N, psum[1023]);
-        __syncthreads();
+        n_top = min(N, sort_unique(buf, L, top, N, psum));
         done += take;
     }
     __syncthreads();

@@ -356,3 +356,22 @@ def test_sampled_threshold_with_planted_signal(order, where):
             c, m = fa.GpuSearch(algorithm=algorithm, chunk_combos=chunk).run_arrays(
                 d, order, fa.Distribution(n, order - 1), 30)
             assert c.tolist() == oc.tolist() and np.array_equal(m, om), (algorithm, chunk)
+
+
+@pytest.mark.parametrize("order,snps,outputs,cap", [
+    (2, 400, 100, 1 << 17),    # 79,800 candidates in one merge: histogram filter, then one sort
+    (2, 400, 3000, 1 << 17),   # the filter keeps 2N = 6,000 entries
+    (3, 60, 50, 1 << 16),      # 34,220 candidates
+    (2, 300, 64, 6000),        # 6,000 candidates per merge, overflowing chunks re-scanned
+])
+def test_merge_of_many_candidates(order, snps, outputs, cap):
+    # with no sampled starting threshold the first chunk pushes every
+    # combination; the merge must still return exactly the oracle's top-N
+    print(f"case: many candidates k={order} n={snps} N={outputs} cap={cap}", flush=True)
+    d, od = synth(snps, 400, 380, 500 + snps)
+    oc, om, _ = O.search(od, order, outputs, threads=4, mi_mode=2)
+    s = fa.GpuSearch(sample_threshold=False, candidate_cap=cap, chunk_combos=1 << 20)
+    c, m = s.run_arrays(d, order, fa.Distribution(snps, order - 1), outputs)
+    assert c.tolist() == oc.tolist() and np.array_equal(m, om)
+    if cap < 10000:
+        assert s.last_stats.redo_chunks > 0
