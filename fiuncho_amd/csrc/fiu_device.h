@@ -9,7 +9,7 @@ namespace fiu {
 
 constexpr int kWave = 64;            // CDNA wavefront width
 constexpr int kBlock = 256;          // threads per scan workgroup (4 waves)
-constexpr int kMaxOrder = 5;         // orders with a scan kernel
+constexpr int kMaxOrder = 6;         // orders with a scan kernel
 constexpr int kMaxBatchOrder = 5;    // orders the batched engine schedules
 constexpr int kMaxDirectOrder = 4;   // orders with an all-cells (direct) kernel
 constexpr int kMaxPrefix = 7;        // longest prefix the unranker handles

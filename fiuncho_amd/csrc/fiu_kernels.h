@@ -5,7 +5,7 @@
 
 namespace fiu {
 
-// Orders 4 and 5, marginal in the last SNP (uniform datasets); P.hist set:
+// Orders 4 to 6, marginal in the last SNP (uniform datasets); P.hist set:
 // the threshold sampling pass.
 hipError_t launch_scan_km(int order, const DevData &D, const ScanParams &P, TopState *st, hipStream_t s);
 hipError_t launch_scan_direct(int order, const DevData &D, const ScanParams &P, TopState *st, hipStream_t s);

@@ -535,7 +535,7 @@ k_scan_direct(const DevData D, const ScanParams P, TopState *st)
 }
 
 // ---------------------------------------------------------------------------
-// Orders 4 and 5, marginal in the last SNP (FIU_ALGO_MARGINAL).
+// Orders 4 to 6, marginal in the last SNP (FIU_ALGO_MARGINAL).
 // ---------------------------------------------------------------------------
 // A wavefront takes one prefix (s_0 < ... < s_{k-2}) of the distribution and
 // the last SNPs c after it.  The prefix's 3^(k-1) genotype rows (ANDs of one
@@ -552,9 +552,10 @@ k_scan_direct(const DevData D, const ScanParams P, TopState *st)
 // (ContingencyTable.h:67-90): the L lanes of a c hold consecutive cell
 // ranges, so the bound and the screen are sums over them and the exact MI is
 // one chain through them in cell order.  Lanes: order 4, 21 c x 3 parts;
-// order 5, 7 c x 9 parts (lane 63 idles).
+// order 5, 7 c x 9 parts (lane 63 idles); order 6, 2 c x 27 parts (ten
+// lanes idle).
 template <int K> struct Km {
-    static_assert(K == 4 || K == 5, "orders 4 and 5");
+    static_assert(K >= 4 && K <= 6, "orders 4 to 6");
     static constexpr int LS = K - 3;            // prefix SNPs split over lanes
     static constexpr int L = Pow3<K - 3>::v;     // lanes per c
     static constexpr int CPW = 64 / L;           // c per wavefront
@@ -2264,6 +2265,10 @@ hipError_t launch_scan_km(int order, const DevData &D, const ScanParams &P, TopS
     case 5:
         if (hist) hipLaunchKernelGGL((k_scan_km<5, true>), grid, block, 0, s, D, P, st);
         else hipLaunchKernelGGL((k_scan_km<5, false>), grid, block, 0, s, D, P, st);
+        break;
+    case 6:
+        if (hist) hipLaunchKernelGGL((k_scan_km<6, true>), grid, block, 0, s, D, P, st);
+        else hipLaunchKernelGGL((k_scan_km<6, false>), grid, block, 0, s, D, P, st);
         break;
     default: return hipErrorInvalidValue;
     }

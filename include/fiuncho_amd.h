@@ -120,7 +120,7 @@ typedef enum {
     FIU_ALGO_DIRECT = 1,   /* AND+popcount of all 3^k cells (the reference's);
                               orders 2-4 */
     FIU_ALGO_MARGINAL = 2, /* orders 2-3: 2^k cells + inclusion-exclusion
-                              from exact lower-order counts; orders 4-5: the
+                              from exact lower-order counts; orders 4-6: the
                               prefix's 3^(k-1) rows against genotypes 0 and 1
                               of the last SNP, genotype 2 from the prefix
                               rows' counts; needs every SNP to cover the same
@@ -166,7 +166,7 @@ typedef struct {
     int32_t variant;         /* its kernel variant (fiu_tune_entry) */
 } fiu_search_stats;
 
-/* Exhaustive order-k search (k = 2..5; order 5 needs every SNP to cover the
+/* Exhaustive order-k search (k = 2..6; orders 5-6 need every SNP to cover the
  * same samples) over the combinations whose (k-1)-prefix belongs
  * to Distribution<int>(snps, order-1, dist_step, dist_offset)
  * (Distribution.h:147-176), keeping the `outputs` best by mutual information.

@@ -117,7 +117,7 @@ def test_tuning_table_roundtrip_and_validation():
                E(3, 0, 10, 0, 10, fa.ALGO_MFMA, 5),         # no such staging variant
                E(4, 0, 10, 0, 10, fa.ALGO_MFMA, 0),         # matrix cores: orders 2-3
                E(5, 0, 10, 0, 10, fa.ALGO_DIRECT, 0),       # all-cells kernel: orders 2-4
-               E(6, 0, 10, 0, 10, fa.ALGO_MARGINAL, 0),     # above order 5
+               E(7, 0, 10, 0, 10, fa.ALGO_MARGINAL, 0),     # above order 6
                E(4, 0, 10, 0, 10, fa.ALGO_MARGINAL, 4),     # prefixes per wave: orders 2-3
                E(2, 10, 10, 0, 10, fa.ALGO_DIRECT, 0),      # empty SNP range
                E(2, 0, 10, 0, 10, fa.ALGO_AUTO, 0),         # AUTO is not a choice

@@ -1,11 +1,11 @@
-"""GPU: orders 4 and 5, marginal in the last SNP (k_scan_km).
+"""GPU: orders 4 to 6, marginal in the last SNP (k_scan_km).
 
 The prefix's 3^(k-1) genotype rows against genotypes 0 and 1 of the last
 SNP, genotype 2 from the prefix rows' own counts, the rows split over 3
-(order 4) or 9 (order 5) lanes per last SNP.  Bar: the same top-N
-(combinations, order and float MI bit for bit) as the oracle's FP64 ranking,
-as the direct all-cells kernel at order 4, and the reference's own top-90 at
-order 5 (tests/data, in test_gpu_parity)."""
+(order 4), 9 (order 5) or 27 (order 6) lanes per last SNP.  Bar: the same
+top-N (combinations, order and float MI bit for bit) as the oracle's FP64
+ranking, as the direct all-cells kernel at order 4, and the reference's own
+top-90 at orders 5 and 6 (tests/data, in test_gpu_parity)."""
 import math
 
 import numpy as np
@@ -28,6 +28,9 @@ pytestmark = pytest.mark.gpu
     (5, 12, 1000, 900, 792, 36),    # every combination (C(12, 5) = 792), 8 + 8 quads
     (5, 6, 40, 2000, 6, 37),        # 1 quad of cases, 16 of controls
     (5, 5, 70, 60, 1, 38),          # the single combination
+    (6, 11, 300, 260, 200, 39),     # tiles of 2 last SNPs
+    (6, 9, 1000, 900, 84, 40),      # every combination (C(9, 6) = 84)
+    (6, 6, 50, 70, 1, 41),          # the single combination
 ])
 def test_marginal_orders_vs_oracle(order, snps, cases, ctrls, outputs, seed):
     print(f"case: order {order} {snps}x{cases}+{ctrls}", flush=True)
@@ -47,10 +50,10 @@ def test_marginal_orders_vs_oracle(order, snps, cases, ctrls, outputs, seed):
         assert cd.tolist() == oc.tolist() and np.array_equal(md, om)
 
 
-@pytest.mark.parametrize("order", [4, 5])
+@pytest.mark.parametrize("order", [4, 5, 6])
 def test_marginal_orders_shares_and_chunks(order):
-    snps = 30 if order == 4 else 18
-    planted = (3, 11, 20, 27) if order == 4 else (1, 6, 9, 13, 17)
+    snps = {4: 30, 5: 18, 6: 13}[order]
+    planted = {4: (3, 11, 20, 27), 5: (1, 6, 9, 13, 17), 6: (0, 2, 5, 7, 9, 12)}[order]
     print(f"case: order {order} shares, chunks", flush=True)
     d, od = synth(snps, 700, 650, 90 + order, planted=planted)
     full = fa.GpuSearch().run_arrays(d, order, fa.Distribution(snps, order - 1), 300)
@@ -78,8 +81,8 @@ def test_marginal_orders_shares_and_chunks(order):
 
 
 def test_incomplete_data():
-    # one uncalled genotype: order 4 falls back to the direct kernel, order 5
-    # (marginal only) is refused
+    # one uncalled genotype: order 4 falls back to the direct kernel, orders
+    # 5 and 6 (marginal only) are refused
     rng = np.random.default_rng(7)
     geno = rng.integers(0, 3, size=(12, 300)).astype(np.int8)
     geno[4, 17] = 3
@@ -97,3 +100,5 @@ def test_incomplete_data():
         fa.GpuSearch(algorithm=fa.ALGO_DIRECT).run_arrays(d, 5, fa.Distribution(12, 4), 10)
     with pytest.raises(fa.FiuError):
         fa.GpuSearch().run_arrays(d, 6, fa.Distribution(12, 5), 10)
+    with pytest.raises(fa.FiuError):
+        fa.GpuSearch().run_arrays(d, 7, fa.Distribution(12, 6), 10)   # above order 6

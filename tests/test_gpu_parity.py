@@ -81,8 +81,8 @@ def test_dataset_read_errors():
 
 
 def test_contingency_all_combinations(ds, tables):
-    print("case: contingency_batch all 627 combinations", flush=True)
-    for k in (2, 3, 4, 5):
+    print("case: contingency_batch all 837 combinations", flush=True)
+    for k in (2, 3, 4, 5, 6):
         combos = tables[f"k{k}_combos"]
         counts, mi = fa.contingency_batch(ds, combos)
         cells = 3 ** k
@@ -129,7 +129,7 @@ def test_mi_batch_known_answers():
 def test_search_testdata(ds, search_golden, tables, algorithm):
     # tests/threadedsearch.cpp:17-31 plus the reference's own top-90
     print(f"case: search tests/data algorithm={algorithm}", flush=True)
-    for o in ((2, 3, 4, 5) if algorithm == fa.ALGO_AUTO else (2, 3, 4)):   # order 5: marginal only
+    for o in ((2, 3, 4, 5, 6) if algorithm == fa.ALGO_AUTO else (2, 3, 4)):   # orders 5-6: marginal only
         res = fa.GpuSearch(algorithm=algorithm).run(ds, o, fa.Distribution(10, o - 1), 90)
         combos = np.array([r.combination for r in res])
         mi = np.array([r.val for r in res], np.float32)
@@ -307,7 +307,7 @@ def test_uncalled_sample_keeps_exact_mi():
     is_ctrl = (np.arange(300) >= 160).astype(np.uint8)
     od = O.pack(geno, is_ctrl)
     d = fa.Dataset.from_words(od.words, 40, od.cases_words, od.ctrls_words, 160, 140)
-    for order in (2, 3, 4, 5):
+    for order in (2, 3, 4, 5, 6):
         print(f"case: uncalled sample k={order}", flush=True)
         oc, om, _ = O.search(od, order, 60, threads=2, mi_mode=2)
         for algorithm in (fa.ALGO_AUTO, fa.ALGO_MARGINAL) + ((fa.ALGO_DIRECT,) if order < 5 else ()):
