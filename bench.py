@@ -222,18 +222,18 @@ def bench_config1(repeats=20):
 
 
 def bench_high_orders():
-    """Orders 4 and 5 (the last-SNP-marginal popcount scan, k_scan_km) on
+    """Orders 4 to 6 (the last-SNP-marginal popcount scan, k_scan_km) on
     seeded HWE datasets of 16,000 samples (8,000 / 8,000), top-100; order 4
     also with the all-cells direct kernel, the reference's algorithm."""
     import fiuncho_amd as fa
     out = {}
-    for order, snps in ((4, 400), (5, 100)):
+    for order, snps in ((4, 400), (5, 100), (6, 40)):
         spec = fa.synth_spec(snps, 8000, 8000, seed=5)
         d = fa.Dataset.synth(spec)
         dist = fa.Distribution(snps, order - 1)
         res = {}
         for name, algo in (("marginal", fa.ALGO_MARGINAL), ("direct", fa.ALGO_DIRECT)):
-            if order == 5 and algo == fa.ALGO_DIRECT:
+            if order >= 5 and algo == fa.ALGO_DIRECT:
                 continue
             s = fa.GpuSearch(algorithm=algo)
             s.run_arrays(d, order, fa.Distribution(snps, order - 1, 64, 63), 100)   # code and tables
@@ -471,7 +471,7 @@ def main():
     ap.add_argument("--no-order2", action="store_true", help="skip the order-2 popcount / matrix-core comparison")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (batch / serve use the defaults)")
     ap.add_argument("--no-ingest", action="store_true", help="skip the .raw read / device packing timing")
-    ap.add_argument("--no-high", action="store_true", help="skip the order-4 / order-5 scans")
+    ap.add_argument("--no-high", action="store_true", help="skip the order-4 to order-6 scans")
     ap.add_argument("--batch-datasets", type=int, default=64)
     ap.add_argument("--serve-requests", type=int, default=300)
     ap.add_argument("--only-extra", action="store_true",
