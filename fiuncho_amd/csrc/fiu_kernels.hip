@@ -920,8 +920,16 @@ k_scan2m(const DevData D, const ScanParams P, TopState *st, uint32_t *pair_out)
     __shared__ uint32_t lds_hist[HIST ? kHistBins : 1];
     hist_begin<HIST>(lds_hist);
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    // gridDim.y > 1: the tiles of last SNPs split in gridDim.y chunks (more
+    // wavefronts for the pair table of a small dataset)
+    int t_hi = INT32_MAX, t_lo = 0;
+    if (gridDim.y > 1) {
+        const int tiles = ((D.n_pad >> 6) + (int)gridDim.y - 1) / (int)gridDim.y;
+        t_hi = (D.n_pad >> 6) - 1 - (int)blockIdx.y * tiles;
+        t_lo = t_hi - tiles + 1;
+    }
     scan2m_unit<PW, TABLE, HIST>(D, P, st, pair_out, P.j0 + ((int64_t)blockIdx.x * (kBlock / kWave) + wid) * PW,
-                                 P.j1, INT32_MAX, 0, lds_hist);
+                                 P.j1, t_hi, t_lo, lds_hist);
     hist_end<HIST>(lds_hist, P.hist);
 }
 
@@ -2403,7 +2411,12 @@ hipError_t launch_pair_table(const DevData &D, const ScanParams &P, uint32_t *pa
     const int64_t prefixes = P.j1 - P.j0;
     if (prefixes <= 0) return hipSuccess;
     const int64_t per_block = (kBlock / kWave) * kMarginalPW;
-    const dim3 grid((unsigned)((prefixes + per_block - 1) / per_block)), block(kBlock);
+    const int64_t blocks = (prefixes + per_block - 1) / per_block;
+    // at least about 2,048 wavefronts: a small dataset's prefixes alone leave
+    // most CUs idle, so the tiles of last SNPs are split as well
+    const int T = D.n_pad >> 6;
+    const int chunks = (int)std::min<int64_t>(T, std::max<int64_t>(1, 512 / std::max<int64_t>(blocks, 1)));
+    const dim3 grid((unsigned)blocks, (unsigned)chunks), block(kBlock);
     hipLaunchKernelGGL((k_scan2m<kMarginalPW, true, false>), grid, block, 0, s, D, P, (TopState *)nullptr, pair_out);
     return hipGetLastError();
 }
