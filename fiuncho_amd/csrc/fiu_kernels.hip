@@ -1328,11 +1328,17 @@ __device__ __forceinline__ void mfma2_tile(const DevData &D, const ScanParams &P
 #pragma unroll
             for (int q = 0; q < kM2Ring - 1; ++q)
                 if (q < iters) issue(q);
-            for (int it = 0; it < iters; ++it) {
+            // the stages with at least kM2Ring - 1 stages after them run a
+            // body without branches (constant wait, always one stage to
+            // issue), so that the fragment reads and the products share a
+            // basic block and each product waits only for its own operands;
+            // the last kM2Ring - 1 stages run the general body
+            auto stage = [&](int it, bool steady) __attribute__((always_inline)) {
                 // this wave's loads of stage it are complete once at most those
                 // of the later stages issued (4 each) are pending; the barrier
                 // publishes every wave's part
-                wait_stages<kM2Ring - 2, 4>(iters - 1 - it);
+                if (steady) wait_vmcnt<4 * (kM2Ring - 2)>();
+                else wait_stages<kM2Ring - 2, 4>(iters - 1 - it);
                 stage_barrier();
                 const auto &sl = lds[it % kM2Ring];
                 v4i fa[2][2], fb[2][2];   // [substep][genotype]: 32 fp4 samples of row r per lane
@@ -1346,7 +1352,7 @@ __device__ __forceinline__ void mfma2_tile(const DevData &D, const ScanParams &P
                     }
                 }
                 // every wave is past stage it - 1, whose slot takes stage it + kM2Ring - 1
-                if (it + kM2Ring - 1 < iters) issue(it + kM2Ring - 1);
+                if (steady || it + kM2Ring - 1 < iters) issue(it + kM2Ring - 1);
 #pragma unroll
                 for (int s = 0; s < 2; ++s)
 #pragma unroll
@@ -1359,7 +1365,10 @@ __device__ __forceinline__ void mfma2_tile(const DevData &D, const ScanParams &P
                             acc[cls][g][h] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a8, b8, acc[cls][g][h],
                                                                                            4, 4, 0, 0, 0, 0);
                         }
-            }
+            };
+            int it = 0;
+            for (; it + kM2Ring - 1 < iters; ++it) stage(it, true);
+            for (; it < iters; ++it) stage(it, false);
             // the next class's first loads reuse the ring
             stage_barrier();
         }
@@ -1546,30 +1555,42 @@ __device__ __forceinline__ void mfma3_tile(const DevData &D, const ScanParams &P
         const int iters = D.quads[cls];
         const uint4 *img = F.base + F.class_off[cls];
         const int64_t qstride = 8 * (int64_t)np;   // uint4 per quad: 2 genotypes x n_pad x 4
-        auto issue = [&](int q) {
-            // resource over quad q of the class image (8 n_pad uint4)
-            const __amdgpu_buffer_rsrc_t rq =
-                __builtin_amdgcn_make_buffer_rsrc((void *)(img + q * qstride), (short)0, 128 * np, 0x00020000);
+        // resource over quad q of the class image (8 n_pad uint4)
+        auto quad_rsrc3 = [&](int q) __attribute__((always_inline)) {
+            return __builtin_amdgcn_make_buffer_rsrc((void *)(img + q * qstride), (short)0, 128 * np, 0x00020000);
+        };
+        // the b and c rows, and (two lanes of each wave) the a rows, of quad
+        // q: 4 loads per wave in this order
+        auto issue_bc = [&](int q) __attribute__((always_inline)) {
+            const __amdgpu_buffer_rsrc_t rq = quad_rsrc3(q);
             int8_t *s = lds[q % R];
             dma16(rq, ub0, s + (2 * w) * kM3Group);
             dma16(rq, ub1, s + (2 * w + 1) * kM3Group);
             dma16(rq, uc, s + (8 + w) * kM3Group);
-            if (a_lane) dma16(rq, ua, s + 12 * kM3Group);
+        };
+        auto issue_a = [&](int q) __attribute__((always_inline)) {
+            if (a_lane) dma16(quad_rsrc3(q), ua, lds[q % R] + 12 * kM3Group);
+        };
+        auto issue = [&](int q) __attribute__((always_inline)) {
+            issue_bc(q);
+            issue_a(q);
         };
         struct Frag {
             v4i ma[2], fb[2][2], fc[2];
         };
+        // the a and c fragments first: the masks need them, the products
+        // need the b fragments only after that
         auto read = [&](Frag &f, int it) __attribute__((always_inline)) {
             const int8_t *sb = lds[it % R];
 #pragma unroll
             for (int x = 0; x < 2; ++x) f.ma[x] = *(const v4i *)(sb + 12 * kM3Group + x * 64 + ao);
 #pragma unroll
+            for (int z = 0; z < 2; ++z) f.fc[z] = *(const v4i *)(sb + (8 + 2 * z + wc) * kM3Group + fo);
+#pragma unroll
             for (int y = 0; y < 2; ++y)
 #pragma unroll
                 for (int bb = 0; bb < 2; ++bb)
                     f.fb[y][bb] = *(const v4i *)(sb + (4 * y + 2 * wb + bb) * kM3Group + fo);
-#pragma unroll
-            for (int z = 0; z < 2; ++z) f.fc[z] = *(const v4i *)(sb + (8 + 2 * z + wc) * kM3Group + fo);
         };
         // the a mask goes on the narrower operand (a wave's 16 c rows per
         // z, against 32 b rows per y): 4 masked fragments per stage
@@ -1594,20 +1615,32 @@ __device__ __forceinline__ void mfma3_tile(const DevData &D, const ScanParams &P
 #pragma unroll
         for (int q = 0; q < R - 1; ++q)
             if (q < iters) issue(q);
-        for (int it = 0; it < iters; ++it) {
+        // the stages with at least R - 1 stages after them run a body without
+        // branches (constant wait, always one stage to issue), so that the
+        // fragment reads, masks and products share a basic block and each
+        // product waits only for its own operands; the last R - 1 stages run
+        // the general body
+        auto stage = [&](int it, bool steady) __attribute__((always_inline)) {
             // this wave's loads of stage it are complete once at most
             // those of the stages after it (min(iters - 1 - it, R - 2),
             // 4 loads each) are pending; the barrier then publishes every
             // wave's part
             static_assert(R >= 2 && 4 * (R - 2) < 16, "ring depth");
-            wait_stages<R - 2, 4>(iters - 1 - it);
+            if (steady) wait_vmcnt<4 * (R - 2)>();
+            else wait_stages<R - 2, 4>(iters - 1 - it);
             stage_barrier();
             Frag f;
             read(f, it);
-            // every wave is past stage it - 1, whose slot now takes stage it + R - 1
-            if (it + R - 1 < iters) issue(it + R - 1);
+            // every wave is past stage it - 1, whose slot now takes stage it +
+            // R - 1; the a rows' load (two lanes, a branch) after the products
+            const bool more = steady || it + R - 1 < iters;
+            if (more) issue_bc(it + R - 1);
             multiply(f);
-        }
+            if (more) issue_a(it + R - 1);
+        };
+        int it = 0;
+        for (; it + R - 1 < iters; ++it) stage(it, true);
+        for (; it < iters; ++it) stage(it, false);
         // the next class's first loads reuse the ring: every wave must be
         // done reading this class's last stages
         stage_barrier();
