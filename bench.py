@@ -224,16 +224,17 @@ def bench_config1(repeats=20):
 def bench_high_orders():
     """Orders 4 to 6 (the last-SNP-marginal popcount scan, k_scan_km) on
     seeded HWE datasets of 16,000 samples (8,000 / 8,000), top-100; order 4
-    also with the all-cells direct kernel, the reference's algorithm."""
+    also with the all-cells direct kernel, the reference's algorithm; orders
+    7 and 8 with the all-cells per-workgroup kernel (k_scan_any)."""
     import fiuncho_amd as fa
     out = {}
-    for order, snps in ((4, 400), (5, 100), (6, 40)):
+    for order, snps in ((4, 400), (5, 100), (6, 40), (7, 20), (8, 16)):
         spec = fa.synth_spec(snps, 8000, 8000, seed=5)
         d = fa.Dataset.synth(spec)
         dist = fa.Distribution(snps, order - 1)
         res = {}
         for name, algo in (("marginal", fa.ALGO_MARGINAL), ("direct", fa.ALGO_DIRECT)):
-            if order >= 5 and algo == fa.ALGO_DIRECT:
+            if (order in (5, 6) and algo == fa.ALGO_DIRECT) or (order >= 7 and algo == fa.ALGO_MARGINAL):
                 continue
             s = fa.GpuSearch(algorithm=algo)
             s.run_arrays(d, order, fa.Distribution(snps, order - 1, 64, 63), 100)   # code and tables
@@ -252,7 +253,7 @@ def bench_high_orders():
             res[name + "_result"] = (c, m)
             print(f"[bench] order {order} {snps} x 16000 {name}: {st.scan_ms:.1f} ms scan, {cs:.3e} combo*samples/s",
                   file=sys.stderr, flush=True)
-        if "direct" in res:
+        if "direct" in res and "marginal" in res:
             res["identical"] = bool(res["marginal_result"][0].tolist() == res["direct_result"][0].tolist() and
                                     np.array_equal(res["marginal_result"][1], res["direct_result"][1]))
             res["speedup"] = res["direct"]["scan_ms"] / res["marginal"]["scan_ms"]

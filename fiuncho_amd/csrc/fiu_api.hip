@@ -621,9 +621,11 @@ void search(fiu_dataset *d, int order, int32_t step, int32_t offset, uint32_t ou
         fail(FIU_ERR_INVALID, "marginal algorithm needs every SNP to cover the same samples");
     if (algo_req == FIU_ALGO_MFMA && order != 2 && order != 3)
         fail(FIU_ERR_UNSUPPORTED, "the matrix-core scan covers orders 2 and 3");
-    if (order > kMaxDirectOrder && (algo_req == FIU_ALGO_DIRECT || !d->dev.uniform))
+    if (order > kMaxDirectOrder && order <= kMaxKmOrder && (algo_req == FIU_ALGO_DIRECT || !d->dev.uniform))
         fail(FIU_ERR_UNSUPPORTED, "order " + std::to_string(order) +
                                       " is scanned marginally only: every SNP must cover the same samples");
+    if (order >= kMinAnyOrder && algo_req == FIU_ALGO_MARGINAL)
+        fail(FIU_ERR_UNSUPPORTED, "orders 7 and 8 count every cell (FIU_ALGO_DIRECT)");
     if (algo_req != FIU_ALGO_AUTO && !valid_variant(algo_req, order, variant))
         fail(FIU_ERR_INVALID, "unknown variant " + std::to_string(variant) + " of this algorithm");
     // AUTO: the tuned choice of this shape class when the dataset allows it
@@ -658,7 +660,8 @@ void search(fiu_dataset *d, int order, int32_t step, int32_t offset, uint32_t ou
     ensure_binom(d, M, s);
     int algo = FIU_ALGO_DIRECT;
     const bool small_classes = d->cases < (1u << 24) && d->ctrls < (1u << 24);   // fp32-exact counts
-    if (algo_req != FIU_ALGO_DIRECT && d->dev.uniform && order >= 4) algo = FIU_ALGO_MARGINAL;
+    if (algo_req != FIU_ALGO_DIRECT && d->dev.uniform && order >= 4 && order <= kMaxKmOrder)
+        algo = FIU_ALGO_MARGINAL;
     if (algo_req != FIU_ALGO_DIRECT && d->dev.uniform && (order == 2 || order == 3)) {
         algo = FIU_ALGO_MARGINAL;
         if (order == 3 && !ensure_pair_table(d, s)) algo = FIU_ALGO_DIRECT;  // table does not fit
@@ -729,7 +732,9 @@ void search(fiu_dataset *d, int order, int32_t step, int32_t offset, uint32_t ou
     // combination it samples with a popcount kernel), 1/256 ahead of a
     // matrix-core scan, which costs 5-12x less per combination
     const double sample_frac = algo == FIU_ALGO_MFMA ? 1.0 / 256.0 : 1.0 / 16.0;
-    const uint64_t sample_cap = std::max<uint64_t>(65536, (uint64_t)((double)n_pref * per_avg * sample_frac));
+    // (orders 7-8 cost about 100x more per combination: a smaller floor)
+    const uint64_t sample_cap = std::max<uint64_t>(order >= kMinAnyOrder ? 4096 : 65536,
+                                                   (uint64_t)((double)n_pref * per_avg * sample_frac));
     const int64_t sample_prefixes =
         chunks.empty() || !sample
             ? 0
@@ -878,8 +883,10 @@ fiu_status fiu_set_tuning(const fiu_tune_entry *entries, size_t n)
                 fail(FIU_ERR_INVALID, "tuning entry with an unknown algorithm");
             if (e.algorithm == FIU_ALGO_MFMA && e.order != 2 && e.order != 3)
                 fail(FIU_ERR_INVALID, "matrix-core tuning entry above order 3");
-            if (e.algorithm == FIU_ALGO_DIRECT && e.order > kMaxDirectOrder)
-                fail(FIU_ERR_INVALID, "direct tuning entry above order 4");
+            if (e.algorithm == FIU_ALGO_DIRECT && e.order > kMaxDirectOrder && e.order <= kMaxKmOrder)
+                fail(FIU_ERR_INVALID, "direct tuning entry at orders 5-6");
+            if (e.algorithm == FIU_ALGO_MARGINAL && e.order >= kMinAnyOrder)
+                fail(FIU_ERR_INVALID, "marginal tuning entry above order 6");
             if (!valid_variant(e.algorithm, e.order, e.variant)) fail(FIU_ERR_INVALID, "tuning entry with an unknown variant");
             if (e.snps_lo >= e.snps_hi || e.samples_lo >= e.samples_hi)
                 fail(FIU_ERR_INVALID, "tuning entry with an empty range");

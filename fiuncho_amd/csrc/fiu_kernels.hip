@@ -535,6 +535,129 @@ k_scan_direct(const DevData D, const ScanParams P, TopState *st)
 }
 
 // ---------------------------------------------------------------------------
+// Orders 7 and 8 (FIU_ALGO_DIRECT: all cells, any dataset).
+// ---------------------------------------------------------------------------
+// A workgroup takes one prefix (s_0 < ... < s_{k-2}) of the distribution and
+// runs through its last SNPs c.  Every one of the 3^k cells is counted as the
+// reference counts it: the AND of one genotype row per SNP, then popcount
+// (gt_combine.cpp:3-27, gt_popcnt.cpp:10-18).  Thread t < 243 owns the
+// genotypes of the first five SNPs (t = sum g_q 3^(4-q)); at order 8 a loop
+// takes the sixth; per 128-bit quad the thread counts the 9 cells of the last
+// two SNPs.  Cell x = sum g_q 3^(k-1-q) (ContingencyTable.h:67-90); the
+// counts of both classes go to LDS.  The MI is then summed in FP64 over the
+// cells in any order (all threads, then a reduction).  Its difference from
+// the cell-order value is below 3^k x 2^-52 x sum |p ln p| < 1e-10 nats, so
+// a combination whose value plus 1e-9 rounds below the current threshold
+// would be rejected by its exact value as well.  The others get the exact
+// value in cell order (mi_exact's chain: wave 0 computes the terms of 64
+// cells at a time and every lane runs the chain over them).
+template <int K, bool HIST>
+__global__ void __launch_bounds__(kBlock) k_scan_any(const DevData D, const ScanParams P, TopState *st)
+{
+    static_assert(K >= 7 && K <= 8, "orders 7 and 8");
+    constexpr int M = K - 1, CELLS = Pow3<K>::v, PASSES = Pow3<K - 7>::v;
+    __shared__ uint32_t cnt[2][CELLS];
+    __shared__ double red[2][kBlock / kWave];
+    __shared__ int pass_sh;
+    const int64_t j = P.j0 + blockIdx.x;
+    int s[K];
+    if (!prefix_at<M>(P, D.n, j, P.j1, s)) return;   // uniform over the workgroup
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const float tau = st->tau;
+    const uint64_t tau_combo = st->tau_combo;
+    const int np = D.n_pad;
+    int g5[5];
+    {
+        int t = tid < 243 ? tid : 0;
+#pragma unroll
+        for (int q = 4; q >= 0; --q) g5[q] = t % 3, t /= 3;
+    }
+    uint64_t pre_packed = 0;
+#pragma unroll
+    for (int i = 0; i < M; ++i) pre_packed = (pre_packed << P.combo_bits) | (uint64_t)s[i];
+    for (int c = s[M - 1] + 1; c < D.n; ++c) {
+        s[M] = c;
+#pragma unroll
+        for (int pi = 0; pi < PASSES; ++pi) {
+#pragma unroll
+            for (int cls = 0; cls < 2; ++cls) {
+                uint32_t n9[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+                const uint4 *base = D.bits + D.class_off[cls];
+                for (int q = 0; q < D.quads[cls]; ++q) {
+                    // row z of SNP x in quad q: rq[z * n_pad + x]
+                    const uint4 *rq = base + (uint64_t)q * 3 * np;
+                    uint4 pr = rq[g5[0] * np + s[0]];
+#pragma unroll
+                    for (int i = 1; i < 5; ++i) pr = pr & rq[g5[i] * np + s[i]];
+                    if constexpr (K == 8) pr = pr & rq[pi * np + s[5]];
+#pragma unroll
+                    for (int z1 = 0; z1 < 3; ++z1) {
+                        const uint4 pz = pr & rq[z1 * np + s[K - 2]];
+#pragma unroll
+                        for (int z2 = 0; z2 < 3; ++z2) {
+                            const uint4 v = pz & rq[z2 * np + s[K - 1]];
+                            n9[3 * z1 + z2] += __builtin_popcount(v.x) + __builtin_popcount(v.y) +
+                                               __builtin_popcount(v.z) + __builtin_popcount(v.w);
+                        }
+                    }
+                }
+                if (tid < 243) {
+#pragma unroll
+                    for (int z = 0; z < 9; ++z) cnt[cls][(tid * PASSES + pi) * 9 + z] = n9[z];
+                }
+            }
+        }
+        __syncthreads();
+        // the MI in any order (the screen)
+        double hx = 0.0, hall = 0.0;
+        for (int x = tid; x < CELLS; x += kBlock) {
+            const double pc = (double)cnt[0][x] / P.n64, pt = (double)cnt[1][x] / P.n64;
+            hall += plogp(pc) + plogp(pt);
+            hx += plogp(pc + pt);
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            hx += __shfl_xor(hx, off);
+            hall += __shfl_xor(hall, off);
+        }
+        if (lane == 0) red[0][wv] = hx, red[1][wv] = hall;
+        const uint64_t combo = (pre_packed << P.combo_bits) | (uint64_t)c;
+        __syncthreads();
+        if (tid == 0) {
+            double sx = 0.0, sa = 0.0;
+#pragma unroll
+            for (int w = 0; w < kBlock / kWave; ++w) sx += red[0][w], sa += red[1][w];
+            const double v = P.hy64 - sx + sa;
+            if constexpr (HIST) atomicAdd(&P.hist[hist_bin((float)v)], 1u);
+            pass_sh = !HIST && (float)(v + 1e-9) >= tau;
+        }
+        __syncthreads();
+        if (pass_sh && wv == 0) {
+#pragma clang fp contract(off)
+            double ex = 0.0, ea = 0.0;
+            for (int x0 = 0; x0 < CELLS; x0 += kWave) {
+                const int x = x0 + lane;
+                double tc = 0.0, tt = 0.0, ta = 0.0;
+                if (x < CELLS) {
+                    const double pc = (double)cnt[0][x] / P.n64, pt = (double)cnt[1][x] / P.n64, pa = pc + pt;
+                    tc = plogp(pc), tt = plogp(pt), ta = plogp(pa);
+                }
+                const int last = CELLS - x0 < kWave ? CELLS - x0 : kWave;
+                for (int l = 0; l < last; ++l) {
+                    const double a = __shfl(tc, l), b = __shfl(tt, l), e = __shfl(ta, l);
+                    ea -= a;
+                    ea -= b;
+                    ex -= e;
+                }
+            }
+            const float mf = (float)(ex + P.hy64 - ea);
+            push_candidates(lane == 0 && better(mf, combo, tau, tau_combo), mf, combo, P, st);
+        }
+        __syncthreads();   // cnt and pass_sh are rewritten for the next c
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Orders 4 to 6, marginal in the last SNP (FIU_ALGO_MARGINAL).
 // ---------------------------------------------------------------------------
 // A wavefront takes one prefix (s_0 < ... < s_{k-2}) of the distribution and
@@ -2286,6 +2409,9 @@ static hipError_t scan_direct(int order, const DevData &D, const ScanParams &P, 
     case 2: hipLaunchKernelGGL((k_scan_direct<2, HIST>), grid, block, 0, s, D, P, st); break;
     case 3: hipLaunchKernelGGL((k_scan_direct<3, HIST>), grid, block, 0, s, D, P, st); break;
     case 4: hipLaunchKernelGGL((k_scan_direct<4, HIST>), grid, block, 0, s, D, P, st); break;
+    // one prefix per workgroup
+    case 7: hipLaunchKernelGGL((k_scan_any<7, HIST>), dim3((unsigned)waves), block, 0, s, D, P, st); break;
+    case 8: hipLaunchKernelGGL((k_scan_any<8, HIST>), dim3((unsigned)waves), block, 0, s, D, P, st); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

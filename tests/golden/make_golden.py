@@ -6,9 +6,9 @@ libfiuncho_ref_base.so: scalar build, both ALIGN=64) and stores inputs and
 outputs only:
 
   testdata_tables.npz   every combination of tests/data (10 SNPs) at orders
-                        2 to 6: counts (cases, controls; 3^k cells) and the
+                        2 to 8: counts (cases, controls; 3^k cells) and the
                         float MI of both reference builds
-  testdata_search.json  ThreadedSearch(1).run top-90 at orders 2 to 6, and the
+  testdata_search.json  ThreadedSearch(1).run top-90 at orders 2 to 8, and the
                         SHA-256 of the bit tables the reference loader builds
                         from the .tped/.tfam, .raw and GAMETES .raw files
   loader_quirk.npz      a small .raw and .tped/.tfam (their text is stored) whose
@@ -49,6 +49,8 @@ SYNTH = [
     ("s4_small", 24, 200, 200, 15, 4, 30, (1, 7, 12, 20)),
     ("s5_small", 16, 300, 260, 16, 5, 40, (2, 5, 9, 11, 14)),
     ("s6_small", 12, 400, 350, 17, 6, 60, (0, 3, 4, 7, 9, 11)),
+    ("s7_small", 11, 300, 280, 18, 7, 50, ()),
+    ("s8_small", 10, 260, 250, 19, 8, 40, ()),
 ]
 
 
@@ -76,7 +78,7 @@ def main():
     raw = ref.load(os.path.join(DATA, "test.raw"))
     gam = ref.load(os.path.join(DATA, "test.gametes.raw"))
     tables = {}
-    for k in (2, 3, 4, 5, 6):
+    for k in (2, 3, 4, 5, 6, 7, 8):
         combos = np.array(list(itertools.combinations(range(tp.snps), k)), np.int32)
         cc = np.zeros((len(combos), 3 ** k), np.uint32)
         ct = np.zeros_like(cc)
@@ -99,7 +101,7 @@ def main():
               "sha256": {"tped": sha(tp.words()), "raw": sha(raw.words()), "gametes": sha(gam.words())},
               "build": "g++ -O3 -march=x86-64-v4 -mavx512vpopcntdq -std=gnu++17 -DALIGN=64",
               "top": {}}
-    for k in (2, 3, 4, 5, 6):
+    for k in (2, 3, 4, 5, 6, 7, 8):
         combos, mi, _ = tp.search(k, 90, threads=1)
         search["top"][str(k)] = {"combos": combos.tolist(), "mi": [float(x) for x in mi]}
     with open(os.path.join(HERE, "testdata_search.json"), "w") as f:

@@ -1,11 +1,12 @@
-"""GPU: orders 4 to 6, marginal in the last SNP (k_scan_km).
+"""GPU: orders 4 to 6, marginal in the last SNP (k_scan_km), and orders 7
+and 8, every cell counted (k_scan_any).
 
 The prefix's 3^(k-1) genotype rows against genotypes 0 and 1 of the last
 SNP, genotype 2 from the prefix rows' own counts, the rows split over 3
 (order 4), 9 (order 5) or 27 (order 6) lanes per last SNP.  Bar: the same
 top-N (combinations, order and float MI bit for bit) as the oracle's FP64
 ranking, as the direct all-cells kernel at order 4, and the reference's own
-top-90 at orders 5 and 6 (tests/data, in test_gpu_parity)."""
+top-90 at orders 5 to 8 (tests/data, in test_gpu_parity)."""
 import math
 
 import numpy as np
@@ -101,4 +102,57 @@ def test_incomplete_data():
     with pytest.raises(fa.FiuError):
         fa.GpuSearch().run_arrays(d, 6, fa.Distribution(12, 5), 10)
     with pytest.raises(fa.FiuError):
-        fa.GpuSearch().run_arrays(d, 7, fa.Distribution(12, 6), 10)   # above order 6
+        fa.GpuSearch().run_arrays(d, 9, fa.Distribution(12, 8), 10)   # above order 8
+    # orders 7 and 8 count every cell: incomplete data is scanned exactly
+    for order in (7, 8):
+        print(f"case: incomplete data order {order}", flush=True)
+        s = fa.GpuSearch()
+        c, m = s.run_arrays(d, order, fa.Distribution(12, order - 1), 40)
+        assert s.last_stats.algorithm == fa.ALGO_DIRECT
+        oc, om, _ = O.search(od, order, 40, threads=2, mi_mode=2)
+        assert c.tolist() == oc.tolist() and np.array_equal(m, om)
+
+
+@pytest.mark.parametrize("order,snps,cases,ctrls,outputs,seed", [
+    (7, 12, 300, 260, 200, 51),     # 792 combinations
+    (7, 9, 130, 70, 36, 52),        # every combination (C(9, 7) = 36), ragged classes
+    (7, 7, 50, 70, 1, 53),          # the single combination
+    (8, 11, 300, 280, 165, 54),     # every combination (C(11, 8) = 165)
+    (8, 10, 1000, 900, 30, 55),     # 8 + 8 quads per row
+    (8, 8, 64, 64, 1, 56),          # the single combination, whole words
+])
+def test_all_cells_orders_vs_oracle(order, snps, cases, ctrls, outputs, seed):
+    # orders 7 and 8: every 3^k cell counted per combination (k_scan_any)
+    print(f"case: order {order} {snps}x{cases}+{ctrls}", flush=True)
+    d, od = synth(snps, cases, ctrls, seed)
+    oc, om, _ = O.search(od, order, outputs, threads=4, mi_mode=2)
+    s = fa.GpuSearch()
+    c, m = s.run_arrays(d, order, fa.Distribution(snps, order - 1), outputs)
+    assert s.last_stats.algorithm == fa.ALGO_DIRECT
+    assert c.tolist() == oc.tolist() and np.array_equal(m, om)
+    assert s.last_stats.combinations == math.comb(snps, order)
+    with pytest.raises(fa.FiuError):
+        fa.GpuSearch(algorithm=fa.ALGO_MARGINAL).run_arrays(d, order, fa.Distribution(snps, order - 1), 5)
+
+
+def test_all_cells_order7_shares_and_chunks():
+    snps = 13
+    print("case: order 7 shares, chunks", flush=True)
+    d, od = synth(snps, 400, 380, 57)
+    full = fa.GpuSearch().run_arrays(d, 7, fa.Distribution(snps, 6), 300)
+    oc, om, _ = O.search(od, 7, 300, threads=4, mi_mode=2)
+    assert full[0].tolist() == oc.tolist() and np.array_equal(full[1], om)
+    merged = {}
+    for off in range(3):
+        dist = fa.Distribution(snps, 6).layer(3, off)
+        c, m = fa.GpuSearch().run_arrays(d, 7, dist, 300)
+        sc, sm, _ = O.search(od, 7, 300, step=3, offset=off, threads=1, mi_mode=2)
+        assert c.tolist() == sc.tolist() and np.array_equal(m, sm)
+        merged.update({tuple(x): v for x, v in zip(c.tolist(), m)})
+    best = sorted(merged.items(), key=lambda t: (-t[1], t[0]))[:300]
+    assert [c for c, _ in best] == [tuple(c) for c in full[0].tolist()]
+    s = fa.GpuSearch(chunk_combos=200, candidate_cap=64, sample_threshold=False)
+    c, m = s.run_arrays(d, 7, fa.Distribution(snps, 6), 60)
+    oc, om, _ = O.search(od, 7, 60, threads=4, mi_mode=2)
+    assert c.tolist() == oc.tolist() and np.array_equal(m, om)
+    assert s.last_stats.redo_chunks > 0

@@ -81,8 +81,8 @@ def test_dataset_read_errors():
 
 
 def test_contingency_all_combinations(ds, tables):
-    print("case: contingency_batch all 837 combinations", flush=True)
-    for k in (2, 3, 4, 5, 6):
+    print("case: contingency_batch all 1,002 combinations", flush=True)
+    for k in (2, 3, 4, 5, 6, 7, 8):
         combos = tables[f"k{k}_combos"]
         counts, mi = fa.contingency_batch(ds, combos)
         cells = 3 ** k
@@ -129,13 +129,13 @@ def test_mi_batch_known_answers():
 def test_search_testdata(ds, search_golden, tables, algorithm):
     # tests/threadedsearch.cpp:17-31 plus the reference's own top-90
     print(f"case: search tests/data algorithm={algorithm}", flush=True)
-    for o in ((2, 3, 4, 5, 6) if algorithm == fa.ALGO_AUTO else (2, 3, 4)):   # orders 5-6: marginal only
+    for o in ((2, 3, 4, 5, 6, 7, 8) if algorithm == fa.ALGO_AUTO else (2, 3, 4, 7, 8)):   # 5-6: marginal only
         res = fa.GpuSearch(algorithm=algorithm).run(ds, o, fa.Distribution(10, o - 1), 90)
         combos = np.array([r.combination for r in res])
         mi = np.array([r.val for r in res], np.float32)
         assert len(res) > 0
         if o > 2:
-            assert len(res) == 90
+            assert len(res) == min(90, math.comb(10, o))
         assert combos.shape[1] == o
         assert len({tuple(c) for c in combos}) == len(combos)
         assert (np.diff(combos, axis=1) > 0).all()

@@ -146,8 +146,8 @@ def test_popcnt_matches_bitset(tped):
 
 
 def test_counts_match_reference_all_combinations(tped, golden_tables):
-    # every combination of tests/data at orders 2-6 against the reference build
-    for k in (2, 3, 4, 5, 6):
+    # every combination of tests/data at orders 2-8 against the reference build
+    for k in (2, 3, 4, 5, 6, 7, 8):
         combos = golden_tables[f"k{k}_combos"]
         for i, c in enumerate(combos):
             cc, ct = O.counts(tped, c)
@@ -164,7 +164,7 @@ def test_counts_bruteforce_synthetic():
     geno[4, 100] = -1
     is_ctrl = (rng.random(157) < 0.45).astype(np.uint8)
     d = O.pack(geno, is_ctrl)
-    for k in (2, 3, 4, 5, 6):
+    for k in (2, 3, 4, 5, 6, 7):
         for c in itertools.combinations(range(7), k):
             assert all(np.array_equal(a, b) for a, b in zip(O.counts(d, c), O.counts_bruteforce(geno, is_ctrl, c)))
 
@@ -183,8 +183,8 @@ def test_mi_known_answers():
 
 
 def test_mi_f32_bit_exact_vs_reference(golden_tables):
-    # the reference's float MI bits, both builds, all 837 tables of tests/data
-    for k in (2, 3, 4, 5, 6):
+    # the reference's float MI bits, both builds, all 1,002 tables of tests/data
+    for k in (2, 3, 4, 5, 6, 7, 8):
         cc, ct = golden_tables[f"k{k}_cases"], golden_tables[f"k{k}_ctrls"]
         for i in range(len(cc)):
             if O.has_avx512():
@@ -237,11 +237,11 @@ def test_distribution_partition(n, k1):
 @pytest.mark.parametrize("threads", [1, 32])
 def test_search_reference_test(tped, threads):
     # tests/threadedsearch.cpp:17-31
-    for o in (2, 3, 4, 5, 6):
+    for o in (2, 3, 4, 5, 6, 7, 8):
         combos, mi, _ = O.search(tped, o, 90, threads=threads, mi_mode=0)
         assert len(combos) > 0
         if o > 2:
-            assert len(combos) == 90
+            assert len(combos) == min(90, math.comb(10, o))
         assert combos.shape[1] == o
         assert len({tuple(c) for c in combos}) == len(combos)
         assert (np.diff(combos, axis=1) > 0).all()
@@ -251,7 +251,7 @@ def test_search_reference_test(tped, threads):
 
 def test_search_matches_reference_top(tped, golden_search):
     # same top-90 sets and values as ThreadedSearch(1) of the reference build
-    for o in (2, 3, 4, 5, 6):
+    for o in (2, 3, 4, 5, 6, 7, 8):
         combos, mi, _ = O.search(tped, o, 90, threads=4, mi_mode=0)
         g = golden_search["top"][str(o)]
         assert {tuple(c) for c in combos} == {tuple(c) for c in g["combos"]}
