@@ -225,10 +225,10 @@ def bench_high_orders():
     """Orders 4 to 6 (the last-SNP-marginal popcount scan, k_scan_km) on
     seeded HWE datasets of 16,000 samples (8,000 / 8,000), top-100; order 4
     also with the all-cells direct kernel, the reference's algorithm; orders
-    7 and 8 with the all-cells per-workgroup kernel (k_scan_any)."""
+    7 to 9 with the all-cells per-workgroup kernel (k_scan_any)."""
     import fiuncho_amd as fa
     out = {}
-    for order, snps in ((4, 400), (5, 100), (6, 40), (7, 20), (8, 16)):
+    for order, snps in ((4, 400), (5, 100), (6, 40), (7, 20), (8, 16), (9, 14)):
         spec = fa.synth_spec(snps, 8000, 8000, seed=5)
         d = fa.Dataset.synth(spec)
         dist = fa.Distribution(snps, order - 1)

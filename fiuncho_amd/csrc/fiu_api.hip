@@ -625,7 +625,7 @@ void search(fiu_dataset *d, int order, int32_t step, int32_t offset, uint32_t ou
         fail(FIU_ERR_UNSUPPORTED, "order " + std::to_string(order) +
                                       " is scanned marginally only: every SNP must cover the same samples");
     if (order >= kMinAnyOrder && algo_req == FIU_ALGO_MARGINAL)
-        fail(FIU_ERR_UNSUPPORTED, "orders 7 and 8 count every cell (FIU_ALGO_DIRECT)");
+        fail(FIU_ERR_UNSUPPORTED, "orders 7 to 9 count every cell (FIU_ALGO_DIRECT)");
     if (algo_req != FIU_ALGO_AUTO && !valid_variant(algo_req, order, variant))
         fail(FIU_ERR_INVALID, "unknown variant " + std::to_string(variant) + " of this algorithm");
     // AUTO: the tuned choice of this shape class when the dataset allows it

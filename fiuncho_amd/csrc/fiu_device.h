@@ -9,11 +9,11 @@ namespace fiu {
 
 constexpr int kWave = 64;            // CDNA wavefront width
 constexpr int kBlock = 256;          // threads per scan workgroup (4 waves)
-constexpr int kMaxOrder = 8;         // orders with a scan kernel
+constexpr int kMaxOrder = 9;         // orders with a scan kernel
 constexpr int kMaxBatchOrder = 5;    // orders the batched engine schedules
 constexpr int kMaxDirectOrder = 4;   // orders 2-4: an all-cells (direct) kernel
 constexpr int kMaxKmOrder = 6;       // orders 4-6: marginal in the last SNP only
-constexpr int kMinAnyOrder = 7;      // orders 7-8: the all-cells per-workgroup kernel
+constexpr int kMinAnyOrder = 7;      // orders 7-9: the all-cells per-workgroup kernel
 constexpr int kMaxPrefix = 7;        // longest prefix the unranker handles
 constexpr int kMarginalPW = 4;       // prefixes per wavefront in marginal scans
 constexpr int kMergeSlots = 8192;    // LDS entries of the top-N merge

@@ -535,15 +535,15 @@ k_scan_direct(const DevData D, const ScanParams P, TopState *st)
 }
 
 // ---------------------------------------------------------------------------
-// Orders 7 and 8 (FIU_ALGO_DIRECT: all cells, any dataset).
+// Orders 7 to 9 (FIU_ALGO_DIRECT: all cells, any dataset).
 // ---------------------------------------------------------------------------
 // A workgroup takes one prefix (s_0 < ... < s_{k-2}) of the distribution and
 // runs through its last SNPs c.  Every one of the 3^k cells is counted as the
 // reference counts it: the AND of one genotype row per SNP, then popcount
 // (gt_combine.cpp:3-27, gt_popcnt.cpp:10-18).  Thread t < 243 owns the
-// genotypes of the first five SNPs (t = sum g_q 3^(4-q)); at order 8 a loop
-// takes the sixth; per 128-bit quad the thread counts the 9 cells of the last
-// two SNPs.  Cell x = sum g_q 3^(k-1-q) (ContingencyTable.h:67-90); the
+// genotypes of the first five SNPs (t = sum g_q 3^(4-q)); at orders 8 and 9 a
+// loop takes those of the middle SNPs (the sixth, and the seventh); per
+// 128-bit quad the thread counts the 9 cells of the last two SNPs.  Cell x = sum g_q 3^(k-1-q) (ContingencyTable.h:67-90); the
 // counts of both classes go to LDS.  The MI is then summed in FP64 over the
 // cells in any order (all threads, then a reduction).  Its difference from
 // the cell-order value is below 3^k x 2^-52 x sum |p ln p| < 1e-10 nats, so
@@ -554,7 +554,7 @@ k_scan_direct(const DevData D, const ScanParams P, TopState *st)
 template <int K, bool HIST>
 __global__ void __launch_bounds__(kBlock) k_scan_any(const DevData D, const ScanParams P, TopState *st)
 {
-    static_assert(K >= 7 && K <= 8, "orders 7 and 8");
+    static_assert(K >= 7 && K <= 9, "orders 7 to 9");   // order 9: 157.5 KB of LDS
     constexpr int M = K - 1, CELLS = Pow3<K>::v, PASSES = Pow3<K - 7>::v;
     __shared__ uint32_t cnt[2][CELLS];
     __shared__ double red[2][kBlock / kWave];
@@ -589,7 +589,8 @@ __global__ void __launch_bounds__(kBlock) k_scan_any(const DevData D, const Scan
                     uint4 pr = rq[g5[0] * np + s[0]];
 #pragma unroll
                     for (int i = 1; i < 5; ++i) pr = pr & rq[g5[i] * np + s[i]];
-                    if constexpr (K == 8) pr = pr & rq[pi * np + s[5]];
+                    if constexpr (K >= 8) pr = pr & rq[(K == 8 ? pi : pi / 3) * np + s[5]];
+                    if constexpr (K == 9) pr = pr & rq[(pi % 3) * np + s[6]];
 #pragma unroll
                     for (int z1 = 0; z1 < 3; ++z1) {
                         const uint4 pz = pr & rq[z1 * np + s[K - 2]];
@@ -2412,6 +2413,7 @@ static hipError_t scan_direct(int order, const DevData &D, const ScanParams &P, 
     // one prefix per workgroup
     case 7: hipLaunchKernelGGL((k_scan_any<7, HIST>), dim3((unsigned)waves), block, 0, s, D, P, st); break;
     case 8: hipLaunchKernelGGL((k_scan_any<8, HIST>), dim3((unsigned)waves), block, 0, s, D, P, st); break;
+    case 9: hipLaunchKernelGGL((k_scan_any<9, HIST>), dim3((unsigned)waves), block, 0, s, D, P, st); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -118,7 +118,7 @@ fiu_status fiu_dataset_destroy(fiu_dataset *ds);
 typedef enum {
     FIU_ALGO_AUTO = 0,     /* fastest exact path the dataset allows */
     FIU_ALGO_DIRECT = 1,   /* AND+popcount of all 3^k cells (the reference's);
-                              orders 2-4 and 7-8 (the only algorithm at 7-8) */
+                              orders 2-4 and 7-9 (the only algorithm at 7-9) */
     FIU_ALGO_MARGINAL = 2, /* orders 2-3: 2^k cells + inclusion-exclusion
                               from exact lower-order counts; orders 4-6: the
                               prefix's 3^(k-1) rows against genotypes 0 and 1
@@ -166,9 +166,9 @@ typedef struct {
     int32_t variant;         /* its kernel variant (fiu_tune_entry) */
 } fiu_search_stats;
 
-/* Exhaustive order-k search (k = 2..8; orders 5-6 need every SNP to cover the
+/* Exhaustive order-k search (k = 2..9; orders 5-6 need every SNP to cover the
  * same samples; a combination packs in 64 bits, so k * ceil(log2(snps)) <= 64,
- * e.g. at most 512 SNPs at order 7) over the combinations whose (k-1)-prefix belongs
+ * e.g. at most 512 SNPs at order 7, 128 at order 9) over the combinations whose (k-1)-prefix belongs
  * to Distribution<int>(snps, order-1, dist_step, dist_offset)
  * (Distribution.h:147-176), keeping the `outputs` best by mutual information.
  * Results are ordered by MI descending, ties by combination ascending;

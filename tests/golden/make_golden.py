@@ -8,7 +8,7 @@ outputs only:
   testdata_tables.npz   every combination of tests/data (10 SNPs) at orders
                         2 to 8: counts (cases, controls; 3^k cells) and the
                         float MI of both reference builds
-  testdata_search.json  ThreadedSearch(1).run top-90 at orders 2 to 8, and the
+  testdata_search.json  ThreadedSearch(1).run top-90 at orders 2 to 9, and the
                         SHA-256 of the bit tables the reference loader builds
                         from the .tped/.tfam, .raw and GAMETES .raw files
   loader_quirk.npz      a small .raw and .tped/.tfam (their text is stored) whose
@@ -51,6 +51,7 @@ SYNTH = [
     ("s6_small", 12, 400, 350, 17, 6, 60, (0, 3, 4, 7, 9, 11)),
     ("s7_small", 11, 300, 280, 18, 7, 50, ()),
     ("s8_small", 10, 260, 250, 19, 8, 40, ()),
+    ("s9_small", 11, 240, 230, 20, 9, 30, ()),
 ]
 
 
@@ -101,7 +102,7 @@ def main():
               "sha256": {"tped": sha(tp.words()), "raw": sha(raw.words()), "gametes": sha(gam.words())},
               "build": "g++ -O3 -march=x86-64-v4 -mavx512vpopcntdq -std=gnu++17 -DALIGN=64",
               "top": {}}
-    for k in (2, 3, 4, 5, 6, 7, 8):
+    for k in (2, 3, 4, 5, 6, 7, 8, 9):
         combos, mi, _ = tp.search(k, 90, threads=1)
         search["top"][str(k)] = {"combos": combos.tolist(), "mi": [float(x) for x in mi]}
     with open(os.path.join(HERE, "testdata_search.json"), "w") as f:

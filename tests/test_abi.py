@@ -119,7 +119,7 @@ def test_tuning_table_roundtrip_and_validation():
                E(4, 0, 10, 0, 10, fa.ALGO_MFMA, 0),         # matrix cores: orders 2-3
                E(5, 0, 10, 0, 10, fa.ALGO_DIRECT, 0),       # all-cells kernels: orders 2-4 and 7-8
                E(7, 0, 10, 0, 10, fa.ALGO_MARGINAL, 0),     # marginal: orders 2-6
-               E(9, 0, 10, 0, 10, fa.ALGO_DIRECT, 0),       # above order 8
+               E(10, 0, 10, 0, 10, fa.ALGO_DIRECT, 0),      # above order 9
                E(4, 0, 10, 0, 10, fa.ALGO_MARGINAL, 4),     # prefixes per wave: orders 2-3
                E(2, 10, 10, 0, 10, fa.ALGO_DIRECT, 0),      # empty SNP range
                E(2, 0, 10, 0, 10, fa.ALGO_AUTO, 0),         # AUTO is not a choice

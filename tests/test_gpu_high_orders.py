@@ -1,12 +1,13 @@
 """GPU: orders 4 to 6, marginal in the last SNP (k_scan_km), and orders 7
-and 8, every cell counted (k_scan_any).
+to 9, every cell counted (k_scan_any).
 
 The prefix's 3^(k-1) genotype rows against genotypes 0 and 1 of the last
 SNP, genotype 2 from the prefix rows' own counts, the rows split over 3
 (order 4), 9 (order 5) or 27 (order 6) lanes per last SNP.  Bar: the same
 top-N (combinations, order and float MI bit for bit) as the oracle's FP64
 ranking, as the direct all-cells kernel at order 4, and the reference's own
-top-90 at orders 5 to 8 (tests/data, in test_gpu_parity)."""
+top-90 at orders 5 to 8 (tests/data, in test_gpu_parity) and its top lists
+of synthetic sets at orders 7 to 9 (test_search_synthetic_golden)."""
 import math
 
 import numpy as np
@@ -102,9 +103,9 @@ def test_incomplete_data():
     with pytest.raises(fa.FiuError):
         fa.GpuSearch().run_arrays(d, 6, fa.Distribution(12, 5), 10)
     with pytest.raises(fa.FiuError):
-        fa.GpuSearch().run_arrays(d, 9, fa.Distribution(12, 8), 10)   # above order 8
-    # orders 7 and 8 count every cell: incomplete data is scanned exactly
-    for order in (7, 8):
+        fa.GpuSearch().run_arrays(d, 10, fa.Distribution(12, 9), 10)   # above order 9
+    # orders 7 to 9 count every cell: incomplete data is scanned exactly
+    for order in (7, 8, 9):
         print(f"case: incomplete data order {order}", flush=True)
         s = fa.GpuSearch()
         c, m = s.run_arrays(d, order, fa.Distribution(12, order - 1), 40)
@@ -120,9 +121,11 @@ def test_incomplete_data():
     (8, 11, 300, 280, 165, 54),     # every combination (C(11, 8) = 165)
     (8, 10, 1000, 900, 30, 55),     # 8 + 8 quads per row
     (8, 8, 64, 64, 1, 56),          # the single combination, whole words
+    (9, 11, 300, 260, 55, 58),      # every combination (C(11, 9) = 55)
+    (9, 9, 64, 64, 1, 59),          # the single combination
 ])
 def test_all_cells_orders_vs_oracle(order, snps, cases, ctrls, outputs, seed):
-    # orders 7 and 8: every 3^k cell counted per combination (k_scan_any)
+    # orders 7 to 9: every 3^k cell counted per combination (k_scan_any)
     print(f"case: order {order} {snps}x{cases}+{ctrls}", flush=True)
     d, od = synth(snps, cases, ctrls, seed)
     oc, om, _ = O.search(od, order, outputs, threads=4, mi_mode=2)

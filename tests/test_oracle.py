@@ -251,7 +251,7 @@ def test_search_reference_test(tped, threads):
 
 def test_search_matches_reference_top(tped, golden_search):
     # same top-90 sets and values as ThreadedSearch(1) of the reference build
-    for o in (2, 3, 4, 5, 6, 7, 8):
+    for o in (2, 3, 4, 5, 6, 7, 8, 9):
         combos, mi, _ = O.search(tped, o, 90, threads=4, mi_mode=0)
         g = golden_search["top"][str(o)]
         assert {tuple(c) for c in combos} == {tuple(c) for c in g["combos"]}
