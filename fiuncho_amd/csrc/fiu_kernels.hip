@@ -1753,6 +1753,9 @@ __device__ __forceinline__ void mfma3_tile(const DevData &D, const ScanParams &P
             if (steady) wait_vmcnt<4 * (R - 2)>();
             else wait_stages<R - 2, 4>(iters - 1 - it);
             stage_barrier();
+            // the stage (fragment reads, products) issues ahead of the other
+            // wave's epilogue
+            __builtin_amdgcn_s_setprio(1);
             Frag f;
             read(f, it);
             // every wave is past stage it - 1, whose slot now takes stage it +
@@ -1760,6 +1763,7 @@ __device__ __forceinline__ void mfma3_tile(const DevData &D, const ScanParams &P
             const bool more = steady || it + R - 1 < iters;
             if (more) issue_bc(it + R - 1);
             multiply(f);
+            __builtin_amdgcn_s_setprio(0);
             if (more) issue_a(it + R - 1);
         };
         int it = 0;
