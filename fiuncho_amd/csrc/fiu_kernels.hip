@@ -578,34 +578,45 @@ __global__ void __launch_bounds__(kBlock) k_scan_any(const DevData D, const Scan
     for (int c = s[M - 1] + 1; c < D.n; ++c) {
         s[M] = c;
 #pragma unroll
-        for (int pi = 0; pi < PASSES; ++pi) {
+        for (int cls = 0; cls < 2; ++cls) {
+            uint32_t n9[PASSES][9];
 #pragma unroll
-            for (int cls = 0; cls < 2; ++cls) {
-                uint32_t n9[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-                const uint4 *base = D.bits + D.class_off[cls];
-                for (int q = 0; q < D.quads[cls]; ++q) {
-                    // row z of SNP x in quad q: rq[z * n_pad + x]
-                    const uint4 *rq = base + (uint64_t)q * 3 * np;
-                    uint4 pr = rq[g5[0] * np + s[0]];
+            for (int pi = 0; pi < PASSES; ++pi)
 #pragma unroll
-                    for (int i = 1; i < 5; ++i) pr = pr & rq[g5[i] * np + s[i]];
-                    if constexpr (K >= 8) pr = pr & rq[(K == 8 ? pi : pi / 3) * np + s[5]];
-                    if constexpr (K == 9) pr = pr & rq[(pi % 3) * np + s[6]];
+                for (int z = 0; z < 9; ++z) n9[pi][z] = 0;
+            const uint4 *base = D.bits + D.class_off[cls];
+            for (int q = 0; q < D.quads[cls]; ++q) {
+                // row z of SNP x in quad q: rq[z * n_pad + x]; the prefix row
+                // of the first five SNPs once, then every middle genotype
+                const uint4 *rq = base + (uint64_t)q * 3 * np;
+                uint4 pr = rq[g5[0] * np + s[0]];
+#pragma unroll
+                for (int i = 1; i < 5; ++i) pr = pr & rq[g5[i] * np + s[i]];
+                uint4 ra[3], rb[3];
+#pragma unroll
+                for (int z = 0; z < 3; ++z) ra[z] = rq[z * np + s[K - 2]], rb[z] = rq[z * np + s[K - 1]];
+#pragma unroll
+                for (int pi = 0; pi < PASSES; ++pi) {
+                    uint4 pm = pr;
+                    if constexpr (K >= 8) pm = pm & rq[(K == 8 ? pi : pi / 3) * np + s[5]];
+                    if constexpr (K == 9) pm = pm & rq[(pi % 3) * np + s[6]];
 #pragma unroll
                     for (int z1 = 0; z1 < 3; ++z1) {
-                        const uint4 pz = pr & rq[z1 * np + s[K - 2]];
+                        const uint4 pz = pm & ra[z1];
 #pragma unroll
                         for (int z2 = 0; z2 < 3; ++z2) {
-                            const uint4 v = pz & rq[z2 * np + s[K - 1]];
-                            n9[3 * z1 + z2] += __builtin_popcount(v.x) + __builtin_popcount(v.y) +
-                                               __builtin_popcount(v.z) + __builtin_popcount(v.w);
+                            const uint4 v = pz & rb[z2];
+                            n9[pi][3 * z1 + z2] += __builtin_popcount(v.x) + __builtin_popcount(v.y) +
+                                                   __builtin_popcount(v.z) + __builtin_popcount(v.w);
                         }
                     }
                 }
-                if (tid < 243) {
+            }
+            if (tid < 243) {
 #pragma unroll
-                    for (int z = 0; z < 9; ++z) cnt[cls][(tid * PASSES + pi) * 9 + z] = n9[z];
-                }
+                for (int pi = 0; pi < PASSES; ++pi)
+#pragma unroll
+                    for (int z = 0; z < 9; ++z) cnt[cls][(tid * PASSES + pi) * 9 + z] = n9[pi][z];
             }
         }
         __syncthreads();
