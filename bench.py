@@ -75,11 +75,20 @@ def slice_combos(n, order, step, offset):
     return total
 
 
+def threaded_combos(n, order, step, offset, threads):
+    """Combinations ThreadedSearch(threads) evaluates on Distribution(n,
+    order-1, step, offset): thread t scans layer(threads, t) = (step *
+    threads, offset * threads + t) (ThreadedSearch.h:219-239,
+    Distribution.h:172-176)."""
+    return sum(slice_combos(n, order, step * threads, offset * threads + t) for t in range(threads))
+
+
 def cpu_baseline(words, raw_path, cw, tw, threads, sample_step):
-    """Reference ThreadedSearch (or the oracle port) on prefix sample 0; with
-    the reference build, also the reference loader's time on the .raw."""
+    """Reference ThreadedSearch (or the oracle port, which splits the work
+    the same way) on Distribution(n, 2, sample_step, 0); with the reference
+    build, also the reference loader's time on the .raw."""
     import oracle
-    combos = slice_combos(N_SNPS, ORDER, sample_step, 0)
+    combos = threaded_combos(N_SNPS, ORDER, sample_step, 0, threads)
     ref = oracle.ref("avx512")
     read_s = None
     if ref is not None and oracle.has_avx512():
@@ -99,11 +108,48 @@ def cpu_baseline(words, raw_path, cw, tw, threads, sample_step):
     return {"value": combos * (N_CASES + N_CTRLS) / sec, "unit": "combo*samples/s", "cores": threads,
             "kind": kind, "reference_read_s": read_s,
             "sample": f"Distribution({N_SNPS}, {ORDER - 1}, {sample_step}, 0) of the same dataset "
-                      f"(read from a PLINK .raw by the reference loader): "
+                      f"(read from a PLINK .raw by the reference loader), whose {threads} threads scan its "
+                      f"layers ({sample_step * threads}, t), t < {threads}: "
                       f"{combos} of {math.comb(N_SNPS, ORDER)} triples, {sec:.1f} s, "
                       f"ThreadedSearch({threads}) default AVX-512 build" if kind == "reference" else
-                      f"Distribution({N_SNPS}, {ORDER - 1}, {sample_step}, 0): {combos} triples, {sec:.1f} s, "
-                      f"oracle AVX-512 restatement, {threads} threads"}
+                      f"Distribution({N_SNPS}, {ORDER - 1}, {sample_step}, 0), layers ({sample_step * threads}, t), "
+                      f"t < {threads}: {combos} triples, {sec:.1f} s, oracle AVX-512 restatement, {threads} threads"}
+
+
+def mi_error(ds, geno, combos, mi):
+    """MI of the headline top list against the reference's own FP32 value
+    (MutualInformation<float>::compute of the same counts, from the reference
+    build or the oracle's bit-exact restatement of it) and against FP64; the
+    GPU's counts of those combinations against a count of the genotypes."""
+    import fiuncho_amd as fa
+    import oracle
+    counts, _ = fa.contingency_batch(ds, combos)
+    is_case = np.arange(N_CASES + N_CTRLS) < N_CASES
+    ref = oracle.ref("avx512") if oracle.has_avx512() else None
+    rel, ab, exact = [], [], True
+    for i, c in enumerate(combos):
+        cell = geno[c[0]].astype(np.int64) * 9 + geno[c[1]] * 3 + geno[c[2]]
+        cc = np.bincount(cell[is_case], minlength=27)
+        ct = np.bincount(cell[~is_case], minlength=27)
+        exact &= bool(np.array_equal(counts[i, 0, :27], cc) and np.array_equal(counts[i, 1, :27], ct))
+        n = float(N_CASES + N_CTRLS)
+        pc, pt = cc / n, ct / n
+        px = pc + pt
+
+        def h(p):
+            p = p[p > 0]
+            return float(-(p * np.log(p)).sum())
+
+        m64 = h(px) + h(np.array([N_CASES / n, N_CTRLS / n])) - h(np.r_[pc, pt])
+        r = ref.mi(cc, ct, 3, N_CASES, N_CTRLS) if ref is not None else \
+            oracle.mi_f32_ref512(cc, ct, 3, N_CASES, N_CTRLS)
+        rel.append(abs(float(mi[i]) - r) / abs(r))
+        ab.append(abs(float(mi[i]) - m64))
+    return {"results": len(combos), "counts_exact": exact,
+            "vs_reference_fp32": {"max_rel": max(rel), "median_rel": float(np.median(rel)),
+                                  "source": "reference build (oracle/_ref)" if ref is not None else
+                                            "oracle restatement of the reference's FP32 order"},
+            "vs_fp64": {"max_abs": max(ab), "median_abs": float(np.median(ab))}}
 
 
 # ---------------------------------------------------------------------------
@@ -369,8 +415,11 @@ def bench_serve(count=300, load=0.7):
         if dt > 0:
             time.sleep(dt)
         sub.append(srv.submit(**r))
-    lat_served = [srv.wait(i)[2].latency_ms for i in sub]
+    stream_res = [srv.wait(i) for i in sub]
+    lat_served = [x[2].latency_ms for x in stream_res]
     t_stream_served = time.perf_counter() - t0
+    same = same and all(a[0].tolist() == b[0].tolist() and np.array_equal(a[1], b[1])
+                        for a, b in zip(plain_res, stream_res))
     srv.close()
     print(f"[bench] serve: served stream p50 {percentile(lat_served, 50):.1f} ms, "
           f"p99 {percentile(lat_served, 99):.1f} ms", file=sys.stderr, flush=True)
@@ -538,7 +587,7 @@ def main():
     # load the code; then the timed steps
     for w in range(args.warmup):
         search.run_arrays(ds, ORDER, fa.Distribution(N_SNPS, 2, 4096, 4095 - w), OUTPUTS)
-    results = []
+    results = {}   # combination -> MI (whole-scan steps repeat the same list)
     combos = 0
     scan_ms = 0.0
     launches = 0
@@ -551,11 +600,10 @@ def main():
         launches += st.launches
         used_algo = st.algorithm
         combos += st.combinations
-        results += list(zip(map(tuple, c.tolist()), m.tolist()))
+        results.update(zip(map(tuple, c.tolist()), m.tolist()))
         print(f"[bench] step {i} done: {st.scan_ms / 1e3:.2f} s device", file=sys.stderr, flush=True)
     elapsed = time.perf_counter() - t_start
-    results.sort(key=lambda r: (-r[1], r[0]))
-    top = results[:OUTPUTS]
+    top = sorted(results.items(), key=lambda r: (-r[1], r[0]))[:OUTPUTS]
     unit_samples = N_CASES + N_CTRLS
     value = combos * unit_samples / elapsed
     algo_name = {1: "direct", 2: "marginal", 3: "mfma"}.get(used_algo, str(used_algo))
@@ -600,6 +648,10 @@ def main():
         "roofline": roofline,
         "top1": {"combination": list(top[0][0]), "mi": top[0][1]} if top else None,
     }
+    if not args.no_cpu and top:
+        # the reference's FP32 MI and an FP64 MI of the headline top list
+        line["mi_error"] = mi_error(ds, geno, np.array([t[0] for t in top], np.int32),
+                                    np.array([t[1] for t in top], np.float32))
     ds.close()
     if not args.no_order2:
         line["config1"] = bench_config1()

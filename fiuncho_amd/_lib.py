@@ -33,6 +33,7 @@ EXPORTS = (
     "fiu_batch_config_init", "fiu_search_batch", "fiu_server_create", "fiu_server_submit",
     "fiu_server_wait", "fiu_server_query", "fiu_server_destroy", "fiu_set_tuning", "fiu_get_tuning",
     "fiu_synth_spec_init", "fiu_synth_generate", "fiu_dataset_synth",
+    "fiu_result_str", "fiu_result_serialize", "fiu_result_deserialize", "fiu_write_results",
 )
 
 
@@ -161,6 +162,12 @@ def lib():
     L.fiu_synth_spec_init.restype = None
     L.fiu_synth_generate.argtypes = [C.POINTER(SynthSpec), vp, vp, C.POINTER(sz), C.POINTER(sz)]
     L.fiu_dataset_synth.argtypes = [C.POINTER(SynthSpec), C.POINTER(vp)]
+    L.fiu_result_str.restype = sz
+    L.fiu_result_str.argtypes = [vp, u16, C.c_float, C.c_char_p, sz]
+    L.fiu_result_serialize.restype = sz
+    L.fiu_result_serialize.argtypes = [vp, u16, C.c_float, vp, sz]
+    L.fiu_result_deserialize.argtypes = [vp, sz, vp, u16, C.POINTER(u16), C.POINTER(C.c_float), C.POINTER(sz)]
+    L.fiu_write_results.argtypes = [C.c_char_p, vp, vp, u32, u16]
     _lib = L
     return L
 

@@ -49,19 +49,16 @@ uint64_t binom_sat(uint64_t n, uint64_t k)
     return (uint64_t)r;
 }
 
-int bit_width(uint64_t x)
-{
-    int b = 0;
-    while (x) ++b, x >>= 1;
-    return std::max(b, 1);
-}
-
 }  // namespace fiu
 
 fiu_dataset::~fiu_dataset()
 {
     for (auto e : events) (void)hipEventDestroy(e);
     if (stream && own_stream) (void)hipStreamDestroy(stream);
+    for (auto &r : retired) {
+        if (r.second) r.second->release(r.first);
+        else (void)hipFree(r.first);
+    }
 }
 
 namespace fiu {
@@ -302,7 +299,7 @@ void host_singles(const uint64_t *words, size_t snps, size_t cw, size_t tw, uint
 size_t staged_host_bytes(size_t snps, size_t cw, size_t tw, int order)
 {
     const Shape sh = shape_of(snps, cw, tw);
-    const size_t M = (size_t)std::max(order - 1, 1);
+    const size_t M = (size_t)std::max(order, 2);   // binomial rows up to k (prefix ranks and keys)
     return align_up(snps * 3 * (cw + tw) * 8) + align_up((size_t)sh.n_pad * 24) + align_up((M + 1) * (snps + 1) * 8);
 }
 
@@ -315,7 +312,7 @@ fiu_dataset *stage_dataset(const uint64_t *words, size_t snps, size_t cw, size_t
         fail(FIU_ERR_INVALID, "more samples than bits per row");
     if (snps > (size_t)INT32_MAX - 64) fail(FIU_ERR_UNSUPPORTED, "more SNPs than INT_MAX");
     const Shape sh = shape_of(snps, cw, tw);
-    const int M = std::max(order - 1, 1);
+    const int M = std::max(order, 2);   // binomial rows up to k (prefix ranks and keys)
     up = StagedUpload();
     up.n_words = snps * 3 * (cw + tw);
     up.off_singles = align_up(up.n_words * 8);
@@ -399,8 +396,15 @@ fiu_dataset *stage_dataset(const uint64_t *words, size_t snps, size_t cw, size_t
 void enqueue_upload(fiu_dataset *d, StagedUpload &up, hipStream_t s)
 {
     const Shape sh = shape_of(d->snps, d->cw, d->tw);
-    if (up.n_words)
+    // every copy below stays inside its pinned block or device buffer
+    check_range(up.host + up.off_singles, (size_t)sh.n_pad * 24, up.host, up.host_bytes, "row-count upload");
+    check_range(up.host + up.off_binom, up.binom_n * 8, up.host, up.host_bytes, "binomial upload");
+    check_range(d->singles.p, (size_t)sh.n_pad * 24, d->singles.p, d->singles.n * 4, "row-count upload");
+    check_range(d->binom.p, up.binom_n * 8, d->binom.p, d->binom.n * 8, "binomial upload");
+    if (up.n_words) {
+        check_range(up.host, up.n_words * 8, up.host, up.host_bytes, "table upload");
         check(hipMemcpyAsync(up.staging_d, up.host, up.n_words * 8, hipMemcpyHostToDevice, s), "upload");
+    }
     check(launch_transpose(up.staging_d, d->bits.p, sh.n, sh.n_pad, 0, d->cw, d->tw, sh.q0, 0, s), "transpose");
     check(launch_transpose(up.staging_d, d->bits.p, sh.n, sh.n_pad, 1, d->cw, d->tw, sh.q1, sh.off1, s), "transpose");
     check(hipMemcpyAsync(d->singles.p, up.host + up.off_singles, (size_t)sh.n_pad * 24, hipMemcpyHostToDevice, s),
@@ -417,8 +421,6 @@ void enqueue_upload(fiu_dataset *d, StagedUpload &up, hipStream_t s)
         check(launch_pair_table(d->dev, P, d->pair.p, s), "pair table launch");
     }
     if (up.fp4) check(launch_fp4_image(d->dev, d->fp4.p, s), "fp4 image launch");
-    d->arena->release(up.staging_d);
-    up.staging_d = nullptr;
 }
 
 namespace {
@@ -466,23 +468,27 @@ void mi_constants(uint32_t cases, uint32_t ctrls, bool complete, ScanParams &P)
     P.delta = 2e-5f;
 }
 
-void ensure_binom(fiu_dataset *d, int m, hipStream_t s)
+const uint64_t *ensure_binom(fiu_dataset *d, int m, hipStream_t s)
 {
     std::lock_guard<std::mutex> lk(d->prep_mu);
-    if (d->binom_m >= m) return;
+    if (d->binom_m >= m) return d->binom.p;
     const uint64_t n = d->snps, stride = n + 1;
     std::vector<uint64_t> h((uint64_t)(m + 1) * stride);
     for (int j = 0; j <= m; ++j)
         for (uint64_t x = 0; x <= n; ++x) h[j * stride + x] = binom_sat(x, (uint64_t)j);
+    // scans queued with the smaller table (another thread's, or the scan
+    // server's) may still read it: it is kept until the dataset goes
+    if (d->binom.p) d->retired.emplace_back((void *)d->binom.detach(), d->binom.arena);
     d->binom.alloc(h.size());
     check(hipMemcpyAsync(d->binom.p, h.data(), h.size() * 8, hipMemcpyHostToDevice, s), "binom upload");
     check(hipStreamSynchronize(s), "binom sync");
     d->binom_m = m;
+    return d->binom.p;
 }
 
 bool ensure_pair_table(fiu_dataset *d, hipStream_t s)
 {
-    ensure_binom(d, 1, s);
+    const uint64_t *binom = ensure_binom(d, 1, s);
     std::lock_guard<std::mutex> lk(d->prep_mu);
     if (d->pair_ready) return true;
     const uint64_t n = d->snps, pairs = n * (n - 1) / 2;
@@ -500,7 +506,7 @@ bool ensure_pair_table(fiu_dataset *d, hipStream_t s)
     P.step = 1;
     P.offset = 0;
     P.total_prefixes = n;
-    P.binom = d->binom.p;
+    P.binom = binom;
     check(launch_pair_table(d->dev, P, d->pair.p, s), "pair table launch");
     check(hipStreamSynchronize(s), "pair table sync");
     d->pair_ready = true;
@@ -528,33 +534,37 @@ bool ensure_fp4_image(fiu_dataset *d, hipStream_t s)
 
 // The prefixes that share their first m-1 indices form a run of consecutive
 // ranks whose last index increases by one; each run is summed in closed form.
+// The (m-1)-combinations heading the runs are walked in lexicographic order
+// (O(C(n, m-1)) steps, no allocation).
 uint64_t count_combinations(uint64_t n, int m, uint64_t step, uint64_t offset)
 {
+    constexpr int kMaxM = 32;
+    if (m < 1 || m > kMaxM) return 0;
     uint64_t total = 0, rank = 0;
-    std::vector<int64_t> c(m);
-    // iterate the (m-1)-prefixes of the m-prefixes in lexicographic order
-    std::function<void(int, int64_t)> rec = [&](int depth, int64_t start) {
-        if (depth == m - 1) {
-            const int64_t first_last = depth ? c[depth - 1] + 1 : 0;
-            if (first_last >= (int64_t)n) return;
-            const uint64_t L = n - (uint64_t)first_last;  // runs of last = first_last .. n-1
+    int64_t c[kMaxM];
+    const int h = m - 1;   // indices that head a run
+    for (int i = 0; i < h; ++i) c[i] = i;
+    if (h > 0 && (uint64_t)h > n) return 0;
+    for (;;) {
+        const int64_t first_last = h ? c[h - 1] + 1 : 0;
+        if (first_last < (int64_t)n) {
+            const uint64_t L = n - (uint64_t)first_last;   // runs of last = first_last .. n-1
             // first local index whose rank is offset + j*step, j >= 0
-            const uint64_t first =
-                rank >= offset ? (step - (rank - offset) % step) % step : offset - rank;
+            const uint64_t first = rank >= offset ? (step - (rank - offset) % step) % step : offset - rank;
             if (first < L) {
                 const uint64_t cnt = (L - 1 - first) / step + 1, t_last = first + (cnt - 1) * step;
                 // combos for local index t: n - 1 - (first_last + t) = L - 1 - t
                 total += cnt * (L - 1) - (first + t_last) * cnt / 2;
             }
             rank += L;
-            return;
         }
-        for (int64_t v = start; v < (int64_t)n; ++v) {
-            c[depth] = v;
-            rec(depth + 1, v + 1);
-        }
-    };
-    if (m >= 1) rec(0, 0);
+        // next h-combination of [0, n) in lexicographic order
+        int i = h - 1;
+        while (i >= 0 && c[i] == (int64_t)n - h + i) --i;
+        if (i < 0) break;
+        ++c[i];
+        for (int j = i + 1; j < h; ++j) c[j] = c[j - 1] + 1;
+    }
     return total;
 }
 
@@ -568,8 +578,9 @@ namespace {
 // reverse lexicographic order (at most 2^26 steps).
 int64_t tail_prefixes(int n, int M, uint64_t step, uint64_t offset, uint64_t total, int64_t n_pref, uint64_t target)
 {
-    if (n_pref <= 0 || total == 0) return 0;
-    std::vector<int> c(M);
+    constexpr int kMaxM = 31;
+    if (n_pref <= 0 || total == 0 || M < 1 || M > kMaxM) return 0;
+    int c[kMaxM];
     for (int i = 0; i < M; ++i) c[i] = n - M + i;   // the last prefix
     uint64_t rank = total - 1, acc = 0;
     int64_t m = 0;
@@ -601,8 +612,138 @@ struct Chunk {
     int64_t j0, j1;
 };
 
-void search(fiu_dataset *d, int order, int32_t step, int32_t offset, uint32_t outputs, const fiu_config *cfg,
-            int32_t *out_combos, float *out_mi, uint32_t *out_count, fiu_search_stats *stats)
+// The launches of one search, computed on demand (no plan is stored):
+//   uniform:   [ci len, (ci + 1) len) of `units` (the distribution's
+//              prefixes, or the order-2 matrix-core super-blocks);
+//   first SNP: order 3 on the matrix cores, first SNPs [a(ci), a(ci + 1)),
+//              a(i) = the first a whose first SNPs 0 .. a - 1 hold at least
+//              i x target triples, so that each chunk holds about `target`
+//              triples (a chunk can be empty: one first SNP above target).
+struct ChunkPlan {
+    bool first_snp = false;
+    int64_t units = 0, len = 1;
+    int64_t n = 0;
+    unsigned __int128 target = 1, triples = 0;
+    uint64_t count = 0;
+
+    static unsigned __int128 c3(int64_t m) { return m < 3 ? 0 : (unsigned __int128)m * (m - 1) * (m - 2) / 6; }
+    void set_uniform(int64_t u, int64_t l)
+    {
+        first_snp = false;
+        units = u;
+        len = std::max<int64_t>(1, l);
+        count = u > 0 ? (uint64_t)((u + len - 1) / len) : 0;
+    }
+    void set_first_snp(int64_t snps, uint64_t t)
+    {
+        first_snp = true;
+        n = snps;
+        triples = c3(n);
+        target = std::max<uint64_t>(t, 1);
+        count = n >= 3 ? (uint64_t)((triples + target - 1) / target) : 0;
+    }
+    int64_t boundary(uint64_t i) const
+    {
+        if (i >= count) return n - 2;
+        const unsigned __int128 want = (unsigned __int128)i * target;
+        int64_t lo = 0, hi = n - 2;   // triples with first SNP below a: C(n, 3) - C(n - a, 3)
+        while (lo < hi) {
+            const int64_t mid = lo + (hi - lo) / 2;
+            if (triples - c3(n - mid) >= want) hi = mid;
+            else lo = mid + 1;
+        }
+        return lo;
+    }
+    Chunk at(uint64_t ci) const
+    {
+        if (first_snp) return {boundary(ci), boundary(ci + 1)};
+        const int64_t j0 = (int64_t)ci * len;
+        return {j0, std::min(units, j0 + len)};
+    }
+};
+
+// Device memory one search works in, and what it may build.
+class Workspace {
+  public:
+    virtual ~Workspace() = default;
+    // binomial table C(x, j), x <= snps, j <= m
+    virtual const uint64_t *binom(fiu_dataset *d, int m, hipStream_t s) = 0;
+    // the dataset's pair table / fp4 image, built when missing if allowed
+    virtual bool pair_table(fiu_dataset *d, hipStream_t s) = 0;
+    virtual bool fp4_image(fiu_dataset *d, hipStream_t s) = 0;
+    virtual uint32_t max_candidates() const = 0;
+    virtual uint64_t max_chunks() const = 0;
+    // candidate buffer (cap entries), top list (max(outputs, kMaxOutputs)),
+    // state, redo list (chunks entries), histogram (kHistBins)
+    virtual void buffers(fiu_dataset *d, uint32_t cap, uint64_t chunks, uint32_t outputs, Cand *&cand, Cand *&top,
+                         TopState *&state, uint32_t *&redo, uint32_t *&hist) = 0;
+    // outputs above kMaxOutputs: the merge's sort buffer of `entries`
+    virtual Cand *merge_work(fiu_dataset *d, size_t entries) = 0;
+    virtual void zero(uint32_t *p, size_t n, hipStream_t s) = 0;
+    // orders 10-12: the count slots of the persistent all-cells kernel
+    virtual uint32_t *big_cells(fiu_dataset *d, int order) = 0;
+    // timing event i, or null (the search is not timed)
+    virtual hipEvent_t event(fiu_dataset *d, size_t i) = 0;
+    // the final top list to the caller's arrays
+    virtual void results(const Cand *top, uint32_t count, int order, int n, const uint64_t *binom, int32_t *out_combos,
+                         float *out_mi, hipStream_t s) = 0;
+};
+
+// fiu_search: the dataset's own buffers, grown on demand and kept for later
+// calls; the pair table and the fp4 image are built when missing.
+class DatasetWorkspace final : public Workspace {
+  public:
+    const uint64_t *binom(fiu_dataset *d, int m, hipStream_t s) override { return ensure_binom(d, m, s); }
+    bool pair_table(fiu_dataset *d, hipStream_t s) override { return ensure_pair_table(d, s); }
+    bool fp4_image(fiu_dataset *d, hipStream_t s) override { return ensure_fp4_image(d, s); }
+    uint32_t max_candidates() const override { return UINT32_MAX; }
+    uint64_t max_chunks() const override { return UINT64_MAX; }
+    void buffers(fiu_dataset *d, uint32_t cap, uint64_t chunks, uint32_t outputs, Cand *&cand, Cand *&top,
+                 TopState *&state, uint32_t *&redo, uint32_t *&hist) override
+    {
+        d->cand.alloc(cap);
+        d->top.alloc(std::max<uint32_t>(outputs, kMaxOutputs));
+        d->state.alloc(1);
+        d->redo.alloc(std::max<uint64_t>(chunks, 1));
+        d->hist.alloc(kHistBins);
+        cand = d->cand.p, top = d->top.p, state = d->state.p, redo = d->redo.p, hist = d->hist.p;
+    }
+    void zero(uint32_t *p, size_t n, hipStream_t s) override
+    {
+        check(hipMemsetAsync(p, 0, n * sizeof(uint32_t), s), "clear");
+    }
+    uint32_t *big_cells(fiu_dataset *d, int order) override
+    {
+        d->big.alloc(big_cells_bytes(order) / sizeof(uint32_t));
+        return d->big.p;
+    }
+    Cand *merge_work(fiu_dataset *d, size_t entries) override
+    {
+        d->mwork.alloc(entries);
+        return d->mwork.p;
+    }
+    hipEvent_t event(fiu_dataset *d, size_t i) override
+    {
+        while (d->events.size() <= i) {
+            hipEvent_t e;
+            check(hipEventCreate(&e), "hipEventCreate");
+            d->events.push_back(e);
+        }
+        return d->events[i];
+    }
+    void results(const Cand *top, uint32_t count, int order, int n, const uint64_t *, int32_t *out_combos,
+                 float *out_mi, hipStream_t s) override
+    {
+        std::vector<Cand> h(count);
+        if (count) check(hipMemcpyAsync(h.data(), top, count * sizeof(Cand), hipMemcpyDeviceToHost, s), "top read");
+        check(hipStreamSynchronize(s), "top read sync");
+        unpack_results(h.data(), count, order, (uint64_t)n, out_combos, out_mi);
+    }
+};
+
+// Exhaustive search of `d` in workspace `ws` (fiu_search).
+void search(fiu_dataset *d, Workspace &ws, int order, int32_t step, int32_t offset, uint32_t outputs,
+            const fiu_config *cfg, int32_t *out_combos, float *out_mi, uint32_t *out_count, fiu_search_stats *stats)
 {
     const auto t_start = std::chrono::steady_clock::now();
     if (!out_count) fail(FIU_ERR_INVALID, "out_count is NULL");
@@ -612,8 +753,7 @@ void search(fiu_dataset *d, int order, int32_t step, int32_t offset, uint32_t ou
     if (offset < 0) fail(FIU_ERR_INVALID, "distribution offset must be >= 0");
     if (outputs > 0 && (!out_combos || !out_mi)) fail(FIU_ERR_INVALID, "output pointers are NULL");
     if (order > kMaxOrder) fail(FIU_ERR_UNSUPPORTED, "order " + std::to_string(order) + " is above this build's limit");
-    if (outputs > (uint32_t)kMaxOutputs)
-        fail(FIU_ERR_UNSUPPORTED, "outputs above " + std::to_string(kMaxOutputs));
+    if (outputs > kMaxOutputsBig) fail(FIU_ERR_UNSUPPORTED, "outputs above " + std::to_string(kMaxOutputsBig));
     int algo_req = cfg ? cfg->algorithm : FIU_ALGO_AUTO;
     int variant = cfg && cfg->struct_size >= offsetof(fiu_config, variant) + sizeof(int32_t) ? cfg->variant : 0;
     if (algo_req < FIU_ALGO_AUTO || algo_req > FIU_ALGO_MFMA) fail(FIU_ERR_INVALID, "unknown algorithm");
@@ -621,13 +761,14 @@ void search(fiu_dataset *d, int order, int32_t step, int32_t offset, uint32_t ou
         fail(FIU_ERR_INVALID, "marginal algorithm needs every SNP to cover the same samples");
     if (algo_req == FIU_ALGO_MFMA && order != 2 && order != 3)
         fail(FIU_ERR_UNSUPPORTED, "the matrix-core scan covers orders 2 and 3");
-    if (order > kMaxDirectOrder && order <= kMaxKmOrder && (algo_req == FIU_ALGO_DIRECT || !d->dev.uniform))
-        fail(FIU_ERR_UNSUPPORTED, "order " + std::to_string(order) +
-                                      " is scanned marginally only: every SNP must cover the same samples");
     if (order >= kMinAnyOrder && algo_req == FIU_ALGO_MARGINAL)
-        fail(FIU_ERR_UNSUPPORTED, "orders 7 to 9 count every cell (FIU_ALGO_DIRECT)");
+        fail(FIU_ERR_UNSUPPORTED, "orders 7 to 12 count every cell (FIU_ALGO_DIRECT)");
     if (algo_req != FIU_ALGO_AUTO && !valid_variant(algo_req, order, variant))
         fail(FIU_ERR_INVALID, "unknown variant " + std::to_string(variant) + " of this algorithm");
+    const uint32_t cap = cfg && cfg->candidate_cap ? cfg->candidate_cap : std::min<uint32_t>(1u << 20, ws.max_candidates());
+    if (cap > ws.max_candidates())
+        fail(FIU_ERR_UNSUPPORTED, "candidate_cap above this entry point's limit of " +
+                                      std::to_string(ws.max_candidates()));
     // AUTO: the tuned choice of this shape class when the dataset allows it
     // (order-3 matrix-core entries are timed on whole scans)
     bool from_table = false;
@@ -648,8 +789,10 @@ void search(fiu_dataset *d, int order, int32_t step, int32_t offset, uint32_t ou
         if (stats) *stats = st;
         return;
     }
-    const int bits = bit_width((uint64_t)n - 1);
-    if (bits * order > 64) fail(FIU_ERR_UNSUPPORTED, "combination does not pack in 64 bits");
+    // candidates are keyed by their lexicographic rank among the C(n, k)
+    // combinations (below the empty marker ~0)
+    if (binom_sat((uint64_t)n, (uint64_t)order) >= UINT64_MAX - 1)
+        fail(FIU_ERR_UNSUPPORTED, "more than 2^64 - 2 combinations: their ranks do not fit the 64-bit key");
     const uint64_t total_prefixes = binom_sat((uint64_t)n, (uint64_t)M);
     if (total_prefixes == UINT64_MAX) fail(FIU_ERR_UNSUPPORTED, "too many prefixes");
     const uint64_t n_pref = total_prefixes > (uint64_t)offset
@@ -657,14 +800,14 @@ void search(fiu_dataset *d, int order, int32_t step, int32_t offset, uint32_t ou
                                 : 0;
     hipStream_t s = stream_of(d, cfg);
     check(hipSetDevice(d->device), "hipSetDevice");
-    ensure_binom(d, M, s);
+    const uint64_t *binom = ws.binom(d, order, s);   // rows up to k: prefix ranks and combination keys
     int algo = FIU_ALGO_DIRECT;
     const bool small_classes = d->cases < (1u << 24) && d->ctrls < (1u << 24);   // fp32-exact counts
     if (algo_req != FIU_ALGO_DIRECT && d->dev.uniform && order >= 4 && order <= kMaxKmOrder)
         algo = FIU_ALGO_MARGINAL;
     if (algo_req != FIU_ALGO_DIRECT && d->dev.uniform && (order == 2 || order == 3)) {
         algo = FIU_ALGO_MARGINAL;
-        if (order == 3 && !ensure_pair_table(d, s)) algo = FIU_ALGO_DIRECT;  // table does not fit
+        if (order == 3 && !ws.pair_table(d, s)) algo = FIU_ALGO_DIRECT;  // table does not fit
         // order 2: the matrix-core scan is the faster exact path (4.4x on
         // 50,000 SNPs x 32,000 samples, also ahead on small inputs)
         if (order == 2 && algo_req != FIU_ALGO_MARGINAL) algo = FIU_ALGO_MFMA;
@@ -672,7 +815,7 @@ void search(fiu_dataset *d, int order, int32_t step, int32_t offset, uint32_t ou
         // so by default it takes whole scans (a round-robin share of the
         // prefixes is masked, which wastes the other shares' work)
         if (order == 3 && algo == FIU_ALGO_MARGINAL && small_classes &&
-            (algo_req == FIU_ALGO_MFMA || (algo_req == FIU_ALGO_AUTO && step == 1)) && ensure_fp4_image(d, s))
+            (algo_req == FIU_ALGO_MFMA || (algo_req == FIU_ALGO_AUTO && step == 1)) && ws.fp4_image(d, s))
             algo = FIU_ALGO_MFMA;
     }
     if (algo != algo_req && from_table) variant = 0;   // the tuned algorithm did not apply
@@ -683,50 +826,39 @@ void search(fiu_dataset *d, int order, int32_t step, int32_t offset, uint32_t ou
     const int ring = algo == FIU_ALGO_MFMA && order == 3 && variant ? variant : 4;
     // matrix-core scans walk super-blocks of 8 x 8 tiles of 64 x 64 pairs;
     // fp4 operands accumulate in fp32, exact while every count is below 2^24;
-    // they come from the dataset's fp4 image (int8 when it does not fit)
+    // they come from the dataset's fp4 image (int8 when it is not there)
     const int T = d->dev.n_pad / 64;
     const int64_t n_super = mfma2_superblocks(T);
     const bool int8 = (cfg && (cfg->flags & FIU_FLAG_MFMA_INT8)) || (order == 2 && variant == 1);
-    const bool mfma_fp4 = algo == FIU_ALGO_MFMA && !int8 && small_classes && ensure_fp4_image(d, s);
+    const bool mfma_fp4 = algo == FIU_ALGO_MFMA && !int8 && small_classes && ws.fp4_image(d, s);
 
-    const uint32_t cap = cfg && cfg->candidate_cap ? cfg->candidate_cap : (1u << 20);
-    d->cand.alloc(cap);
-    d->top.alloc(kMaxOutputs);
-    d->state.alloc(1);
-    // chunk plan: even chunks of about `target` combinations; before the
-    // first, a sampling pass over the last prefixes of the distribution
-    // (about 2^24 combinations, spread over many short wavefronts)
-    // histograms the screening MI and sets a starting threshold, so that the
-    // first chunks push few candidates
+    // chunk plan: even chunks of about `target` combinations (at most
+    // max_chunks of them); before the first, a sampling pass over the last
+    // prefixes of the distribution (about 2^24 combinations, spread over many
+    // short wavefronts) histograms the screening MI and sets a starting
+    // threshold, so that the first chunks push few candidates
     const double per_avg = (double)(n - M) / (double)(M + 1);
     const uint64_t target = cfg && cfg->chunk_combos ? cfg->chunk_combos : (1ull << 28);
-    std::vector<Chunk> chunks;
+    const uint64_t max_chunks = ws.max_chunks();
+    ChunkPlan plan;
     if (algo == FIU_ALGO_MFMA && order == 2) {
         // chunks of super-blocks (262,144 pairs each) instead of prefixes
-        const int64_t len = std::max<int64_t>(1, (int64_t)(target / (64 * 4096)));
-        for (int64_t t = 0; t < n_super; t += len) chunks.push_back({t, std::min(n_super, t + len)});
+        int64_t len = std::max<int64_t>(1, (int64_t)(target / (64 * 4096)));
+        len = std::max<int64_t>(len, (int64_t)(((uint64_t)n_super + max_chunks - 1) / max_chunks));
+        plan.set_uniform(n_super, len);
     } else if (algo == FIU_ALGO_MFMA) {
-        // chunks of first SNPs a, about `target` triples (a, b, c) each
-        int64_t a = 0;
-        while (a < n - 2) {
-            uint64_t acc = 0;
-            int64_t e = a;
-            while (e < n - 2 && (acc < target || e == a) && e - a < 65535) {
-                acc += binom_sat((uint64_t)(n - 1 - e), 2);
-                ++e;
-            }
-            chunks.push_back({a, e});
-            a = e;
-        }
+        // chunks of first SNPs a, about `target` triples (a, b, c) each; at
+        // most 2^36 triples keep a launch's grid within its limit
+        uint64_t t = std::min<uint64_t>(target, 1ull << 36);
+        const unsigned __int128 all = ChunkPlan::c3(n);
+        if (all / t >= max_chunks) t = (uint64_t)(all / max_chunks + 1);
+        plan.set_first_snp(n, t);
     } else {
-        int64_t j = 0;
-        const int64_t len = (int64_t)std::max<double>(1.0, (double)target / std::max(per_avg, 1.0));
-        while ((uint64_t)j < n_pref) {
-            const int64_t e = std::min<int64_t>((int64_t)n_pref, j + len);
-            chunks.push_back({j, e});
-            j = e;
-        }
+        int64_t len = (int64_t)std::max<double>(1.0, (double)target / std::max(per_avg, 1.0));
+        len = std::max<int64_t>(len, (int64_t)((n_pref + max_chunks - 1) / max_chunks));
+        plan.set_uniform((int64_t)n_pref, len);
     }
+    if (plan.count > max_chunks) fail(FIU_ERR_UNSUPPORTED, "too many launches for this entry point");
     const bool sample = !(cfg && (cfg->flags & FIU_FLAG_NO_THRESHOLD_SAMPLE));
     // at most about 1/16 of the scan's combinations (the pass screens every
     // combination it samples with a popcount kernel), 1/256 ahead of a
@@ -736,101 +868,132 @@ void search(fiu_dataset *d, int order, int32_t step, int32_t offset, uint32_t ou
     const uint64_t sample_cap = std::max<uint64_t>(order >= kMinAnyOrder ? 4096 : 65536,
                                                    (uint64_t)((double)n_pref * per_avg * sample_frac));
     const int64_t sample_prefixes =
-        chunks.empty() || !sample
+        plan.count == 0 || !sample
             ? 0
             : tail_prefixes(n, M, (uint64_t)step, (uint64_t)offset, total_prefixes, (int64_t)n_pref,
                             std::min<uint64_t>({1ull << 24, target, sample_cap}));
-    d->redo.alloc(std::max<size_t>(chunks.size(), 1));
+    Cand *cand = nullptr, *top = nullptr;
+    TopState *state = nullptr;
+    uint32_t *redo = nullptr, *hist = nullptr;
+    ws.buffers(d, cap, plan.count, outputs, cand, top, state, redo, hist);
+    // more than kMaxOutputs results: the merge runs in global memory
+    const bool big_merge = outputs > (uint32_t)kMaxOutputs;
+    auto pow2_at_least = [](uint64_t x) {
+        uint64_t p = 2;
+        while (p < x) p <<= 1;
+        return p;
+    };
+    Cand *mwork = big_merge ? ws.merge_work(d, pow2_at_least((uint64_t)outputs + cap)) : nullptr;
     TopState init{};
     init.tau = -INFINITY;
     init.tau_combo = ~0ull;
-    check(hipMemcpyAsync(d->state.p, &init, sizeof(init), hipMemcpyHostToDevice, s), "state init");
+    check(hipMemcpyAsync(state, &init, sizeof(init), hipMemcpyHostToDevice, s), "state init");
 
     ScanParams P{};
     P.step = step;
     P.offset = offset;
     P.total_prefixes = total_prefixes;
-    P.binom = d->binom.p;
-    P.cand = d->cand.p;
+    P.binom = binom;
+    P.cand = cand;
     P.cand_cap = cap;
-    P.combo_bits = bits;
+    P.order = order;
     P.tag = 0;
     P.pair = d->pair_ready ? d->pair.p : nullptr;
+    P.cells = order >= kMinBigOrder ? ws.big_cells(d, order) : nullptr;
     mi_constants(d->cases, d->ctrls, d->complete(), P);
 
-    while (d->events.size() < 2 * chunks.size() + 2) {
-        hipEvent_t e;
-        check(hipEventCreate(&e), "hipEventCreate");
-        d->events.push_back(e);
-    }
     size_t ev = 0;
-    auto run_chunk = [&](size_t ci, bool timed) {
-        P.j0 = chunks[ci].j0;
-        P.j1 = chunks[ci].j1;
-        if (timed) check(hipEventRecord(d->events[ev], s), "event");
-        if (algo == FIU_ALGO_MFMA && order == 2) {
-            check(launch_mfma2(d->dev, P, d->state.p, chunks[ci].j0, chunks[ci].j1, T, mfma_fp4 ? d->fp4.p : nullptr,
-                               s),
-                  "scan launch");
-        } else if (algo == FIU_ALGO_MFMA) {
-            check(launch_mfma3(d->dev, P, d->state.p, d->fp4.p, (int)chunks[ci].j0, (int)chunks[ci].j1, T, ring, s),
-                  "scan launch");
-        } else if (algo == FIU_ALGO_MARGINAL && order >= 4) {
-            check(launch_scan_km(order, d->dev, P, d->state.p, s), "scan launch");
-        } else if (algo == FIU_ALGO_MARGINAL) {
-            check(launch_scan_marginal(order, d->dev, P, d->state.p, s, pw), "scan launch");
-        } else {
-            check(launch_scan_direct(order, d->dev, P, d->state.p, s), "scan launch");
+    bool timed = true;
+    auto run_chunk = [&](uint64_t ci, bool time_it) {
+        const Chunk c = plan.at(ci);
+        if (c.j1 <= c.j0) return;   // an empty first-SNP chunk
+        P.j0 = c.j0;
+        P.j1 = c.j1;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (time_it && timed) {
+            e0 = ws.event(d, ev);
+            e1 = ws.event(d, ev + 1);
+            timed = e0 && e1;
         }
-        if (timed) check(hipEventRecord(d->events[ev + 1], s), "event");
-        if (timed) ev += 2;
-        check(launch_merge(d->state.p, d->top.p, d->cand.p, outputs, cap, (uint32_t)ci, d->redo.p, s), "merge launch");
+        if (e0 && e1) check(hipEventRecord(e0, s), "event");
+        if (algo == FIU_ALGO_MFMA && order == 2) {
+            check(launch_mfma2(d->dev, P, state, c.j0, c.j1, T, mfma_fp4 ? d->fp4.p : nullptr, s), "scan launch");
+        } else if (algo == FIU_ALGO_MFMA) {
+            check(launch_mfma3(d->dev, P, state, d->fp4.p, (int)c.j0, (int)c.j1, T, ring, s), "scan launch");
+        } else if (algo == FIU_ALGO_MARGINAL && order >= 4) {
+            check(launch_scan_km(order, d->dev, P, state, s), "scan launch");
+        } else if (algo == FIU_ALGO_MARGINAL) {
+            check(launch_scan_marginal(order, d->dev, P, state, s, pw), "scan launch");
+        } else {
+            check(launch_scan_direct(order, d->dev, P, state, s), "scan launch");
+        }
+        if (e0 && e1) {
+            check(hipEventRecord(e1, s), "event");
+            ev += 2;
+        }
+        if (big_merge) {
+            TopState h;
+            check(hipMemcpyAsync(&h, state, sizeof(h), hipMemcpyDeviceToHost, s), "state read");
+            check(hipStreamSynchronize(s), "merge sync");
+            const uint32_t count = std::min(h.cand_count, cap);
+            check(launch_merge_big(state, top, h.n_top, cand, count, outputs, mwork,
+                                   (uint32_t)pow2_at_least((uint64_t)h.n_top + count), (uint32_t)ci, redo, s),
+                  "merge launch");
+        } else {
+            check(launch_merge(state, top, cand, outputs, cap, (uint32_t)ci, redo, s), "merge launch");
+        }
         ++st.launches;
     };
     if (sample_prefixes > 0) {
-        d->hist.alloc(kHistBins);
-        check(hipMemsetAsync(d->hist.p, 0, kHistBins * sizeof(uint32_t), s), "hist clear");
+        ws.zero(hist, kHistBins, s);
         ScanParams H = P;
         H.j0 = (int64_t)n_pref - sample_prefixes;
         H.j1 = (int64_t)n_pref;
-        H.hist = d->hist.p;
+        H.hist = hist;
         if (algo == FIU_ALGO_MARGINAL && order >= 4)
-            check(launch_scan_km(order, d->dev, H, d->state.p, s), "sample launch");
+            check(launch_scan_km(order, d->dev, H, state, s), "sample launch");
         else if (algo == FIU_ALGO_MARGINAL || algo == FIU_ALGO_MFMA)
-            check(launch_scan_marginal(order, d->dev, H, d->state.p, s), "sample launch");
+            check(launch_scan_marginal(order, d->dev, H, state, s), "sample launch");
         else
-            check(launch_scan_direct(order, d->dev, H, d->state.p, s), "sample launch");
-        check(launch_tau_from_hist(d->hist.p, outputs, P.delta, d->state.p, s), "threshold launch");
+            check(launch_scan_direct(order, d->dev, H, state, s), "sample launch");
+        check(launch_tau_from_hist(hist, outputs, P.delta, state, s), "threshold launch");
     }
-    for (size_t ci = 0; ci < chunks.size(); ++ci) run_chunk(ci, true);
+    for (uint64_t ci = 0; ci < plan.count; ++ci) run_chunk(ci, true);
     // chunks whose candidates overflowed the buffer are scanned again with
     // the (higher) final threshold until none overflows
+    TopState hs;
     for (;;) {
-        TopState hs;
-        check(hipMemcpyAsync(&hs, d->state.p, sizeof(hs), hipMemcpyDeviceToHost, s), "state read");
+        check(hipMemcpyAsync(&hs, state, sizeof(hs), hipMemcpyDeviceToHost, s), "state read");
         check(hipStreamSynchronize(s), "search sync");
         if (hs.redo_count == 0) break;
-        std::vector<uint32_t> redo(hs.redo_count);
-        check(hipMemcpy(redo.data(), d->redo.p, redo.size() * 4, hipMemcpyDeviceToHost), "redo read");
+        const uint32_t n_redo = hs.redo_count;
         hs.redo_count = 0;
-        check(hipMemcpyAsync(d->state.p, &hs, sizeof(hs), hipMemcpyHostToDevice, s), "state reset");
-        for (uint32_t ci : redo) {
-            run_chunk(ci, false);
-            ++st.redo_chunks;
+        check(hipMemcpyAsync(state, &hs, sizeof(hs), hipMemcpyHostToDevice, s), "state reset");
+        // the list is read in blocks of 1024, each block's re-scans queued
+        // before the next block is read.  Re-scans append to the list from
+        // entry 0 again, one entry at most per re-scan, so the re-scans of
+        // blocks 0 .. b write below entry 1024 (b + 1): the unread blocks
+        // are intact when their (stream-ordered) read runs.
+        uint32_t ids[1024];
+        for (uint32_t b0 = 0; b0 < n_redo; b0 += 1024) {
+            const uint32_t nb = std::min<uint32_t>(1024, n_redo - b0);
+            check(hipMemcpyAsync(ids, redo + b0, nb * 4, hipMemcpyDeviceToHost, s), "redo read");
+            check(hipStreamSynchronize(s), "redo read sync");
+            for (uint32_t i = 0; i < nb; ++i) {
+                run_chunk(ids[i], false);
+                ++st.redo_chunks;
+            }
         }
     }
     double scan_ms = 0.0;
     for (size_t i = 0; i + 1 < ev; i += 2) {
         float ms = 0.0f;
-        check(hipEventElapsedTime(&ms, d->events[i], d->events[i + 1]), "event time");
+        check(hipEventElapsedTime(&ms, ws.event(d, i), ws.event(d, i + 1)), "event time");
         scan_ms += ms;
     }
-    TopState hs;
-    check(hipMemcpy(&hs, d->state.p, sizeof(hs), hipMemcpyDeviceToHost), "state read");
-    std::vector<Cand> top(hs.n_top);
-    if (hs.n_top) check(hipMemcpy(top.data(), d->top.p, hs.n_top * sizeof(Cand), hipMemcpyDeviceToHost), "top read");
-    unpack_results(top.data(), hs.n_top, order, bits, out_combos, out_mi);
-    *out_count = hs.n_top;
+    const uint32_t n_top = std::min<uint32_t>(hs.n_top, outputs);
+    ws.results(top, n_top, order, n, binom, out_combos, out_mi, s);
+    *out_count = n_top;
     // combinations evaluated: closed form for the whole space, else a walk
     // over the runs of prefixes (O(C(n, order-2)))
     st.combinations = (step == 1 && offset == 0) ? binom_sat((uint64_t)n, (uint64_t)order)
@@ -844,12 +1007,30 @@ void search(fiu_dataset *d, int order, int32_t step, int32_t offset, uint32_t ou
 
 namespace fiu {
 
-void unpack_results(const Cand *top, uint32_t count, int order, int bits, int32_t *out_combos, float *out_mi)
+// Lexicographic unranking of an m-combination of [0, n) on the host (the
+// device's unrank, with binom_sat for the table).
+static void unrank_host(uint64_t r, uint64_t n, int m, int32_t *out)
 {
-    const uint64_t mask = bits == 64 ? ~0ull : ((1ull << bits) - 1);
+    uint64_t x = binom_sat(n, (uint64_t)m) - 1 - r;
+    int64_t hi = (int64_t)n - 1;
+    for (int i = 0; i < m; ++i) {
+        const uint64_t j = (uint64_t)(m - i);
+        int64_t lo = 0, h = hi;   // largest lo <= hi with C(lo, j) <= x
+        while (lo < h) {
+            const int64_t mid = (lo + h + 1) / 2;
+            if (binom_sat((uint64_t)mid, j) <= x) lo = mid;
+            else h = mid - 1;
+        }
+        x -= binom_sat((uint64_t)lo, j);
+        out[i] = (int32_t)((int64_t)n - 1 - lo);
+        hi = lo - 1;
+    }
+}
+
+void unpack_results(const Cand *top, uint32_t count, int order, uint64_t n, int32_t *out_combos, float *out_mi)
+{
     for (uint32_t i = 0; i < count; ++i) {
-        for (int q = 0; q < order; ++q)
-            out_combos[(size_t)i * order + q] = (int32_t)((top[i].combo >> (bits * (order - 1 - q))) & mask);
+        unrank_host(top[i].combo, n, order, out_combos + (size_t)i * order);
         out_mi[i] = top[i].mi;
     }
 }
@@ -883,8 +1064,6 @@ fiu_status fiu_set_tuning(const fiu_tune_entry *entries, size_t n)
                 fail(FIU_ERR_INVALID, "tuning entry with an unknown algorithm");
             if (e.algorithm == FIU_ALGO_MFMA && e.order != 2 && e.order != 3)
                 fail(FIU_ERR_INVALID, "matrix-core tuning entry above order 3");
-            if (e.algorithm == FIU_ALGO_DIRECT && e.order > kMaxDirectOrder && e.order <= kMaxKmOrder)
-                fail(FIU_ERR_INVALID, "direct tuning entry at orders 5-6");
             if (e.algorithm == FIU_ALGO_MARGINAL && e.order >= kMinAnyOrder)
                 fail(FIU_ERR_INVALID, "marginal tuning entry above order 6");
             if (!valid_variant(e.algorithm, e.order, e.variant)) fail(FIU_ERR_INVALID, "tuning entry with an unknown variant");
@@ -1007,7 +1186,8 @@ fiu_status fiu_search(const fiu_dataset *ds, uint16_t order, int32_t dist_step, 
     return guard([&] {
         if (!ds) fail(FIU_ERR_INVALID, "dataset is NULL");
         if (cfg && cfg->struct_size != sizeof(fiu_config)) fail(FIU_ERR_INVALID, "fiu_config.struct_size mismatch");
-        search(const_cast<fiu_dataset *>(ds), order, dist_step, dist_offset, outputs, cfg, out_combos, out_mi,
+        DatasetWorkspace ws;
+        search(const_cast<fiu_dataset *>(ds), ws, order, dist_step, dist_offset, outputs, cfg, out_combos, out_mi,
                out_count, stats);
     });
 }
@@ -1057,7 +1237,7 @@ fiu_status fiu_contingency_batch(const fiu_dataset *ds, const int32_t *combos, u
 {
     return guard([&] {
         if (!ds || (!out_counts && n)) fail(FIU_ERR_INVALID, "NULL argument");
-        if (order < 1 || order > 8) fail(FIU_ERR_UNSUPPORTED, "order outside 1..8");
+        if (order < 1 || order > kMaxOrder) fail(FIU_ERR_UNSUPPORTED, "order outside 1..12");
         check_combos(ds, combos, order, n);
         if (!n) return;
         check(hipSetDevice(ds->device), "hipSetDevice");

@@ -9,15 +9,16 @@ namespace fiu {
 
 constexpr int kWave = 64;            // CDNA wavefront width
 constexpr int kBlock = 256;          // threads per scan workgroup (4 waves)
-constexpr int kMaxOrder = 9;         // orders with a scan kernel
+constexpr int kMaxOrder = 12;        // orders with a scan kernel
 constexpr int kMaxBatchOrder = 5;    // orders the batched engine schedules
-constexpr int kMaxDirectOrder = 4;   // orders 2-4: an all-cells (direct) kernel
+constexpr int kMaxDirectOrder = 4;   // orders 2-4: the all-cells kernel with one prefix per wave
 constexpr int kMaxKmOrder = 6;       // orders 4-6: marginal in the last SNP only
-constexpr int kMinAnyOrder = 7;      // orders 7-9: the all-cells per-workgroup kernel
-constexpr int kMaxPrefix = 7;        // longest prefix the unranker handles
+constexpr int kMinAnyOrder = 7;      // orders 7-12: the all-cells per-workgroup kernels
+constexpr int kMinBigOrder = 10;     // orders 10-12: the counts in global memory
 constexpr int kMarginalPW = 4;       // prefixes per wavefront in marginal scans
 constexpr int kMergeSlots = 8192;    // LDS entries of the top-N merge
-constexpr int kMaxOutputs = kMergeSlots / 2;
+constexpr int kMaxOutputs = kMergeSlots / 2;   // top-N merged in LDS; above: in global memory
+constexpr uint32_t kMaxOutputsBig = 1u << 24;   // single searches
 
 // Device image of a dataset.
 //
@@ -39,9 +40,9 @@ struct DevData {
     int32_t pad_;
 };
 
-// A top-N candidate: float MI, then the combination packed with the first
-// SNP in the most significant field, so that numeric order of `combo` is the
-// lexicographic order of combinations.
+// A top-N candidate: float MI, then the combination as its lexicographic
+// rank among the k-combinations of the dataset's SNPs (prefix_key), so that
+// numeric order of `combo` is the lexicographic order of combinations.
 struct Cand {
     float mi;
     uint32_t tag;            // dataset / request id in batched scans
@@ -63,10 +64,10 @@ struct ScanParams {
     int64_t j0, j1;          // prefix indices [j0, j1) of the distribution
     int64_t step, offset;    // Distribution step / offset (prefix lex ranks)
     uint64_t total_prefixes; // C(n, order-1)
-    const uint64_t *binom;   // binom[j*(n+1)+d] = C(d, j), j <= order-1
+    const uint64_t *binom;   // binom[j*(n+1)+d] = C(d, j), j <= order
     Cand *cand;              // candidate buffer
     uint32_t cand_cap;
-    int32_t combo_bits;      // bits per SNP index in Cand::combo
+    int32_t order;           // k (combination keys are ranks, prefix_key)
     float delta;             // screening margin on MI
     float hy;                // H(Y) in nats (from FP64)
     float inv_n;             // 1 / (cases + ctrls)
@@ -78,7 +79,19 @@ struct ScanParams {
     const uint32_t *pair;    // marginal pair table (order 3), or null
     uint32_t *hist;          // threshold sampling pass: histogram of the
                              // screening MI (kHistBins bins), else null
+    uint32_t *cells;         // orders 10-12: big_slots(order) slots of
+                             // 2 x 3^order counters (big_cells_bytes)
 };
+
+// Orders 10-12: resident workgroups of the persistent all-cells kernel, each
+// with a slot of 2 x 3^k 32-bit counters in global memory.
+__host__ __device__ inline int big_slots(int order) { return order <= 10 ? 256 : order == 11 ? 128 : 64; }
+__host__ __device__ inline uint64_t big_cells_bytes(int order)
+{
+    uint64_t c = 1;
+    for (int i = 0; i < order; ++i) c *= 3;
+    return (uint64_t)big_slots(order) * 2 * c * sizeof(uint32_t);
+}
 
 // Histogram of screening MI values by the top 12 bits of their float image
 // (sign, exponent, 3 mantissa bits): monotone for non-negative values, so the
@@ -110,6 +123,7 @@ enum BatchKind : int32_t {
     kKindMfma3 = 6,    // order 3 on the matrix cores (fp4): units are (first SNP, tile) pairs
     kKindKm4 = 7,      // order 4, marginal in the last SNP (launch 3)
     kKindKm5 = 8,      // order 5, marginal in the last SNP (launch 3)
+    kKindKm5All = 9,   // order 5, every cell counted (any dataset; launch 3)
 };
 
 // fp4 image of a dataset's genotype rows 0 and 1 (operands of the matrix-core

@@ -139,11 +139,12 @@ struct Job {
     uint32_t cases = 0, ctrls = 0;
     bool staged = false;                  // device work of the upload still to enqueue
     StagedUpload up;
+    const uint64_t *binom = nullptr;      // binomial table of the dataset for this order
     int order = 0;
     int32_t step = 1, offset = 0;
     uint32_t outputs = 0;
     // plan
-    int kind = kKindDirect2, pw = 1, bits = 1, chunks = 1, tiles = 1;
+    int kind = kKindDirect2, pw = 1, chunks = 1, tiles = 1;
     // units of work: prefixes (popcount kinds), first SNPs (order 3 on the
     // matrix cores) or super-blocks (order 2 on the matrix cores)
     int64_t n_pref = 0, next = 0, given = 0, seed = 1;
@@ -191,16 +192,17 @@ class Engine {
     bool pump();
     bool idle();
     void set_on_done(std::function<void(Job *)> f) { on_done_ = std::move(f); }
-    // Pinned staging of staged uploads is released here once the step after
-    // its copies has completed; `freed` runs after each such release.
-    void set_pinned(Arena *pinned, std::function<void()> freed)
+    // The stagings of staged uploads (pinned host block, device block) are
+    // released here once the step after their copies has completed; `freed`
+    // runs after each such release.
+    void set_staging(Arena *pinned, std::function<void()> freed)
     {
         pinned_ = pinned;
-        on_pinned_freed_ = std::move(freed);
+        on_staging_freed_ = std::move(freed);
     }
     fiu_batch_stats stats() const { return stats_; }
     hipStream_t prep_stream() const { return prep_; }
-    // Plan constants of a job (kind, bits, prefixes, unit shape); throws on
+    // Plan constants of a job (kind, prefixes, unit shape); throws on
     // invalid or unsupported requests.
     static void plan_job(Job *j);
 
@@ -229,6 +231,7 @@ class Engine {
     TopState *init_h_ = nullptr;
     char *desc_h_[2] = {nullptr, nullptr};
     TopState *states_h_[2] = {nullptr, nullptr};
+    Cand *top_h_ = nullptr;   // pinned readback of one finished scan's top list
     hipEvent_t ev_start_[2] = {}, ev_end_[2] = {}, ev_done_[2] = {};
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t exec_ = nullptr;
@@ -245,8 +248,13 @@ class Engine {
     fiu_batch_stats stats_{};
     std::function<void(Job *)> on_done_;
     Arena *pinned_ = nullptr;
-    std::function<void()> on_pinned_freed_;
-    std::vector<std::pair<uint64_t, void *>> deferred_;   // (step index, pinned block)
+    std::function<void()> on_staging_freed_;
+    struct Deferred {
+        uint64_t step;   // released once this step has completed
+        void *block;
+        Arena *arena;
+    };
+    std::vector<Deferred> deferred_;
 };
 
 Engine::Engine(int device, const EngineConfig &cfg, Clock::time_point epoch) : device_(device), cfg_(cfg), epoch_(epoch)
@@ -271,6 +279,7 @@ Engine::Engine(int device, const EngineConfig &cfg, Clock::time_point epoch) : d
     top_d_.alloc((size_t)A * kMaxOutputs);
     desc_d_.alloc(desc_bytes_);
     jobs_h_ = pinned<BatchJob>(A);
+    top_h_ = pinned<Cand>(kMaxOutputs);
     init_h_ = pinned<TopState>(1);
     std::memset(init_h_, 0, sizeof(TopState));
     init_h_->tau = -INFINITY;
@@ -313,6 +322,7 @@ Engine::~Engine()
         if (ev_done_[p]) (void)hipEventDestroy(ev_done_[p]);
     }
     if (jobs_h_) (void)hipHostFree(jobs_h_);
+    if (top_h_) (void)hipHostFree(top_h_);
     if (init_h_) (void)hipHostFree(init_h_);
     if (stream_) (void)hipStreamDestroy(stream_);
     if (rb_) (void)hipStreamDestroy(rb_);
@@ -330,8 +340,8 @@ void Engine::plan_job(Job *j)
     if (j->offset < 0) fail(FIU_ERR_INVALID, "distribution offset must be >= 0");
     if (j->outputs > (uint32_t)kMaxOutputs) fail(FIU_ERR_UNSUPPORTED, "outputs above " + std::to_string(kMaxOutputs));
     const int n = (int)d->snps;
-    j->bits = bit_width(n > 0 ? (uint64_t)n - 1 : 0);
-    if (j->bits * order > 64) fail(FIU_ERR_UNSUPPORTED, "combination does not pack in 64 bits");
+    if (binom_sat((uint64_t)n, (uint64_t)order) >= UINT64_MAX - 1)
+        fail(FIU_ERR_UNSUPPORTED, "more than 2^64 - 2 combinations: their ranks do not fit the 64-bit key");
     j->n_pref = 0;
     if (n < order || j->outputs == 0) return;
     const uint64_t total_prefixes = binom_sat((uint64_t)n, (uint64_t)M);
@@ -362,13 +372,14 @@ void Engine::plan_job(Job *j)
         j->rs.algorithm = FIU_ALGO_MFMA;
         return;
     }
-    if (order > kMaxDirectOrder && !d->dev.uniform)
-        fail(FIU_ERR_UNSUPPORTED, "order " + std::to_string(order) +
-                                      " is scanned marginally only: every SNP must cover the same samples");
     const bool marginal = tuned_algo != FIU_ALGO_DIRECT && d->dev.uniform &&
                           (order == 2 || (order == 3 && d->pair_ready) || order >= 4);
+    // order 5 with every cell counted goes to the same launch as the
+    // marginal order-5 kind (lanes split the prefix rows)
+    const bool split = marginal ? order >= 4 : order == 5;
     j->kind = marginal ? (order == 2 ? kKindScan2m : order == 3 ? kKindScan3m : order == 4 ? kKindKm4 : kKindKm5)
-                       : (order == 2 ? kKindDirect2 : order == 3 ? kKindDirect3 : kKindDirect4);
+                       : (order == 2 ? kKindDirect2 : order == 3 ? kKindDirect3 : order == 4 ? kKindDirect4
+                                                                                            : kKindKm5All);
     j->pw = batch_group(j->kind);
     // cells counted per word: 2^k (marginal orders 2-3), 2 * 3^(k-1)
     // (marginal orders 4-5), 3^k (direct)
@@ -383,7 +394,7 @@ void Engine::plan_job(Job *j)
     j->tiles = (int)std::clamp(std::lround(40000.0 / (j->pw * per_tile)), 1L, (long)n_tiles);
     // marginal orders 4-5 count the prefix rows once per unit: at least 4
     // tiles (256 last SNPs) per unit keep that pass a few per cent
-    if (marginal && order >= 4) j->tiles = std::min(n_tiles, std::max(j->tiles, 4));
+    if (split) j->tiles = std::min(n_tiles, std::max(j->tiles, 4));
     j->chunks = (n_tiles + j->tiles - 1) / j->tiles;
     const double per_pref = (double)(n - M) / (double)(M + 1);   // mean combinations per prefix
     j->instr_per_prefix = (per_pref / 64.0 + 0.5) * per_tile;
@@ -450,13 +461,17 @@ void Engine::activate(Job *j, int slot)
     active_.push_back(j);
     if (j->staged) {
         // copies, transposes and the pair table go ahead of the job's first
-        // step on the same stream; the pinned source is released once the
-        // next step (launched after them) has completed
+        // step on the same stream; both stagings they read (pinned host,
+        // device) are released once the next step, launched after them, has
+        // completed
         enqueue_upload(j->ds, j->up, stream_);
-        deferred_.emplace_back(step_no_, j->up.host);
+        deferred_.push_back({step_no_, j->up.host, pinned_});
+        deferred_.push_back({step_no_, j->up.staging_d, j->ds->arena});
         j->up.host = nullptr;
+        j->up.staging_d = nullptr;
         j->staged = false;
     }
+    if (slot < 0 || slot >= cfg_.max_active) fail(FIU_ERR_DEVICE, "internal: job slot out of range");
     BatchJob &B = jobs_h_[slot];
     std::memset(&B, 0, sizeof(B));
     fiu_dataset *d = j->ds;
@@ -465,10 +480,10 @@ void Engine::activate(Job *j, int slot)
     P.step = j->step;
     P.offset = j->offset;
     P.total_prefixes = binom_sat(d->snps, (uint64_t)(j->order - 1));
-    P.binom = d->binom.p;
+    P.binom = j->binom;
     P.cand = cand_d_.p + (size_t)slot * cfg_.cand_cap;
     P.cand_cap = cfg_.cand_cap;
-    P.combo_bits = j->bits;
+    P.order = j->order;
     P.tag = (uint32_t)slot;
     P.pair = (j->kind == kKindScan3m || j->kind == kKindMfma3) ? d->pair.p : nullptr;
     P.hist = nullptr;
@@ -485,6 +500,9 @@ void Engine::activate(Job *j, int slot)
         B.F.class_off[0] = 0;
         B.F.class_off[1] = fp4_image_units(d->dev, 0);
     }
+    if (!P.binom) fail(FIU_ERR_DEVICE, "internal: job without a binomial table");
+    check_range(jobs_d_.p + slot, sizeof(B), jobs_d_.p, jobs_d_.n * sizeof(BatchJob), "job upload");
+    check_range(states_d_.p + slot, sizeof(TopState), states_d_.p, states_d_.n * sizeof(TopState), "state init");
     check(hipMemcpyAsync(jobs_d_.p + slot, &B, sizeof(B), hipMemcpyHostToDevice, stream_), "job upload");
     check(hipMemcpyAsync(states_d_.p + slot, init_h_, sizeof(TopState), hipMemcpyHostToDevice, stream_), "state init");
 }
@@ -582,7 +600,9 @@ bool Engine::launch_step()
     };
     auto add_seg = [&](Job *j, int64_t a, int64_t b) {
         if (batch_mfma(j->kind)) return add_mseg(j, a, b);
-        const int L = j->kind == kKindDirect4 ? 1 : (j->kind == kKindKm4 || j->kind == kKindKm5) ? 2 : 0;
+        const int L = j->kind == kKindDirect4 ? 1
+                      : (j->kind == kKindKm4 || j->kind == kKindKm5 || j->kind == kKindKm5All) ? 2
+                                                                                                 : 0;
         if (nseg[L] >= max_segs_) return false;
         const uint64_t u = (uint64_t)((b - a + j->pw - 1) / j->pw) * (uint64_t)j->chunks;
         if (units[L] + u > (1ull << 31)) return false;
@@ -639,7 +659,12 @@ bool Engine::launch_step()
     H->mseg = nmseg;
     H->munits = (uint32_t)munits;
     // descriptor: header + merge list + launch-0 segments in one copy, the
-    // launch-1 segments in a second one when there are any
+    // launch-1 segments in a second one when there are any; each range stays
+    // inside its region of the descriptor
+    if (nmerge > (uint32_t)cfg_.max_active || nseg[0] > max_segs_ || nseg[1] > max_segs_ || nseg[2] > max_segs_ ||
+        nmseg > max_msegs_)
+        fail(FIU_ERR_DEVICE, "internal: step descriptor overflow");
+    check_range(desc + off_segs3_, nseg[2] * sizeof(BatchSeg), desc, desc_bytes_, "step descriptor");
     check(hipMemcpyAsync(desc_d_.p, desc, off_segs0_ + nseg[0] * sizeof(BatchSeg), hipMemcpyHostToDevice, stream_),
           "step descriptor");
     if (nseg[1])
@@ -670,17 +695,18 @@ bool Engine::launch_step()
 
 void Engine::finish(Job *j, uint32_t n_top)
 {
-    j->count = std::min(n_top, j->outputs);
-    std::vector<Cand> top(j->count);
+    // read into pinned memory: a direct device-to-host copy, no host-side
+    // staging in the runtime
+    j->count = std::min<uint32_t>({n_top, j->outputs, (uint32_t)kMaxOutputs});
     if (j->count) {
-        check(hipMemcpyAsync(top.data(), top_d_.p + (size_t)j->slot * kMaxOutputs, j->count * sizeof(Cand),
-                             hipMemcpyDeviceToHost, rb_),
-              "top read");
+        const Cand *src = top_d_.p + (size_t)j->slot * kMaxOutputs;
+        check_range(src, j->count * sizeof(Cand), top_d_.p, top_d_.n * sizeof(Cand), "top read");
+        check(hipMemcpyAsync(top_h_, src, j->count * sizeof(Cand), hipMemcpyDeviceToHost, rb_), "top read");
         check(hipStreamSynchronize(rb_), "top read sync");
     }
     j->combos.assign((size_t)j->count * j->order, 0);
     j->mi.assign(j->count, 0.0f);
-    unpack_results(top.data(), j->count, j->order, j->bits, j->combos.data(), j->mi.data());
+    unpack_results(top_h_, j->count, j->order, j->ds->snps, j->combos.data(), j->mi.data());
     slots_[j->slot] = nullptr;
     active_.erase(std::find(active_.begin(), active_.end(), j));
     j->slot = -1;
@@ -714,12 +740,12 @@ void Engine::complete_step(int parity)
     for (Job *j : R.jobs)
         if (j->slot >= 0 && j->inflight == 0 && j->next >= j->n_pref && j->redo.empty()) finish(j, S[j->slot].n_top);
     R.valid = false;
-    // pinned staging whose copies were enqueued before this step is free
+    // stagings whose copies and transposes were enqueued before this step are free
     const uint64_t finished = done_no_;
     bool freed = false;
     for (size_t i = 0; i < deferred_.size();) {
-        if (deferred_[i].first <= finished) {
-            pinned_->release(deferred_[i].second);
+        if (deferred_[i].step <= finished) {
+            if (deferred_[i].block && deferred_[i].arena) deferred_[i].arena->release(deferred_[i].block);
             deferred_[i] = deferred_.back();
             deferred_.pop_back();
             freed = true;
@@ -727,7 +753,7 @@ void Engine::complete_step(int parity)
             ++i;
         }
     }
-    if (freed && on_pinned_freed_) on_pinned_freed_();
+    if (freed && on_staging_freed_) on_staging_freed_();
     ++done_no_;
 }
 
@@ -808,10 +834,13 @@ Server::Server(int device, const EngineConfig &ec, size_t budget) : device_(devi
         budget = std::min<size_t>(free_b / 4, 8ull << 30);
     }
     arena_ = std::make_unique<Arena>(budget, device_);
-    pinned_ = std::make_unique<Arena>(std::min<size_t>(budget, 1ull << 30), device_, true);
+    // a staged upload holds its tables twice on the device (staging and
+    // transposed image): pinned staging beyond a quarter of the budget is
+    // never usable; larger tables take the synchronous path
+    pinned_ = std::make_unique<Arena>(std::min<size_t>(budget / 4, 1ull << 30), device_, true);
     eng_ = std::make_unique<Engine>(device_, ec, epoch_);
     check(hipStreamCreateWithFlags(&ingest_stream_, hipStreamNonBlocking), "hipStreamCreate");
-    eng_->set_pinned(pinned_.get(), [this] {
+    eng_->set_staging(pinned_.get(), [this] {
         {
             std::lock_guard<std::mutex> lk(mu_);
             ++mem_gen_;
@@ -852,7 +881,7 @@ size_t Server::footprint(const Job *j) const
 {
     if (j->ds) return 0;  // resident dataset: nothing to upload
     size_t b = dataset_bytes(j->snps, j->cw, j->tw) + staging_bytes(j->snps, j->cw, j->tw);
-    b += (size_t)(j->order + 1) * (j->snps + 1) * 8 + Arena::kAlign;   // binomial table
+    b += (size_t)(j->order + 1) * (j->snps + 1) * 8 + Arena::kAlign;   // binomial table (rows up to k)
     if (j->order == 3) b += pair_table_bytes(j->snps) + Arena::kAlign;
     // the fp4 image of the matrix-core scans is not counted: it is allocated
     // at upload only when it fits, else the popcount kinds scan the dataset
@@ -954,6 +983,7 @@ void Server::ingest_main()
                 } else {
                     j->owned.reset(d);
                     j->ds = d;
+                    j->binom = d->binom.p;   // this job's own dataset, built for its order
                     j->staged = true;
                     std::vector<uint64_t>().swap(j->words);
                     Engine::plan_job(j);
@@ -965,7 +995,7 @@ void Server::ingest_main()
                     j->ds = j->owned.get();
                     std::vector<uint64_t>().swap(j->words);
                 }
-                ensure_binom(j->ds, j->order - 1, ingest_stream_);
+                j->binom = ensure_binom(j->ds, j->order, ingest_stream_);
                 if (j->order == 3 && j->ds->dev.uniform) (void)ensure_pair_table(j->ds, ingest_stream_);
                 prepare_fp4(j, ingest_stream_);
                 Engine::plan_job(j);
@@ -1174,7 +1204,7 @@ fiu_status fiu_search_batch(fiu_scan *scans, size_t n, const fiu_batch_config *c
         for (auto &jp : jobs) {
             Job *j = jp.get();
             try {
-                ensure_binom(j->ds, std::max(1, j->order - 1), eng.prep_stream());
+                j->binom = ensure_binom(j->ds, j->order, eng.prep_stream());
                 if (j->order == 3 && j->ds->dev.uniform) (void)ensure_pair_table(j->ds, eng.prep_stream());
                 prepare_fp4(j, eng.prep_stream());
                 Engine::plan_job(j);

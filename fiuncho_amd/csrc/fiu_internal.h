@@ -101,6 +101,15 @@ template <class T> struct DBuf {
         p = nullptr;
         n = 0;
     }
+    // Hand the allocation over to the caller (who frees it later, through
+    // `arena` when set) and forget it.
+    T *detach()
+    {
+        T *q = p;
+        p = nullptr;
+        n = 0;
+        return q;
+    }
     void alloc(size_t count)
     {
         if (count <= n && p) return;
@@ -118,8 +127,17 @@ template <class T> struct DBuf {
     }
 };
 
+// Every host-issued copy names the allocation it touches: a range that does
+// not lie inside it is a bug in the caller, reported as FIU_ERR_DEVICE
+// instead of being handed to the runtime.
+inline void check_range(const void *p, size_t bytes, const void *base, size_t cap, const char *what)
+{
+    const uintptr_t a = (uintptr_t)p, b = (uintptr_t)base;
+    if (!p || !base || a < b || a - b > cap || bytes > cap - (a - b))
+        fail(FIU_ERR_DEVICE, std::string("internal: ") + what + " copy outside its allocation");
+}
+
 uint64_t binom_sat(uint64_t n, uint64_t k);
-int bit_width(uint64_t x);
 
 // Exact number of combinations whose m-prefix has lexicographic rank
 // = offset (mod step).
@@ -159,10 +177,15 @@ struct fiu_dataset {
     fiu::DBuf<uint32_t> pair;        // marginal pair table (order-3 marginal scans)
     fiu::DBuf<uint32_t> hist;        // screening-MI histogram of the threshold sampling pass
     fiu::DBuf<uint4> fp4;            // fp4 image of genotype rows 0/1 (matrix-core scans)
+    fiu::DBuf<uint32_t> big;         // orders 10-12: count slots (big_cells_bytes)
+    fiu::DBuf<fiu::Cand> mwork;      // outputs above kMaxOutputs: the merge's sort buffer
     bool fp4_ready = false;
     bool pair_ready = false;
     std::vector<hipEvent_t> events;
     std::mutex prep_mu;              // binom / pair table preparation
+    // device buffers replaced while work queued earlier may still read them
+    // (a binomial table grown for a higher order): freed with the dataset
+    std::vector<std::pair<void *, fiu::Arena *>> retired;
 
     // every sample has a genotype in {0, 1, 2} at every SNP: the cells of a
     // table sum to the class sizes
@@ -172,7 +195,7 @@ struct fiu_dataset {
     {
         arena = a;
         bits.arena = singles.arena = flags.arena = binom.arena = redo.arena = pair.arena = hist.arena = a;
-        fp4.arena = a;
+        fp4.arena = big.arena = mwork.arena = a;
         cand.arena = top.arena = a;
         state.arena = a;
     }
@@ -223,13 +246,17 @@ size_t staged_host_bytes(size_t snps, size_t cw, size_t tw, int order);
 // needs the pair table) and it fits.
 fiu_dataset *stage_dataset(const uint64_t *words, size_t snps, size_t cw, size_t tw, uint32_t cases,
                            uint32_t ctrls, int order, bool want_fp4, Arena *arena, Arena *pinned, StagedUpload &up);
-// Copies, transposes and builds the pair table on `s`; the device staging is
-// released at once (later users of that memory are ordered after it on `s`),
-// the pinned staging must be kept until the work completes.
+// Copies, transposes and builds the pair table on `s`.  Both stagings --
+// the pinned host block up.host and the device block up.staging_d -- are
+// read by that queued work: the caller releases them only once it has
+// completed (the scan engine releases them after the next step on `s`).
 void enqueue_upload(fiu_dataset *d, StagedUpload &up, hipStream_t s);
 
-// binom[j*(n+1)+x] = C(x, j) for j <= m, on the device.
-void ensure_binom(fiu_dataset *d, int m, hipStream_t s);
+// binom[j*(n+1)+x] = C(x, j) for j <= m, on the device; returns the table.
+// A table grown for a higher order replaces the dataset's current one, which
+// stays allocated (work queued earlier may read it) until the dataset is
+// destroyed, so a returned pointer stays valid for the dataset's lifetime.
+const uint64_t *ensure_binom(fiu_dataset *d, int m, hipStream_t s);
 // Pair counts f(x, y), x, y in {0, 1}, of every SNP pair (32 bytes per
 // pair).  Returns false when the table does not fit in free device memory
 // (hipMalloc datasets) or in the arena.
@@ -246,7 +273,8 @@ inline bool mfma_dataset(const fiu_dataset *d)
     return d->dev.uniform && d->cases < (1u << 24) && d->ctrls < (1u << 24);
 }
 
-// Top-N entries -> ascending SNP indices (order per entry) and float MI.
-void unpack_results(const Cand *top, uint32_t count, int order, int bits, int32_t *out_combos, float *out_mi);
+// Top-N entries -> ascending SNP indices (order per entry, unranked from the
+// key among the k-combinations of n SNPs) and float MI.
+void unpack_results(const Cand *top, uint32_t count, int order, uint64_t n, int32_t *out_combos, float *out_mi);
 
 }  // namespace fiu

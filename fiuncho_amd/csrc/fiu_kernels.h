@@ -5,9 +5,11 @@
 
 namespace fiu {
 
-// Orders 4 to 6, marginal in the last SNP (uniform datasets); P.hist set:
-// the threshold sampling pass.
-hipError_t launch_scan_km(int order, const DevData &D, const ScanParams &P, TopState *st, hipStream_t s);
+// Orders 4 to 6 with the prefix rows split over lanes: marginal in the last
+// SNP (uniform datasets), or all_cells: every cell counted (any dataset);
+// P.hist set: the threshold sampling pass.
+hipError_t launch_scan_km(int order, const DevData &D, const ScanParams &P, TopState *st, hipStream_t s,
+                          bool all_cells = false);
 hipError_t launch_scan_direct(int order, const DevData &D, const ScanParams &P, TopState *st, hipStream_t s);
 // pw: prefixes per wavefront (2, 4 or, order 2 only, 8; the threshold
 // sampling pass uses 4)
@@ -34,6 +36,11 @@ hipError_t launch_mfma2(const DevData &D, const ScanParams &P, TopState *st, int
                         const uint4 *fp4, hipStream_t s);
 hipError_t launch_merge(TopState *st, Cand *top, const Cand *cand, uint32_t N, uint32_t cap, uint32_t chunk_id,
                         uint32_t *redo, hipStream_t s);
+// Top-N merge for N above kMaxOutputs, in global memory: the list's n_top
+// entries and `count` candidates (sizes read by the host) sorted in `work`
+// (L entries, a power of two >= n_top + count), the first N unique kept.
+hipError_t launch_merge_big(TopState *st, Cand *top, uint32_t n_top, const Cand *cand, uint32_t count, uint32_t N,
+                            Cand *work, uint32_t L, uint32_t chunk_id, uint32_t *redo, hipStream_t s);
 // Genotypes (SNP-major int8; the buffer allocated to a multiple of 16 bytes)
 // -> reference-layout words of snps x 3(cw+tw).  `idx` null: class c is the
 // run of n_class[c] samples from start[c]; else the per-class sample lists.

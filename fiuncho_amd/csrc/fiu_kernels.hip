@@ -123,6 +123,20 @@ __device__ inline void unrank(uint64_t r, int n, int m, const uint64_t *binom, i
     }
 }
 
+// Lexicographic rank of the k-combination (pre[0] < ... < pre[k-2] < c) of
+// [0, n), the key of a candidate: rank = C(n,k) - 1 - sum_i C(n-1-c_i, k-i),
+// whose last term is C(n-1-c, 1) = n-1-c, so the rank is a constant of the
+// prefix plus c.  Needs the binomial table up to row k and C(n, k) < 2^64
+// (checked on the host).  Ranks keep the lexicographic order of the
+// combinations, so comparing keys compares combinations.
+__device__ __forceinline__ uint64_t prefix_key(const uint64_t *binom, int n, const int *pre, int M)
+{
+    const uint64_t stride = (uint64_t)n + 1;
+    uint64_t s = binom[(uint64_t)(M + 1) * stride + n] - (uint64_t)n;   // C(n,k) - 1 - (n - 1)
+    for (int i = 0; i < M; ++i) s -= binom[(uint64_t)(M + 1 - i) * stride + (n - 1 - pre[i])];
+    return s;
+}
+
 // Screening MI in FP32 from exact counts, in the conditional-entropy form
 // MI = H(Y) + (1/N) sum_x [n_cx ln(n_cx/n_x) + n_tx ln(n_tx/n_x)], whose terms
 // all have one sign (no cancellation inside the sum).  Error < 2e-6 nats for
@@ -287,14 +301,16 @@ __device__ __forceinline__ void push_candidates(bool want, float mi, uint64_t co
 }
 
 // Epilogue shared by every scan kernel: screening MI, exact MI for the
-// survivors, key test against the current N-th best, push.
-template <int CELLS, class T>
-__device__ __forceinline__ void score_and_push(bool valid, const T *a0, const T *a1,
-                                               uint64_t combo, float tau, uint64_t tau_combo,
-                                               const ScanParams &P, TopState *st)
+// survivors, key test against the current N-th best, push.  key() gives the
+// combination's key; it is only called for the few combinations that reach
+// the exact MI.
+template <int CELLS, class T, class K>
+__device__ __forceinline__ void score_and_push(bool valid, const T *a0, const T *a1, K &&key, float tau,
+                                               uint64_t tau_combo, const ScanParams &P, TopState *st)
 {
     bool want = false;
     float mf = 0.0f;
+    uint64_t combo = ~0ull;
     if (valid && (!P.bounded || bound_passes(mi_bound<CELLS>(a0, a1, P.frac_c, P.chi_scale), tau, P.delta))) {
         const float ms = mi_screen<CELLS>(a0, a1, P.hy, P.inv_n);
         if (ms >= tau - P.delta) {
@@ -306,6 +322,7 @@ __device__ __forceinline__ void score_and_push(bool valid, const T *a0, const T 
                 for (int x = 0; x < CELLS; ++x) u0[x] = (uint32_t)a0[x], u1[x] = (uint32_t)a1[x];
                 mf = (float)mi_exact<CELLS>(u0, u1, P.n64, P.hy64);
             }
+            combo = key();
             want = better(mf, combo, tau, tau_combo);
         }
     }
@@ -325,15 +342,15 @@ __device__ __forceinline__ bool prefix_at(const ScanParams &P, int n, int64_t j,
 
 // Either the normal epilogue, or (threshold sampling pass) one count in the
 // workgroup's LDS histogram of screening MI values.
-template <bool HIST, int CELLS>
-__device__ __forceinline__ void score_or_sample(bool valid, const uint32_t *a0, const uint32_t *a1, uint64_t combo,
+template <bool HIST, int CELLS, class K>
+__device__ __forceinline__ void score_or_sample(bool valid, const uint32_t *a0, const uint32_t *a1, K &&key,
                                                 float tau, uint64_t tau_combo, const ScanParams &P, TopState *st,
                                                 uint32_t *lds_hist)
 {
     if constexpr (HIST) {
         if (valid) atomicAdd(&lds_hist[hist_bin(mi_screen<CELLS>(a0, a1, P.hy, P.inv_n))], 1u);
     } else {
-        score_and_push<CELLS>(valid, a0, a1, combo, tau, tau_combo, P, st);
+        score_and_push<CELLS>(valid, a0, a1, key, tau, tau_combo, P, st);
     }
 }
 
@@ -500,10 +517,7 @@ __device__ __forceinline__ void direct_unit(const DevData &D, const ScanParams &
     const int lane = threadIdx.x & 63;
     int pre[M];
     if (!prefix_at<M>(P, D.n, j, j1, pre)) return;
-    uint64_t pre_packed = 0;
-#pragma unroll
-    for (int i = 0; i < M; ++i) pre_packed = (pre_packed << P.combo_bits) | (uint64_t)pre[i];
-    pre_packed <<= P.combo_bits;
+    const uint64_t pre_key = prefix_key(P.binom, D.n, pre, M);
     const int last = pre[M - 1];
     const float tau = st->tau;
     const uint64_t tau_combo = st->tau_combo;
@@ -517,8 +531,8 @@ __device__ __forceinline__ void direct_unit(const DevData &D, const ScanParams &
         uint32_t a0[CELLS], a1[CELLS];
         accumulate_direct<K>(b0, D.n_pad, D.quads[0], pre, c * 16, a0);
         accumulate_direct<K>(b1, D.n_pad, D.quads[1], pre, c * 16, a1);
-        score_or_sample<HIST, CELLS>(c > last && c < D.n, a0, a1, pre_packed | (uint64_t)c, tau, tau_combo, P, st,
-                                     lds_hist);
+        score_or_sample<HIST, CELLS>(c > last && c < D.n, a0, a1, [&] { return pre_key + (uint64_t)c; }, tau,
+                                     tau_combo, P, st, lds_hist);
     }
 }
 
@@ -572,9 +586,7 @@ __global__ void __launch_bounds__(kBlock) k_scan_any(const DevData D, const Scan
 #pragma unroll
         for (int q = 4; q >= 0; --q) g5[q] = t % 3, t /= 3;
     }
-    uint64_t pre_packed = 0;
-#pragma unroll
-    for (int i = 0; i < M; ++i) pre_packed = (pre_packed << P.combo_bits) | (uint64_t)s[i];
+    const uint64_t pre_key = prefix_key(P.binom, D.n, s, M);
     for (int c = s[M - 1] + 1; c < D.n; ++c) {
         s[M] = c;
 #pragma unroll
@@ -633,7 +645,7 @@ __global__ void __launch_bounds__(kBlock) k_scan_any(const DevData D, const Scan
             hall += __shfl_xor(hall, off);
         }
         if (lane == 0) red[0][wv] = hx, red[1][wv] = hall;
-        const uint64_t combo = (pre_packed << P.combo_bits) | (uint64_t)c;
+        const uint64_t combo = pre_key + (uint64_t)c;
         __syncthreads();
         if (tid == 0) {
             double sx = 0.0, sa = 0.0;
@@ -670,6 +682,143 @@ __global__ void __launch_bounds__(kBlock) k_scan_any(const DevData D, const Scan
 }
 
 // ---------------------------------------------------------------------------
+// Orders 10 to 12 (FIU_ALGO_DIRECT: all cells, any dataset).
+// ---------------------------------------------------------------------------
+// The same counting as k_scan_any, with the 3^k cells of both classes in
+// global memory (472 KB per combination at order 10, 4.25 MB at order 12):
+// the launch is persistent, workgroup w owns slot w of `cells` (2 x 3^k
+// counters) and takes the prefixes j0 + w, j0 + w + gridDim.x, ... of the
+// chunk, so that no two resident workgroups share a slot.  Thread t < 243
+// owns the genotypes of the first five SNPs; the 3^(k-7) genotypes of the
+// middle SNPs (the sixth to the (k-2)-th) are taken 9 at a time, each block
+// re-reading the quads (they stay in L2); per 128-bit quad the thread counts
+// the 9 cells of the last two SNPs.  Cell x = sum g_q 3^(k-1-q) = (t 3^(k-7) +
+// p) 9 + z, p the middle genotypes' index, z those of the last two.  Screen
+// and exact MI as in k_scan_any, reading the counts from the slot.
+template <int K, bool HIST>
+__global__ void __launch_bounds__(kBlock) k_scan_big(const DevData D, const ScanParams P, TopState *st,
+                                                     uint32_t *cells)
+{
+    static_assert(K >= 10 && K <= 12, "orders 10 to 12");
+    constexpr int M = K - 1, CELLS = Pow3<K>::v, PASSES = Pow3<K - 7>::v, PB = 9, MID = K - 7;
+    __shared__ double red[2][kBlock / kWave];
+    __shared__ int pass_sh;
+    uint32_t *cnt[2] = {cells + (uint64_t)blockIdx.x * 2 * CELLS, cells + (uint64_t)blockIdx.x * 2 * CELLS + CELLS};
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int np = D.n_pad;
+    int g5[5];
+    {
+        int t = tid < 243 ? tid : 0;
+#pragma unroll
+        for (int q = 4; q >= 0; --q) g5[q] = t % 3, t /= 3;
+    }
+    for (int64_t j = P.j0 + blockIdx.x; j < P.j1; j += gridDim.x) {
+        int s[K];
+        if (!prefix_at<M>(P, D.n, j, P.j1, s)) break;   // uniform; later j lie past the last prefix too
+        const float tau = st->tau;
+        const uint64_t tau_combo = st->tau_combo;
+        const uint64_t pre_key = prefix_key(P.binom, D.n, s, M);
+        for (int c = s[M - 1] + 1; c < D.n; ++c) {
+            s[M] = c;
+            for (int cls = 0; cls < 2; ++cls) {
+                const uint4 *base = D.bits + D.class_off[cls];
+                for (int p0 = 0; p0 < PASSES; p0 += PB) {
+                    uint32_t n9[PB][9];
+#pragma unroll
+                    for (int pi = 0; pi < PB; ++pi)
+#pragma unroll
+                        for (int z = 0; z < 9; ++z) n9[pi][z] = 0;
+                    for (int q = 0; q < D.quads[cls]; ++q) {
+                        const uint4 *rq = base + (uint64_t)q * 3 * np;
+                        uint4 pr = rq[g5[0] * np + s[0]];
+#pragma unroll
+                        for (int i = 1; i < 5; ++i) pr = pr & rq[g5[i] * np + s[i]];
+                        uint4 ra[3], rb[3];
+#pragma unroll
+                        for (int z = 0; z < 3; ++z) ra[z] = rq[z * np + s[K - 2]], rb[z] = rq[z * np + s[K - 1]];
+#pragma unroll
+                        for (int pi = 0; pi < PB; ++pi) {
+                            // middle genotypes of pass p0 + pi, the first middle SNP most significant
+                            uint4 pm = pr;
+                            int p = p0 + pi;
+#pragma unroll
+                            for (int i = MID - 1; i >= 0; --i) {
+                                pm = pm & rq[(p % 3) * np + s[5 + i]];
+                                p /= 3;
+                            }
+#pragma unroll
+                            for (int z1 = 0; z1 < 3; ++z1) {
+                                const uint4 pz = pm & ra[z1];
+#pragma unroll
+                                for (int z2 = 0; z2 < 3; ++z2) {
+                                    const uint4 v = pz & rb[z2];
+                                    n9[pi][3 * z1 + z2] += __builtin_popcount(v.x) + __builtin_popcount(v.y) +
+                                                           __builtin_popcount(v.z) + __builtin_popcount(v.w);
+                                }
+                            }
+                        }
+                    }
+                    if (tid < 243) {
+#pragma unroll
+                        for (int pi = 0; pi < PB; ++pi)
+#pragma unroll
+                            for (int z = 0; z < 9; ++z) cnt[cls][(tid * PASSES + p0 + pi) * 9 + z] = n9[pi][z];
+                    }
+                }
+            }
+            __threadfence_block();
+            __syncthreads();
+            // the MI in any order (the screen), as in k_scan_any
+            double hx = 0.0, hall = 0.0;
+            for (int x = tid; x < CELLS; x += kBlock) {
+                const double pc = (double)cnt[0][x] / P.n64, pt = (double)cnt[1][x] / P.n64;
+                hall += plogp(pc) + plogp(pt);
+                hx += plogp(pc + pt);
+            }
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                hx += __shfl_xor(hx, off);
+                hall += __shfl_xor(hall, off);
+            }
+            if (lane == 0) red[0][wv] = hx, red[1][wv] = hall;
+            const uint64_t combo = pre_key + (uint64_t)c;
+            __syncthreads();
+            if (tid == 0) {
+                double sx = 0.0, sa = 0.0;
+#pragma unroll
+                for (int w = 0; w < kBlock / kWave; ++w) sx += red[0][w], sa += red[1][w];
+                const double v = P.hy64 - sx + sa;
+                if constexpr (HIST) atomicAdd(&P.hist[hist_bin((float)v)], 1u);
+                pass_sh = !HIST && (float)(v + 1e-9) >= tau;
+            }
+            __syncthreads();
+            if (pass_sh && wv == 0) {
+#pragma clang fp contract(off)
+                double ex = 0.0, ea = 0.0;
+                for (int x0 = 0; x0 < CELLS; x0 += kWave) {
+                    const int x = x0 + lane;
+                    double tc = 0.0, tt = 0.0, ta = 0.0;
+                    if (x < CELLS) {
+                        const double pc = (double)cnt[0][x] / P.n64, pt = (double)cnt[1][x] / P.n64, pa = pc + pt;
+                        tc = plogp(pc), tt = plogp(pt), ta = plogp(pa);
+                    }
+                    const int last = CELLS - x0 < kWave ? CELLS - x0 : kWave;
+                    for (int l = 0; l < last; ++l) {
+                        const double a = __shfl(tc, l), b = __shfl(tt, l), e = __shfl(ta, l);
+                        ea -= a;
+                        ea -= b;
+                        ex -= e;
+                    }
+                }
+                const float mf = (float)(ex + P.hy64 - ea);
+                push_candidates(lane == 0 && better(mf, combo, tau, tau_combo), mf, combo, P, st);
+            }
+            __syncthreads();   // the slot and pass_sh are rewritten for the next c
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Orders 4 to 6, marginal in the last SNP (FIU_ALGO_MARGINAL).
 // ---------------------------------------------------------------------------
 // A wavefront takes one prefix (s_0 < ... < s_{k-2}) of the distribution and
@@ -699,7 +848,8 @@ template <int K> struct Km {
 };
 
 // One class: acc[r][z] = sum over words of popcount(row r & f & c_z), z < Z
-// (Z = 2: c's rows 0 and 1; Z = 1: no c, the prefix rows' own counts).
+// (Z = 3: c's three rows; Z = 2: c's rows 0 and 1; Z = 1: no c, the prefix
+// rows' own counts).
 // f = AND of rows digit_i(p) of the lane-split SNPs pre[i] (lane offsets
 // voff_f[i]); row r = AND of rows r / 3 and r % 3 of the last two prefix
 // SNPs (scalar loads: their indices are wave-uniform); c at voff_c.
@@ -721,7 +871,7 @@ __device__ __forceinline__ void km_count(const uint4 *__restrict__ base, int np,
         __attribute__((always_inline)) {
 #pragma unroll
         for (int i = 0; i < LS; ++i) lf[i] = ld_lane(r, np, 0, voff_f[i]);
-        if constexpr (Z == 2) {
+        if constexpr (Z >= 2) {
 #pragma unroll
             for (int z = 0; z < Z; ++z) lc[z] = ld_lane(r, np, z, voff_c);
         }
@@ -747,13 +897,14 @@ __device__ __forceinline__ void km_count(const uint4 *__restrict__ base, int np,
             uint32_t fw = comp(f[0], w);
 #pragma unroll
             for (int i = 1; i < LS; ++i) fw &= comp(f[i], w);
-            if constexpr (Z == 2) {
-                const uint32_t x0 = comp(cz[0], w) & fw, x1 = comp(cz[1], w) & fw;
+            if constexpr (Z >= 2) {
+                uint32_t x[Z];
 #pragma unroll
-                for (int r = 0; r < NR; ++r) {
-                    acc[r][0] += __builtin_popcount(rows[r] & x0);
-                    acc[r][1] += __builtin_popcount(rows[r] & x1);
-                }
+                for (int z = 0; z < Z; ++z) x[z] = comp(cz[z], w) & fw;
+#pragma unroll
+                for (int r = 0; r < NR; ++r)
+#pragma unroll
+                    for (int z = 0; z < Z; ++z) acc[r][z] += __builtin_popcount(rows[r] & x[z]);
             } else {
 #pragma unroll
                 for (int r = 0; r < NR; ++r) acc[r][0] += __builtin_popcount(rows[r] & fw);
@@ -772,8 +923,11 @@ __device__ __forceinline__ void km_count(const uint4 *__restrict__ base, int np,
     }
 }
 
-// Last SNPs c in tiles t_lo .. t_hi (64 SNPs each) only.
-template <int K, bool HIST>
+// Last SNPs c in tiles t_lo .. t_hi (64 SNPs each) only.  ALL: the three
+// rows of c are counted (every one of the 3^k cells by AND + popcount, the
+// reference's algorithm), which needs no coverage property of the dataset;
+// otherwise genotype 2 of c follows from the prefix rows' own counts.
+template <int K, bool HIST, bool ALL = false>
 __device__ __forceinline__ void km_unit(const DevData &D, const ScanParams &P, TopState *st, int64_t j, int64_t j1,
                                         int t_hi, int t_lo, uint32_t *lds_hist)
 {
@@ -782,10 +936,7 @@ __device__ __forceinline__ void km_unit(const DevData &D, const ScanParams &P, T
     if (!prefix_at<M>(P, D.n, j, j1, pre)) return;
 #pragma unroll
     for (int i = 0; i < M; ++i) pre[i] = __builtin_amdgcn_readfirstlane(pre[i]);
-    uint64_t pre_packed = 0;
-#pragma unroll
-    for (int i = 0; i < M; ++i) pre_packed = (pre_packed << P.combo_bits) | (uint64_t)pre[i];
-    pre_packed <<= P.combo_bits;
+    const uint64_t pre_key = prefix_key(P.binom, D.n, pre, M);
     const int last = pre[M - 1], n = D.n, np = D.n_pad;
     const int c_begin = max(last + 1, 64 * t_lo), c_end = min(n, 64 * min(t_hi, (np >> 6) - 1) + 64);
     if (c_begin >= c_end) return;
@@ -802,9 +953,12 @@ __device__ __forceinline__ void km_unit(const DevData &D, const ScanParams &P, T
     const uint4 *b0 = D.bits + D.class_off[0];
     const uint4 *b1 = D.bits + D.class_off[1];
     // the prefix rows' own counts (genotype "any" of c)
+    constexpr int Z = ALL ? 3 : 2;
     uint32_t pr0[NR][1], pr1[NR][1];
-    km_count<K, 1>(b0, np, D.quads[0], pre, voff_f, 0, pr0);
-    km_count<K, 1>(b1, np, D.quads[1], pre, voff_f, 0, pr1);
+    if constexpr (!ALL) {
+        km_count<K, 1>(b0, np, D.quads[0], pre, voff_f, 0, pr0);
+        km_count<K, 1>(b1, np, D.quads[1], pre, voff_f, 0, pr1);
+    }
     const float tau = st->tau;
     const uint64_t tau_combo = st->tau_combo;
     // sums over the L lanes of this c, in the same order in each
@@ -818,15 +972,20 @@ __device__ __forceinline__ void km_unit(const DevData &D, const ScanParams &P, T
         const int c = c0 + lane / L;
         const bool valid = lane_ok && c < c_end;
         const int voff_c = 16 * min(c, np - 1);
-        uint32_t k0[NR][2], k1[NR][2];
-        km_count<K, 2>(b0, np, D.quads[0], pre, voff_f, voff_c, k0);
-        km_count<K, 2>(b1, np, D.quads[1], pre, voff_f, voff_c, k1);
-        // this lane's cell x = 3 r + z (z = 2: the prefix row's count minus
-        // genotypes 0 and 1), computed where it is used
+        uint32_t k0[NR][Z], k1[NR][Z];
+        km_count<K, Z>(b0, np, D.quads[0], pre, voff_f, voff_c, k0);
+        km_count<K, Z>(b1, np, D.quads[1], pre, voff_f, voff_c, k1);
+        // this lane's cell x = 3 r + z (z = 2, marginal form: the prefix
+        // row's count minus genotypes 0 and 1), computed where it is used
         auto cnt = [&](int x, uint32_t &n0, uint32_t &n1) __attribute__((always_inline)) {
             const int r = x / 3, z = x % 3;
-            n0 = z < 2 ? k0[r][z] : pr0[r][0] - k0[r][0] - k0[r][1];
-            n1 = z < 2 ? k1[r][z] : pr1[r][0] - k1[r][0] - k1[r][1];
+            if constexpr (ALL) {
+                n0 = k0[r][z];
+                n1 = k1[r][z];
+            } else {
+                n0 = z < 2 ? k0[r][z] : pr0[r][0] - k0[r][0] - k0[r][1];
+                n1 = z < 2 ? k1[r][z] : pr1[r][0] - k1[r][0] - k1[r][1];
+            }
         };
         auto cnt_screen = [&](int x, float &fc, float &ft) __attribute__((always_inline)) {
             uint32_t n0, n1;
@@ -867,21 +1026,22 @@ __device__ __forceinline__ void km_unit(const DevData &D, const ScanParams &P, T
                 if (exact && part == L - 1) {
 #pragma clang fp contract(off)
                     mf = (float)(hx + P.hy64 - hall);
-                    want = better(mf, pre_packed | (uint64_t)c, tau, tau_combo);
+                    want = better(mf, pre_key + (uint64_t)c, tau, tau_combo);
                 }
             }
-            push_candidates(want, mf, pre_packed | (uint64_t)c, P, st);
+            push_candidates(want, mf, pre_key + (uint64_t)c, P, st);
         }
     }
 }
 
-template <int K, bool HIST>
+template <int K, bool HIST, bool ALL>
 __global__ void __launch_bounds__(kBlock) k_scan_km(const DevData D, const ScanParams P, TopState *st)
 {
     __shared__ uint32_t lds_hist[HIST ? kHistBins : 1];
     hist_begin<HIST>(lds_hist);
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    km_unit<K, HIST>(D, P, st, P.j0 + (int64_t)blockIdx.x * (kBlock / kWave) + wid, P.j1, INT32_MAX, 0, lds_hist);
+    km_unit<K, HIST, ALL>(D, P, st, P.j0 + (int64_t)blockIdx.x * (kBlock / kWave) + wid, P.j1, INT32_MAX, 0,
+                          lds_hist);
     hist_end<HIST>(lds_hist, P.hist);  // every wave of the block reaches the barrier
 }
 
@@ -1042,8 +1202,9 @@ __device__ __forceinline__ void scan2m_unit(const DevData &D, const ScanParams &
                 m[8] = D.total[cls];
                 mobius2(m);
             }
-            score_or_sample<HIST, 9>(valid, m0, m1, ((uint64_t)pa[i] << P.combo_bits) | (uint64_t)c, tau, tau_combo,
-                                     P, st, lds_hist);
+            score_or_sample<HIST, 9>(
+                valid, m0, m1, [&] { return prefix_key(P.binom, D.n, &pa[i], 1) + (uint64_t)c; }, tau, tau_combo, P,
+                st, lds_hist);
         }
     }
 }
@@ -1223,8 +1384,11 @@ __device__ __forceinline__ void scan3m_unit(const DevData &D, const ScanParams &
                 m[26] = D.total[cls];
                 mobius3(m);
             }
-            const uint64_t combo = ((((uint64_t)pa[i] << P.combo_bits) | (uint64_t)pb[i]) << P.combo_bits) | (uint64_t)c;
-            score_or_sample<HIST, 27>(valid, m0, m1, combo, tau, tau_combo, P, st, lds_hist);
+            auto key = [&] {
+                const int pre[2] = {pa[i], pb[i]};
+                return prefix_key(P.binom, n, pre, 2) + (uint64_t)c;
+            };
+            score_or_sample<HIST, 27>(valid, m0, m1, key, tau, tau_combo, P, st, lds_hist);
         }
     }
 }
@@ -1588,7 +1752,8 @@ __device__ __forceinline__ void mfma2_tile(const DevData &D, const ScanParams &P
                 m[8] = (Cnt)D.total[cls];
                 mobius2(m);
             }
-            score_and_push<9>(valid, m0, m1, ((uint64_t)a << P.combo_bits) | (uint64_t)b, tau, tau_combo, P, st);
+            score_and_push<9>(valid, m0, m1, [&] { return prefix_key(P.binom, D.n, &a, 1) + (uint64_t)b; }, tau,
+                              tau_combo, P, st);
         }
 }
 
@@ -1855,8 +2020,11 @@ __device__ __forceinline__ void mfma3_tile(const DevData &D, const ScanParams &P
                 mm[26] = (float)D.total[cls];
                 mobius3(mm);
             }
-            const uint64_t combo = ((((uint64_t)a << P.combo_bits) | (uint64_t)b) << P.combo_bits) | (uint64_t)c;
-            score_and_push<27>(valid, m0, m1, combo, tau, tau_combo, P, st);
+            auto key = [&] {
+                const int pre[2] = {a, b};
+                return prefix_key(P.binom, n, pre, 2) + (uint64_t)c;
+            };
+            score_and_push<27>(valid, m0, m1, key, tau, tau_combo, P, st);
         });
 }
 
@@ -2045,6 +2213,90 @@ __device__ __forceinline__ void merge_topn(TopState *st, Cand *top, const Cand *
     }
 }
 
+// ---------------------------------------------------------------------------
+// Top-N merge for N above kMaxOutputs (global memory).  The host reads the
+// list's and the candidates' sizes first (a sync per chunk on this path only)
+// and sorts next_pow2(n_top + count) entries: gather, one bitonic step per
+// launch, then one workgroup keeps the first N unique entries in order.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ Cand empty_cand()
+{
+    Cand e;
+    e.mi = -INFINITY;
+    e.tag = 0;
+    e.combo = ~0ull;
+    return e;
+}
+
+__global__ void k_big_gather(const Cand *top, uint32_t n_top, const Cand *cand, uint32_t count, Cand *work, uint32_t L)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= L) return;
+    work[i] = i < n_top ? top[i] : i < n_top + count ? cand[i - n_top] : empty_cand();
+}
+
+// One compare-exchange step (k, jj) of a bitonic sort of L entries into
+// key order (MI descending, combination ascending).
+__global__ void k_big_bitonic(Cand *work, uint32_t L, uint32_t k, uint32_t jj)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= L) return;
+    const uint32_t ixj = i ^ jj;
+    if (ixj <= i) return;
+    const Cand a = work[i], b = work[ixj];
+    const bool desc = (i & k) == 0;
+    const bool sw = desc ? better(b.mi, b.combo, a.mi, a.combo) : better(a.mi, a.combo, b.mi, b.combo);
+    if (sw) {
+        work[i] = b;
+        work[ixj] = a;
+    }
+}
+
+// The first N unique non-empty entries of the sorted work buffer -> top;
+// state: list size, threshold, candidates consumed, overflow -> re-scan list.
+__global__ void __launch_bounds__(1024)
+k_big_finish(const Cand *work, uint32_t L, Cand *top, uint32_t N, TopState *st, uint32_t chunk_id, uint32_t *redo)
+{
+    __shared__ uint32_t psum[1024];
+    const int tid = threadIdx.x;
+    const uint32_t seg = (L + 1023) / 1024;
+    const uint32_t s0 = min(L, tid * seg), s1 = min(L, s0 + seg);
+    auto keep = [&](uint32_t i) {
+        const Cand c = work[i];
+        const bool empty = c.combo == ~0ull && c.mi == -INFINITY;
+        const bool dup = i > 0 && c.combo == work[i - 1].combo && c.mi == work[i - 1].mi && c.tag == work[i - 1].tag;
+        return !empty && !dup;
+    };
+    uint32_t local = 0;
+    for (uint32_t i = s0; i < s1; ++i) local += keep(i);
+    psum[tid] = local;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const uint32_t v = tid >= off ? psum[tid - off] : 0;
+        __syncthreads();
+        psum[tid] += v;
+        __syncthreads();
+    }
+    uint32_t pos = psum[tid] - local;
+    for (uint32_t i = s0; i < s1 && pos < N; ++i)
+        if (keep(i)) top[pos++] = work[i];
+    __threadfence_block();
+    __syncthreads();
+    if (tid == 0) {
+        const uint32_t n_top = min(N, psum[1023]);
+        st->n_top = n_top;
+        if (n_top >= N && N > 0) {
+            st->tau = top[N - 1].mi;
+            st->tau_combo = top[N - 1].combo;
+        }
+        st->cand_count = 0;
+        if (st->overflow) {
+            redo[st->redo_count++] = chunk_id;
+            st->overflow = 0;
+        }
+    }
+}
+
 __global__ void __launch_bounds__(1024)
 k_merge(TopState *st, Cand *top, const Cand *cand, uint32_t N, uint32_t cap, uint32_t chunk_id,
         uint32_t *redo)
@@ -2120,7 +2372,8 @@ k_batch_scan(const BatchJob *__restrict__ jobs, const BatchSeg *__restrict__ seg
             direct_unit<4, false>(D, P, st, j0 + g, j1, t_hi, t_lo, nullptr);
         } else {
             if (J.kind == kKindKm4) km_unit<4, false>(D, P, st, j0 + g, j1, t_hi, t_lo, nullptr);
-            else km_unit<5, false>(D, P, st, j0 + g, j1, t_hi, t_lo, nullptr);
+            else if (J.kind == kKindKm5) km_unit<5, false>(D, P, st, j0 + g, j1, t_hi, t_lo, nullptr);
+            else km_unit<5, false, true>(D, P, st, j0 + g, j1, t_hi, t_lo, nullptr);
         }
     }
 }
@@ -2429,38 +2682,52 @@ static hipError_t scan_direct(int order, const DevData &D, const ScanParams &P, 
     case 7: hipLaunchKernelGGL((k_scan_any<7, HIST>), dim3((unsigned)waves), block, 0, s, D, P, st); break;
     case 8: hipLaunchKernelGGL((k_scan_any<8, HIST>), dim3((unsigned)waves), block, 0, s, D, P, st); break;
     case 9: hipLaunchKernelGGL((k_scan_any<9, HIST>), dim3((unsigned)waves), block, 0, s, D, P, st); break;
+    // persistent, one count slot of P.cells per workgroup
+    case 10: case 11: case 12: {
+        if (!P.cells) return hipErrorInvalidValue;
+        const dim3 g((unsigned)std::min<int64_t>(waves, big_slots(order)));
+        if (order == 10) hipLaunchKernelGGL((k_scan_big<10, HIST>), g, block, 0, s, D, P, st, P.cells);
+        else if (order == 11) hipLaunchKernelGGL((k_scan_big<11, HIST>), g, block, 0, s, D, P, st, P.cells);
+        else hipLaunchKernelGGL((k_scan_big<12, HIST>), g, block, 0, s, D, P, st, P.cells);
+        break;
+    }
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
-hipError_t launch_scan_km(int order, const DevData &D, const ScanParams &P, TopState *st, hipStream_t s)
+hipError_t launch_scan_km(int order, const DevData &D, const ScanParams &P, TopState *st, hipStream_t s,
+                          bool all_cells)
 {
     const int64_t prefixes = P.j1 - P.j0;
     if (prefixes <= 0) return hipSuccess;
     const int64_t per_block = kBlock / kWave;
     const dim3 grid((unsigned)((prefixes + per_block - 1) / per_block)), block(kBlock);
     const bool hist = P.hist != nullptr;
+#define FIU_KM(K, H, A) hipLaunchKernelGGL((k_scan_km<K, H, A>), grid, block, 0, s, D, P, st)
+#define FIU_KM_ORDER(K)                                                     \
+    if (all_cells) {                                                       \
+        if (hist) FIU_KM(K, true, true);                                   \
+        else FIU_KM(K, false, true);                                       \
+    } else {                                                               \
+        if (hist) FIU_KM(K, true, false);                                  \
+        else FIU_KM(K, false, false);                                      \
+    }
     switch (order) {
-    case 4:
-        if (hist) hipLaunchKernelGGL((k_scan_km<4, true>), grid, block, 0, s, D, P, st);
-        else hipLaunchKernelGGL((k_scan_km<4, false>), grid, block, 0, s, D, P, st);
-        break;
-    case 5:
-        if (hist) hipLaunchKernelGGL((k_scan_km<5, true>), grid, block, 0, s, D, P, st);
-        else hipLaunchKernelGGL((k_scan_km<5, false>), grid, block, 0, s, D, P, st);
-        break;
-    case 6:
-        if (hist) hipLaunchKernelGGL((k_scan_km<6, true>), grid, block, 0, s, D, P, st);
-        else hipLaunchKernelGGL((k_scan_km<6, false>), grid, block, 0, s, D, P, st);
-        break;
+    case 4: FIU_KM_ORDER(4) break;
+    case 5: FIU_KM_ORDER(5) break;
+    case 6: FIU_KM_ORDER(6) break;
     default: return hipErrorInvalidValue;
     }
+#undef FIU_KM_ORDER
+#undef FIU_KM
     return hipGetLastError();
 }
 
 hipError_t launch_scan_direct(int order, const DevData &D, const ScanParams &P, TopState *st, hipStream_t s)
 {
+    // orders 5 and 6: every cell, the prefix rows split over lanes
+    if (order == 5 || order == 6) return launch_scan_km(order, D, P, st, s, true);
     return P.hist ? scan_direct<true>(order, D, P, st, s) : scan_direct<false>(order, D, P, st, s);
 }
 
@@ -2671,6 +2938,19 @@ hipError_t launch_mi_batch(const uint32_t *tables, uint64_t n, uint64_t ct_size,
     if (!n) return hipSuccess;
     hipLaunchKernelGGL(k_mi_batch, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, tables, n, ct_size, n64, hy64,
                        out);
+    return hipGetLastError();
+}
+
+hipError_t launch_merge_big(TopState *st, Cand *top, uint32_t n_top, const Cand *cand, uint32_t count, uint32_t N,
+                            Cand *work, uint32_t L, uint32_t chunk_id, uint32_t *redo, hipStream_t s)
+{
+    if (L < 2 || (L & (L - 1)) || L < n_top + count) return hipErrorInvalidValue;
+    const unsigned blocks = (L + 255) / 256;
+    hipLaunchKernelGGL(k_big_gather, dim3(blocks), dim3(256), 0, s, top, n_top, cand, count, work, L);
+    for (uint32_t k = 2; k <= L; k <<= 1)
+        for (uint32_t jj = k >> 1; jj > 0; jj >>= 1)
+            hipLaunchKernelGGL(k_big_bitonic, dim3(blocks), dim3(256), 0, s, work, L, k, jj);
+    hipLaunchKernelGGL(k_big_finish, dim3(1), dim3(1024), 0, s, work, L, top, N, st, chunk_id, redo);
     return hipGetLastError();
 }
 

@@ -118,7 +118,8 @@ fiu_status fiu_dataset_destroy(fiu_dataset *ds);
 typedef enum {
     FIU_ALGO_AUTO = 0,     /* fastest exact path the dataset allows */
     FIU_ALGO_DIRECT = 1,   /* AND+popcount of all 3^k cells (the reference's);
-                              orders 2-4 and 7-9 (the only algorithm at 7-9) */
+                              every order, any dataset (the only algorithm
+                              at orders 7-12) */
     FIU_ALGO_MARGINAL = 2, /* orders 2-3: 2^k cells + inclusion-exclusion
                               from exact lower-order counts; orders 4-6: the
                               prefix's 3^(k-1) rows against genotypes 0 and 1
@@ -166,9 +167,11 @@ typedef struct {
     int32_t variant;         /* its kernel variant (fiu_tune_entry) */
 } fiu_search_stats;
 
-/* Exhaustive order-k search (k = 2..9; orders 5-6 need every SNP to cover the
- * same samples; a combination packs in 64 bits, so k * ceil(log2(snps)) <= 64,
- * e.g. at most 512 SNPs at order 7, 128 at order 9) over the combinations whose (k-1)-prefix belongs
+/* Exhaustive order-k search (k = 2..12, any dataset; a result is keyed by
+ * its lexicographic rank among the C(snps, k) combinations, so C(snps, k)
+ * must stay below 2^64 - 1: at most 145,056 SNPs at order 4, 18,580 at
+ * order 5, 4,868 at order 6, 1,913 at order 7, 967 at order 8, 577 at order
+ * 9, 386 at order 10, 282 at order 11, 218 at order 12) over the combinations whose (k-1)-prefix belongs
  * to Distribution<int>(snps, order-1, dist_step, dist_offset)
  * (Distribution.h:147-176), keeping the `outputs` best by mutual information.
  * Results are ordered by MI descending, ties by combination ascending;
@@ -212,6 +215,44 @@ fiu_status fiu_contingency_batch(const fiu_dataset *ds, const int32_t *combos,
  * `ctrls` controls (MutualInformation.h:58-65, 83-84). */
 fiu_status fiu_mi_batch(const uint32_t *tables, size_t n, uint16_t order,
                         uint32_t cases, uint32_t ctrls, float *out_mi);
+
+/* ------------------------------------------------------------------------ */
+/* Result formats (host only)                                                */
+/* ------------------------------------------------------------------------ */
+
+/* The text of one result as Result<int, float>::str writes it
+ * (include/fiuncho/utils/Result.h:37-45): each SNP index followed by a
+ * space, then the MI as a default-formatted stream writes a float (6
+ * significant digits).  Writes at most cap - 1 characters and a NUL (buf may
+ * be NULL with cap 0); returns the full length, as snprintf does.
+ * Replaces: Result::str. */
+size_t fiu_result_str(const int32_t *combo, uint16_t order, float mi,
+                      char *buf, size_t cap);
+
+/* The binary record of Result<int, float>::serialize (Result.h:47-55):
+ * the order as a size_t, the order indices as int32, the MI as float, in host
+ * byte order.  Returns the record's size; writes it only when cap is at least
+ * that (buf may be NULL to ask for the size).
+ * Replaces: Result::serialize. */
+size_t fiu_result_serialize(const int32_t *combo, uint16_t order, float mi,
+                            void *buf, size_t cap);
+
+/* Read one record written by Result::serialize / fiu_result_serialize
+ * (Result.h:57-66) from buf (len bytes available): *order indices into combo
+ * (room for max_order), the MI into *mi, the record's size into *used (may be
+ * NULL).  FIU_ERR_INVALID for a truncated record or an order above
+ * max_order.
+ * Replaces: Result::deserialize. */
+fiu_status fiu_result_deserialize(const void *buf, size_t len, int32_t *combo,
+                                  uint16_t max_order, uint16_t *order,
+                                  float *mi, size_t *used);
+
+/* Write count results (order indices each in combos, MI in mi) to a text
+ * file, one fiu_result_str line and a newline each, as the reference's
+ * command line writes its output file (app/main.cpp:152-158); FIU_ERR_IO when
+ * the file cannot be written. */
+fiu_status fiu_write_results(const char *path, const int32_t *combos,
+                             const float *mi, uint32_t count, uint16_t order);
 
 /* ------------------------------------------------------------------------ */
 /* Tuning table                                                              */

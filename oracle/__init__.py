@@ -95,6 +95,21 @@ class Dataset:
         return t[:3 * self.cases_words], t[3 * self.cases_words:]
 
 
+class WordsDataset:
+    """Tables already in the reference host layout (a caller's array, kept
+    alive here), for counts() and search()."""
+
+    def __init__(self, words, snps, cases, ctrls, cases_words, ctrls_words):
+        self.words = np.ascontiguousarray(words, dtype=np.uint64)
+        assert self.words.size == snps * 3 * (cases_words + ctrls_words)
+        self.snps, self.cases, self.ctrls = snps, cases, ctrls
+        self.cases_words, self.ctrls_words = cases_words, ctrls_words
+
+
+def from_words(words, snps, cases, ctrls, cases_words, ctrls_words):
+    return WordsDataset(words, snps, cases, ctrls, cases_words, ctrls_words)
+
+
 def _err():
     return C.create_string_buffer(512)
 
@@ -210,6 +225,9 @@ class Ref:
         L.ref_combine_pair.argtypes = [C.c_void_p, C.c_int, C.c_int, u64p]
         L.ref_mi.restype = C.c_float
         L.ref_mi.argtypes = [u32p, u32p, C.c_int, C.c_uint, C.c_uint]
+        for f in ("ref_result_str", "ref_result_serialize"):
+            getattr(L, f).restype = C.c_size_t
+            getattr(L, f).argtypes = [i32p, C.c_int, C.c_float, C.c_char_p, C.c_size_t]
         L.ref_search.restype = C.c_long
         L.ref_search.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_uint, C.c_uint,
                                  i32p, f32p, C.POINTER(C.c_double)]
@@ -220,6 +238,22 @@ class Ref:
         if not h:
             raise RuntimeError(self.L.ref_last_error().decode())
         return RefDataset(self, h)
+
+    def result_str(self, combo, val):
+        """Result<int, float>::str of one result (bytes)."""
+        c = np.ascontiguousarray(combo, np.int32)
+        n = self.L.ref_result_str(c, len(c), val, None, 0)
+        buf = C.create_string_buffer(n + 1)
+        self.L.ref_result_str(c, len(c), val, buf, n)
+        return buf.raw[:n]
+
+    def result_serialize(self, combo, val):
+        """Result<int, float>::serialize of one result (bytes)."""
+        c = np.ascontiguousarray(combo, np.int32)
+        n = self.L.ref_result_serialize(c, len(c), val, None, 0)
+        buf = C.create_string_buffer(n)
+        self.L.ref_result_serialize(c, len(c), val, buf, n)
+        return buf.raw[:n]
 
     def mi(self, cc, ct, k, cases, ctrls):
         return float(self.L.ref_mi(np.ascontiguousarray(cc, np.uint32),

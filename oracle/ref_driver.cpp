@@ -13,9 +13,11 @@
 #include <fiuncho/GenotypeTable.h>
 #include <fiuncho/ThreadedSearch.h>
 #include <fiuncho/algorithms/MutualInformation.h>
+#include <fiuncho/utils/Result.h>
 
 #include <chrono>
 #include <cstring>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -145,6 +147,33 @@ long ref_search(void *h, int order, int step, int offset, unsigned outputs,
         g_err = e.what();
         return -1;
     }
+}
+
+// Result<int, float>::str (Result.h:37-45) and Result::serialize
+// (Result.h:47-55) of one result: copies at most cap bytes into buf (str adds
+// no terminator) and returns the full size.
+static size_t copy_out(const std::string &s, char *buf, size_t cap)
+{
+    if (buf) std::memcpy(buf, s.data(), s.size() < cap ? s.size() : cap);
+    return s.size();
+}
+
+size_t ref_result_str(const int *combo, int k, float val, char *buf, size_t cap)
+{
+    Result<int, float> r;
+    r.combination.assign(combo, combo + k);
+    r.val = val;
+    return copy_out(r.str(), buf, cap);
+}
+
+size_t ref_result_serialize(const int *combo, int k, float val, char *buf, size_t cap)
+{
+    Result<int, float> r;
+    r.combination.assign(combo, combo + k);
+    r.val = val;
+    std::ostringstream os;
+    Result<int, float>::serialize(os, r);
+    return copy_out(os.str(), buf, cap);
 }
 
 } // extern "C"

@@ -106,7 +106,7 @@ def test_tuning_table_roundtrip_and_validation():
     E = tune.Entry
     good = [E(2, 0, 1000, 0, 4096, fa.ALGO_MARGINAL, 8, 1.5), E(3, 100, 200, 10, 20, fa.ALGO_MFMA, 3),
             E(4, 0, 50, 0, 100, fa.ALGO_DIRECT, 0), E(5, 0, 50, 0, 100, fa.ALGO_MARGINAL, 0),
-            E(7, 0, 50, 0, 100, fa.ALGO_DIRECT, 0)]
+            E(7, 0, 50, 0, 100, fa.ALGO_DIRECT, 0), E(6, 0, 50, 0, 100, fa.ALGO_DIRECT, 0)]
     try:
         tune.set_table(good)
         got = tune.get_table()
@@ -117,9 +117,9 @@ def test_tuning_table_roundtrip_and_validation():
                E(2, 0, 10, 0, 10, fa.ALGO_MFMA, 3),         # ring depth: order 3 only
                E(3, 0, 10, 0, 10, fa.ALGO_MFMA, 5),         # no such staging variant
                E(4, 0, 10, 0, 10, fa.ALGO_MFMA, 0),         # matrix cores: orders 2-3
-               E(5, 0, 10, 0, 10, fa.ALGO_DIRECT, 0),       # all-cells kernels: orders 2-4 and 7-8
+               E(13, 0, 10, 0, 10, fa.ALGO_DIRECT, 0),      # orders 2-12 only
                E(7, 0, 10, 0, 10, fa.ALGO_MARGINAL, 0),     # marginal: orders 2-6
-               E(10, 0, 10, 0, 10, fa.ALGO_DIRECT, 0),      # above order 9
+               E(10, 0, 10, 0, 10, fa.ALGO_MARGINAL, 0),    # marginal above order 6
                E(4, 0, 10, 0, 10, fa.ALGO_MARGINAL, 4),     # prefixes per wave: orders 2-3
                E(2, 10, 10, 0, 10, fa.ALGO_DIRECT, 0),      # empty SNP range
                E(2, 0, 10, 0, 10, fa.ALGO_AUTO, 0),         # AUTO is not a choice

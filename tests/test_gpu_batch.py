@@ -144,9 +144,12 @@ def test_batch_edge_cases():
     assert same(res[3], plain(d, 2, fa.Distribution(12, 1), 1000))
     with pytest.raises(fa.FiuError):
         fa.search_batch([(d, 6, fa.Distribution(12, 5), 10)])   # above order 5
-    d_inc, _ = incomplete(12, 60, 60, 43)
-    with pytest.raises(fa.FiuError):
-        fa.search_batch([(d_inc, 5, fa.Distribution(12, 4), 10)])   # order 5 needs complete genotypes
+    # order 5 on incomplete genotypes: every cell counted (its own kind)
+    d_inc, od_inc = incomplete(12, 60, 60, 43)
+    res, _ = fa.search_batch([(d_inc, 5, fa.Distribution(12, 4), 10), (d, 2, fa.Distribution(12, 1), 5)])
+    assert same(res[0], plain(d_inc, 5, fa.Distribution(12, 4), 10))
+    oc, om, _ = O.search(od_inc, 5, 10, threads=2, mi_mode=2)
+    assert res[0][0].tolist() == oc.tolist() and np.array_equal(res[0][1], om)
     assert fa.search_batch([])[0] == []
 
 

@@ -84,34 +84,107 @@ def test_marginal_orders_shares_and_chunks(order):
 
 def test_incomplete_data():
     # one uncalled genotype: order 4 falls back to the direct kernel, orders
-    # 5 and 6 (marginal only) are refused
+    # 5 and 6 count every cell with the prefix rows split over lanes, orders
+    # 7 to 12 count every cell per workgroup -- all against the oracle
     rng = np.random.default_rng(7)
-    geno = rng.integers(0, 3, size=(12, 300)).astype(np.int8)
+    geno = rng.integers(0, 3, size=(13, 300)).astype(np.int8)
     geno[4, 17] = 3
     is_ctrl = (np.arange(300) >= 140).astype(np.uint8)
     od = O.pack(geno, is_ctrl)
-    d = fa.Dataset.from_words(od.words, 12, od.cases_words, od.ctrls_words, 140, 160)
-    s = fa.GpuSearch()
-    c, m = s.run_arrays(d, 4, fa.Distribution(12, 3), 50)
-    assert s.last_stats.algorithm == fa.ALGO_DIRECT
-    oc, om, _ = O.search(od, 4, 50, threads=2, mi_mode=2)
-    assert c.tolist() == oc.tolist() and np.array_equal(m, om)
-    with pytest.raises(fa.FiuError):
-        fa.GpuSearch().run_arrays(d, 5, fa.Distribution(12, 4), 10)
-    with pytest.raises(fa.FiuError):
-        fa.GpuSearch(algorithm=fa.ALGO_DIRECT).run_arrays(d, 5, fa.Distribution(12, 4), 10)
-    with pytest.raises(fa.FiuError):
-        fa.GpuSearch().run_arrays(d, 6, fa.Distribution(12, 5), 10)
-    with pytest.raises(fa.FiuError):
-        fa.GpuSearch().run_arrays(d, 10, fa.Distribution(12, 9), 10)   # above order 9
-    # orders 7 to 9 count every cell: incomplete data is scanned exactly
-    for order in (7, 8, 9):
+    d = fa.Dataset.from_words(od.words, 13, od.cases_words, od.ctrls_words, 140, 160)
+    for order in (4, 5, 6, 7, 8, 9, 10, 11, 12):
         print(f"case: incomplete data order {order}", flush=True)
         s = fa.GpuSearch()
-        c, m = s.run_arrays(d, order, fa.Distribution(12, order - 1), 40)
+        c, m = s.run_arrays(d, order, fa.Distribution(13, order - 1), 40)
         assert s.last_stats.algorithm == fa.ALGO_DIRECT
-        oc, om, _ = O.search(od, order, 40, threads=2, mi_mode=2)
+        oc, om, _ = O.search(od, order, 40, threads=4, mi_mode=2)
+        assert c.tolist() == oc.tolist() and np.array_equal(m, om), order
+    with pytest.raises(fa.FiuError):
+        fa.GpuSearch(algorithm=fa.ALGO_MARGINAL).run_arrays(d, 5, fa.Distribution(13, 4), 10)
+    with pytest.raises(fa.FiuError):
+        fa.GpuSearch().run_arrays(d, 13, fa.Distribution(13, 12), 10)   # above order 12
+
+
+def test_orders_5_6_all_cells_on_complete_data():
+    # FIU_ALGO_DIRECT at orders 5 and 6 (every cell counted) equals the
+    # marginal scan and the oracle; shares and tiny chunks included
+    for order, snps in ((5, 22), (6, 14)):
+        print(f"case: order {order} all cells vs marginal", flush=True)
+        d, od = synth(snps, 600, 520, 70 + order)
+        dist = fa.Distribution(snps, order - 1)
+        s = fa.GpuSearch(algorithm=fa.ALGO_DIRECT)
+        c, m = s.run_arrays(d, order, dist, 200)
+        assert s.last_stats.algorithm == fa.ALGO_DIRECT
+        mc, mm = fa.GpuSearch(algorithm=fa.ALGO_MARGINAL).run_arrays(d, order, dist, 200)
+        assert c.tolist() == mc.tolist() and np.array_equal(m, mm)
+        oc, om, _ = O.search(od, order, 200, threads=4, mi_mode=2)
         assert c.tolist() == oc.tolist() and np.array_equal(m, om)
+        share = fa.Distribution(snps, order - 1).layer(4, 3)
+        c, m = fa.GpuSearch(algorithm=fa.ALGO_DIRECT).run_arrays(d, order, share, 50)
+        oc, om, _ = O.search(od, order, 50, step=4, offset=3, threads=1, mi_mode=2)
+        assert c.tolist() == oc.tolist() and np.array_equal(m, om)
+        s = fa.GpuSearch(algorithm=fa.ALGO_DIRECT, chunk_combos=300, candidate_cap=64, sample_threshold=False)
+        c, m = s.run_arrays(d, order, dist, 40)
+        oc, om, _ = O.search(od, order, 40, threads=4, mi_mode=2)
+        assert c.tolist() == oc.tolist() and np.array_equal(m, om)
+        assert s.last_stats.redo_chunks > 0
+
+
+@pytest.mark.parametrize("order,snps,cases,ctrls,outputs,seed", [
+    (10, 12, 300, 260, 66, 81),     # every combination (C(12, 10) = 66)
+    (10, 13, 130, 70, 100, 82),     # 286 combinations, ragged classes
+    (11, 13, 200, 180, 78, 83),     # every combination (C(13, 11) = 78)
+    (12, 14, 160, 150, 40, 84),     # 91 combinations
+    (12, 12, 64, 64, 1, 85),        # the single combination
+])
+def test_orders_10_to_12_vs_oracle(order, snps, cases, ctrls, outputs, seed):
+    # 3^k counts per class in global memory (k_scan_big)
+    print(f"case: order {order} {snps}x{cases}+{ctrls}", flush=True)
+    d, od = synth(snps, cases, ctrls, seed)
+    oc, om, _ = O.search(od, order, outputs, threads=4, mi_mode=2)
+    s = fa.GpuSearch()
+    c, m = s.run_arrays(d, order, fa.Distribution(snps, order - 1), outputs)
+    assert s.last_stats.algorithm == fa.ALGO_DIRECT
+    assert c.tolist() == oc.tolist() and np.array_equal(m, om)
+    assert s.last_stats.combinations == math.comb(snps, order)
+    # a share and tiny chunks (re-scans)
+    share = fa.Distribution(snps, order - 1).layer(3, 2)
+    c, m = fa.GpuSearch().run_arrays(d, order, share, outputs)
+    sc, sm, _ = O.search(od, order, outputs, step=3, offset=2, threads=1, mi_mode=2)
+    assert c.tolist() == sc.tolist() and np.array_equal(m, sm)
+
+
+def test_wide_keys_beyond_the_packed_limit():
+    # combination keys are lexicographic ranks: orders 7 and 9 run on more
+    # SNPs than k * ceil(log2 n) <= 64 allowed (512 and 128); a share of a
+    # few prefixes keeps the scan small.  Every returned combination lies in
+    # the share, and its MI is the FP64 MI of the oracle's counts of it.
+    for order, snps, step in ((7, 600, (1 << 31) - 1), (9, 200, (1 << 31) - 1)):
+        print(f"case: wide keys order {order}, {snps} SNPs", flush=True)
+        d, od = synth(snps, 64, 64, 90 + order)
+        total = math.comb(snps, order - 1)
+        for off in (0, total // 2 % step, total % step - 1):
+            dist = fa.Distribution(snps, order - 1, step, off)
+            s = fa.GpuSearch()
+            c, m = s.run_arrays(d, order, dist, 30)
+            assert len(c) > 0
+            assert (np.diff(c, axis=1) > 0).all() and c.max() < snps
+            for x, v in zip(c.tolist(), m):
+                # lexicographic rank of the prefix among the (order-1)-combinations
+                rank, prev = 0, -1
+                for i, xi in enumerate(x[:-1]):
+                    for v2 in range(prev + 1, xi):
+                        rank += math.comb(snps - 1 - v2, order - 2 - i)
+                    prev = xi
+                assert rank % step == off, (x, rank)
+                cc, ct = O.counts(od, x)
+                assert abs(float(v) - O.mi_f64(cc, ct, order, 64, 64)) <= 0.5 * float(np.spacing(np.float32(v))) + 1e-12
+            keys = [(-float(v), tuple(x)) for x, v in zip(c.tolist(), m)]
+            assert keys == sorted(keys)
+            # the same share in tiny chunks: identical
+            t = fa.GpuSearch(chunk_combos=200, candidate_cap=128, sample_threshold=False)
+            c2, m2 = t.run_arrays(d, order, dist, 30)
+            assert c2.tolist() == c.tolist() and np.array_equal(m2, m)
 
 
 @pytest.mark.parametrize("order,snps,cases,ctrls,outputs,seed", [
