@@ -49,10 +49,31 @@ uint64_t binom_sat(uint64_t n, uint64_t k)
     return (uint64_t)r;
 }
 
+// Lexicographic unranking of an m-combination of [0, n) on the host (the
+// device's unrank, with binom_sat for the table).
+void unrank_host(uint64_t r, uint64_t n, int m, int32_t *out)
+{
+    uint64_t x = binom_sat(n, (uint64_t)m) - 1 - r;
+    int64_t hi = (int64_t)n - 1;
+    for (int i = 0; i < m; ++i) {
+        const uint64_t j = (uint64_t)(m - i);
+        int64_t lo = 0, h = hi;   // largest lo <= hi with C(lo, j) <= x
+        while (lo < h) {
+            const int64_t mid = (lo + h + 1) / 2;
+            if (binom_sat((uint64_t)mid, j) <= x) lo = mid;
+            else h = mid - 1;
+        }
+        x -= binom_sat((uint64_t)lo, j);
+        out[i] = (int32_t)((int64_t)n - 1 - lo);
+        hi = lo - 1;
+    }
+}
+
 }  // namespace fiu
 
 fiu_dataset::~fiu_dataset()
 {
+    if (readback) (void)hipHostFree(readback);
     for (auto e : events) (void)hipEventDestroy(e);
     if (stream && own_stream) (void)hipStreamDestroy(stream);
     for (auto &r : retired) {
@@ -540,6 +561,19 @@ uint64_t count_combinations(uint64_t n, int m, uint64_t step, uint64_t offset)
 {
     constexpr int kMaxM = 32;
     if (m < 1 || m > kMaxM) return 0;
+    // far fewer prefixes in the share than run heads (a large step): unrank
+    // each prefix of the share instead
+    const uint64_t all = binom_sat(n, (uint64_t)m);
+    const uint64_t n_pref = all > offset ? (all - offset + step - 1) / step : 0;
+    if (n_pref < binom_sat(n, (uint64_t)m - 1) / 64) {
+        int32_t c[kMaxM];
+        uint64_t sum = 0;
+        for (uint64_t j = 0; j < n_pref; ++j) {
+            unrank_host(offset + j * step, n, m, c);
+            sum += n - 1 - (uint64_t)c[m - 1];
+        }
+        return sum;
+    }
     uint64_t total = 0, rank = 0;
     int64_t c[kMaxM];
     const int h = m - 1;   // indices that head a run
@@ -580,6 +614,19 @@ int64_t tail_prefixes(int n, int M, uint64_t step, uint64_t offset, uint64_t tot
 {
     constexpr int kMaxM = 31;
     if (n_pref <= 0 || total == 0 || M < 1 || M > kMaxM) return 0;
+    if (step > 64) {
+        // a large step: unrank the share's last prefixes directly
+        int32_t c[kMaxM];
+        uint64_t acc = 0;
+        int64_t m = 0;
+        for (int64_t j = n_pref - 1; j >= 0 && m < (1 << 20); --j) {
+            unrank_host(offset + (uint64_t)j * step, (uint64_t)n, M, c);
+            ++m;
+            acc += (uint64_t)(n - 1 - c[M - 1]);
+            if (acc >= target) break;
+        }
+        return m;
+    }
     int c[kMaxM];
     for (int i = 0; i < M; ++i) c[i] = n - M + i;   // the last prefix
     uint64_t rank = total - 1, acc = 0;
@@ -684,6 +731,15 @@ class Workspace {
     virtual uint32_t *big_cells(fiu_dataset *d, int order) = 0;
     // timing event i, or null (the search is not timed)
     virtual hipEvent_t event(fiu_dataset *d, size_t i) = 0;
+    // the search state, after a synchronisation of s; where it can, the
+    // workspace reads the top list's first `outputs` entries in the same
+    // synchronisation, for results()
+    virtual void read_state(fiu_dataset *d, const TopState *state, const Cand *top, uint32_t outputs, TopState &hs,
+                            hipStream_t s)
+    {
+        check(hipMemcpyAsync(&hs, state, sizeof(hs), hipMemcpyDeviceToHost, s), "state read");
+        check(hipStreamSynchronize(s), "search sync");
+    }
     // the final top list to the caller's arrays
     virtual void results(const Cand *top, uint32_t count, int order, int n, const uint64_t *binom, int32_t *out_combos,
                          float *out_mi, hipStream_t s) = 0;
@@ -731,14 +787,43 @@ class DatasetWorkspace final : public Workspace {
         }
         return d->events[i];
     }
+    // up to kMaxOutputs results: the state and the list in one copy each
+    // into the dataset's pinned readback block, one synchronisation
+    void read_state(fiu_dataset *d, const TopState *state, const Cand *top, uint32_t outputs, TopState &hs,
+                    hipStream_t s) override
+    {
+        listed_ = nullptr;
+        if (outputs > (uint32_t)kMaxOutputs) return Workspace::read_state(d, state, top, outputs, hs, s);
+        if (!d->readback) {
+            void *p = nullptr;
+            check(hipHostMalloc(&p, sizeof(TopState) + kMaxOutputs * sizeof(Cand), hipHostMallocDefault),
+                  "hipHostMalloc");
+            d->readback = p;
+        }
+        char *rb = (char *)d->readback;
+        check(hipMemcpyAsync(rb, state, sizeof(TopState), hipMemcpyDeviceToHost, s), "state read");
+        if (outputs)
+            check(hipMemcpyAsync(rb + sizeof(TopState), top, outputs * sizeof(Cand), hipMemcpyDeviceToHost, s),
+                  "top read");
+        check(hipStreamSynchronize(s), "search sync");
+        std::memcpy(&hs, rb, sizeof(hs));
+        listed_ = (const Cand *)(rb + sizeof(TopState));
+    }
     void results(const Cand *top, uint32_t count, int order, int n, const uint64_t *, int32_t *out_combos,
                  float *out_mi, hipStream_t s) override
     {
+        if (listed_) {   // read with the final state
+            unpack_results(listed_, count, order, (uint64_t)n, out_combos, out_mi);
+            return;
+        }
         std::vector<Cand> h(count);
         if (count) check(hipMemcpyAsync(h.data(), top, count * sizeof(Cand), hipMemcpyDeviceToHost, s), "top read");
         check(hipStreamSynchronize(s), "top read sync");
         unpack_results(h.data(), count, order, (uint64_t)n, out_combos, out_mi);
     }
+
+  private:
+    const Cand *listed_ = nullptr;
 };
 
 // Exhaustive search of `d` in workspace `ws` (fiu_search).
@@ -765,7 +850,12 @@ void search(fiu_dataset *d, Workspace &ws, int order, int32_t step, int32_t offs
         fail(FIU_ERR_UNSUPPORTED, "orders 7 to 12 count every cell (FIU_ALGO_DIRECT)");
     if (algo_req != FIU_ALGO_AUTO && !valid_variant(algo_req, order, variant))
         fail(FIU_ERR_INVALID, "unknown variant " + std::to_string(variant) + " of this algorithm");
-    const uint32_t cap = cfg && cfg->candidate_cap ? cfg->candidate_cap : std::min<uint32_t>(1u << 20, ws.max_candidates());
+    // at least `outputs` candidates per chunk: an overflowing chunk then
+    // merges enough entries to set the threshold, and its re-scan pushes only
+    // the entries above the N-th best (fewer than N), so re-scans end
+    const uint32_t cap = std::max(cfg && cfg->candidate_cap ? cfg->candidate_cap
+                                                            : std::min<uint32_t>(1u << 20, ws.max_candidates()),
+                                  outputs);
     if (cap > ws.max_candidates())
         fail(FIU_ERR_UNSUPPORTED, "candidate_cap above this entry point's limit of " +
                                       std::to_string(ws.max_candidates()));
@@ -963,8 +1053,7 @@ void search(fiu_dataset *d, Workspace &ws, int order, int32_t step, int32_t offs
     // the (higher) final threshold until none overflows
     TopState hs;
     for (;;) {
-        check(hipMemcpyAsync(&hs, state, sizeof(hs), hipMemcpyDeviceToHost, s), "state read");
-        check(hipStreamSynchronize(s), "search sync");
+        ws.read_state(d, state, top, outputs, hs, s);
         if (hs.redo_count == 0) break;
         const uint32_t n_redo = hs.redo_count;
         hs.redo_count = 0;
@@ -1006,26 +1095,6 @@ void search(fiu_dataset *d, Workspace &ws, int order, int32_t step, int32_t offs
 }  // namespace
 
 namespace fiu {
-
-// Lexicographic unranking of an m-combination of [0, n) on the host (the
-// device's unrank, with binom_sat for the table).
-static void unrank_host(uint64_t r, uint64_t n, int m, int32_t *out)
-{
-    uint64_t x = binom_sat(n, (uint64_t)m) - 1 - r;
-    int64_t hi = (int64_t)n - 1;
-    for (int i = 0; i < m; ++i) {
-        const uint64_t j = (uint64_t)(m - i);
-        int64_t lo = 0, h = hi;   // largest lo <= hi with C(lo, j) <= x
-        while (lo < h) {
-            const int64_t mid = (lo + h + 1) / 2;
-            if (binom_sat((uint64_t)mid, j) <= x) lo = mid;
-            else h = mid - 1;
-        }
-        x -= binom_sat((uint64_t)lo, j);
-        out[i] = (int32_t)((int64_t)n - 1 - lo);
-        hi = lo - 1;
-    }
-}
 
 void unpack_results(const Cand *top, uint32_t count, int order, uint64_t n, int32_t *out_combos, float *out_mi)
 {

@@ -1133,7 +1133,9 @@ EngineConfig engine_config(const fiu_batch_config *cfg, double default_step_ms)
         if (cfg->struct_size != sizeof(fiu_batch_config)) fail(FIU_ERR_INVALID, "fiu_batch_config.struct_size mismatch");
         if (cfg->max_active) ec.max_active = (int)std::min<uint32_t>(cfg->max_active, 1024);
         if (cfg->step_ms > 0.0) ec.step_ms = cfg->step_ms;
-        if (cfg->candidate_cap) ec.cand_cap = std::max<uint32_t>(cfg->candidate_cap, 1024);
+        // at least twice the largest top list: an overflowing range's
+        // re-scan then pushes fewer entries than the buffer holds
+        if (cfg->candidate_cap) ec.cand_cap = std::max<uint32_t>(cfg->candidate_cap, 2 * kMaxOutputs);
     }
     return ec;
 }

@@ -138,6 +138,8 @@ inline void check_range(const void *p, size_t bytes, const void *base, size_t ca
 }
 
 uint64_t binom_sat(uint64_t n, uint64_t k);
+// Lexicographic unranking of an m-combination of [0, n) (ascending indices).
+void unrank_host(uint64_t r, uint64_t n, int m, int32_t *out);
 
 // Exact number of combinations whose m-prefix has lexicographic rank
 // = offset (mod step).
@@ -182,6 +184,7 @@ struct fiu_dataset {
     bool fp4_ready = false;
     bool pair_ready = false;
     std::vector<hipEvent_t> events;
+    void *readback = nullptr;        // pinned: search state + up to kMaxOutputs results
     std::mutex prep_mu;              // binom / pair table preparation
     // device buffers replaced while work queued earlier may still read them
     // (a binomial table grown for a higher order): freed with the dataset

@@ -33,11 +33,16 @@ def test_large_outputs_vs_oracle(order, snps, outputs, algo):
     assert len(oc) == min(outputs, math.comb(snps, order))
     c, m = fa.GpuSearch(algorithm=algo).run_arrays(d, order, fa.Distribution(snps, order - 1), outputs)
     assert c.tolist() == oc.tolist() and np.array_equal(m, om)
-    # small chunks and a small candidate buffer: overflows and re-scans
-    s = fa.GpuSearch(algorithm=algo, chunk_combos=2000, candidate_cap=1024, sample_threshold=False)
+    # small chunks, no starting threshold
+    s = fa.GpuSearch(algorithm=algo, chunk_combos=2000, sample_threshold=False)
     c, m = s.run_arrays(d, order, fa.Distribution(snps, order - 1), outputs)
     assert c.tolist() == oc.tolist() and np.array_equal(m, om)
-    assert s.last_stats.redo_chunks > 0
+    # one chunk, no threshold, the smallest candidate buffer (raised to
+    # `outputs`): the chunk overflows and is scanned again
+    s = fa.GpuSearch(algorithm=algo, candidate_cap=1, sample_threshold=False)
+    c, m = s.run_arrays(d, order, fa.Distribution(snps, order - 1), outputs)
+    assert c.tolist() == oc.tolist() and np.array_equal(m, om)
+    assert (s.last_stats.redo_chunks > 0) == (math.comb(snps, order) > outputs)
 
 
 def test_large_outputs_limits():
