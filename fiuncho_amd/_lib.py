@@ -28,7 +28,8 @@ ALGO_AUTO, ALGO_DIRECT, ALGO_MARGINAL, ALGO_MFMA = 0, 1, 2, 3
 EXPORTS = (
     "fiu_last_error", "fiu_version", "fiu_set_device",
     "fiu_dataset_create", "fiu_dataset_read", "fiu_dataset_from_genotypes", "fiu_read_files", "fiu_dataset_info", "fiu_dataset_words",
-    "fiu_dataset_destroy", "fiu_config_init", "fiu_search", "fiu_ct_size",
+    "fiu_dataset_destroy", "fiu_config_init", "fiu_search", "fiu_search_static", "fiu_search_static_bytes",
+    "fiu_ct_size",
     "fiu_combine_batch", "fiu_contingency_batch", "fiu_mi_batch",
     "fiu_batch_config_init", "fiu_search_batch", "fiu_server_create", "fiu_server_submit",
     "fiu_server_wait", "fiu_server_query", "fiu_server_destroy", "fiu_set_tuning", "fiu_get_tuning",
@@ -143,6 +144,9 @@ def lib():
     L.fiu_config_init.restype = None
     L.fiu_search.argtypes = [vp, u16, i32, i32, u32, C.POINTER(Config), vp, vp,
                              C.POINTER(u32), C.POINTER(SearchStats)]
+    L.fiu_search_static.argtypes = L.fiu_search.argtypes
+    L.fiu_search_static_bytes.restype = sz
+    L.fiu_search_static_bytes.argtypes = []
     L.fiu_ct_size.restype = sz
     L.fiu_ct_size.argtypes = [u16]
     L.fiu_combine_batch.argtypes = [vp, vp, u16, sz, vp]

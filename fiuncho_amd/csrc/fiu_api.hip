@@ -826,7 +826,7 @@ class DatasetWorkspace final : public Workspace {
     const Cand *listed_ = nullptr;
 };
 
-// Exhaustive search of `d` in workspace `ws` (fiu_search).
+// Exhaustive search of `d` in workspace `ws` (fiu_search, fiu_search_static).
 void search(fiu_dataset *d, Workspace &ws, int order, int32_t step, int32_t offset, uint32_t outputs,
             const fiu_config *cfg, int32_t *out_combos, float *out_mi, uint32_t *out_count, fiu_search_stats *stats)
 {
@@ -1092,6 +1092,121 @@ void search(fiu_dataset *d, Workspace &ws, int order, int32_t step, int32_t offs
     if (stats) *stats = st;
 }
 
+// ---------------------------------------------------------------------------
+// fiu_search_static: the same search in one fixed block of device memory.
+// ---------------------------------------------------------------------------
+// Layout of the block (each region 256-byte aligned).  The limits it sets:
+// 2^18 candidates per chunk, 2^14 launches per search, 2^19 binomial entries
+// ((order + 1) x (snps + 1)).
+constexpr size_t kStaticCand = 1u << 18;
+constexpr size_t kStaticChunks = 1u << 14;
+constexpr size_t kStaticBinom = 1u << 19;
+constexpr size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+constexpr size_t kOffState = 0;
+constexpr size_t kOffTop = kOffState + align256(sizeof(TopState));
+constexpr size_t kOffCand = kOffTop + align256(kMaxOutputs * sizeof(Cand));
+constexpr size_t kOffRedo = kOffCand + align256(kStaticCand * sizeof(Cand));
+constexpr size_t kOffHist = kOffRedo + align256(kStaticChunks * sizeof(uint32_t));
+constexpr size_t kOffCombos = kOffHist + align256(kHistBins * sizeof(uint32_t));
+constexpr size_t kOffMi = kOffCombos + align256((size_t)kMaxOutputs * kMaxOrder * sizeof(int32_t));
+constexpr size_t kOffBinom = kOffMi + align256(kMaxOutputs * sizeof(float));
+constexpr size_t kStaticBytes = kOffBinom + align256(kStaticBinom * sizeof(uint64_t));
+static_assert(kStaticBytes < (9u << 20), "the fixed block stays below 9 MiB");
+
+}  // namespace
+
+// The fixed block (one copy per GPU, held until the process ends).  In host
+// code this name is not a device address: every host use goes through the
+// address hipGetSymbolAddress returns for the current device (static_block).
+static __device__ __align__(256) unsigned char scratch[kStaticBytes];
+
+namespace {
+
+constexpr int kStaticDevices = 64;
+std::mutex g_static_mu;
+void *g_static_ptr[kStaticDevices];   // per device, filled at the first call on it
+
+// The fixed block of `device` (current): its device address, looked up once
+// per device with that device current.
+char *static_block(int device)
+{
+    if (device < 0 || device >= kStaticDevices) fail(FIU_ERR_UNSUPPORTED, "device index above 63");
+    std::lock_guard<std::mutex> lk(g_static_mu);
+    if (g_static_ptr[device]) return (char *)g_static_ptr[device];
+    void *ptr = nullptr;
+    check(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    if (!ptr) fail(FIU_ERR_DEVICE, "no device address for the fixed search block");
+    g_static_ptr[device] = ptr;
+    return (char *)ptr;
+}
+
+class StaticWorkspace final : public Workspace {
+  public:
+    explicit StaticWorkspace(char *base) : base_(base) {}
+    const uint64_t *binom(fiu_dataset *d, int m, hipStream_t s) override
+    {
+        const uint64_t entries = (uint64_t)(m + 1) * ((uint64_t)d->snps + 1);
+        if (entries > kStaticBinom)
+            fail(FIU_ERR_UNSUPPORTED, "fiu_search_static: (order + 1) x (snps + 1) = " + std::to_string(entries) +
+                                          " binomial entries, above its limit of " + std::to_string(kStaticBinom));
+        uint64_t *b = (uint64_t *)(base_ + kOffBinom);
+        check(launch_binom_table(b, d->snps, m, s), "binomial table launch");
+        return b;
+    }
+    // derived tables an earlier fiu_search built; none is built here
+    bool pair_table(fiu_dataset *d, hipStream_t) override
+    {
+        std::lock_guard<std::mutex> lk(d->prep_mu);
+        return d->pair_ready;
+    }
+    bool fp4_image(fiu_dataset *d, hipStream_t) override
+    {
+        std::lock_guard<std::mutex> lk(d->prep_mu);
+        return d->fp4_ready;
+    }
+    uint32_t max_candidates() const override { return (uint32_t)kStaticCand; }
+    uint64_t max_chunks() const override { return kStaticChunks; }
+    void buffers(fiu_dataset *, uint32_t cap, uint64_t chunks, uint32_t outputs, Cand *&cand, Cand *&top,
+                 TopState *&state, uint32_t *&redo, uint32_t *&hist) override
+    {
+        if (cap > kStaticCand || chunks > kStaticChunks) fail(FIU_ERR_UNSUPPORTED, "fiu_search_static: limits exceeded");
+        if (outputs > (uint32_t)kMaxOutputs)
+            fail(FIU_ERR_UNSUPPORTED, "fiu_search_static: outputs above " + std::to_string(kMaxOutputs));
+        cand = (Cand *)(base_ + kOffCand);
+        top = (Cand *)(base_ + kOffTop);
+        state = (TopState *)(base_ + kOffState);
+        redo = (uint32_t *)(base_ + kOffRedo);
+        hist = (uint32_t *)(base_ + kOffHist);
+    }
+    void zero(uint32_t *p, size_t n, hipStream_t s) override { check(launch_fill_u32(p, n, 0u, s), "clear"); }
+    uint32_t *big_cells(fiu_dataset *, int order) override
+    {
+        fail(FIU_ERR_UNSUPPORTED, "fiu_search_static: orders above 9 need " + std::to_string(big_cells_bytes(order)) +
+                                      " bytes of counters, beyond its fixed block");
+    }
+    Cand *merge_work(fiu_dataset *, size_t) override
+    {
+        fail(FIU_ERR_UNSUPPORTED, "fiu_search_static: outputs above " + std::to_string(kMaxOutputs));
+    }
+    hipEvent_t event(fiu_dataset *, size_t) override { return nullptr; }
+    void results(const Cand *top, uint32_t count, int order, int n, const uint64_t *binom, int32_t *out_combos,
+                 float *out_mi, hipStream_t s) override
+    {
+        int32_t *combos = (int32_t *)(base_ + kOffCombos);
+        float *mi = (float *)(base_ + kOffMi);
+        check(launch_unpack_top(top, count, order, n, binom, combos, mi, s), "unpack launch");
+        if (count) {
+            check(hipMemcpyAsync(out_combos, combos, (size_t)count * order * sizeof(int32_t), hipMemcpyDeviceToHost, s),
+                  "results read");
+            check(hipMemcpyAsync(out_mi, mi, (size_t)count * sizeof(float), hipMemcpyDeviceToHost, s), "results read");
+        }
+        check(hipStreamSynchronize(s), "results sync");
+    }
+
+  private:
+    char *base_;
+};
+
 }  // namespace
 
 namespace fiu {
@@ -1260,6 +1375,22 @@ fiu_status fiu_search(const fiu_dataset *ds, uint16_t order, int32_t dist_step, 
                out_count, stats);
     });
 }
+
+fiu_status fiu_search_static(const fiu_dataset *ds, uint16_t order, int32_t dist_step, int32_t dist_offset,
+                             uint32_t outputs, const fiu_config *cfg, int32_t *out_combos, float *out_mi,
+                             uint32_t *out_count, fiu_search_stats *stats)
+{
+    return guard([&] {
+        if (!ds) fail(FIU_ERR_INVALID, "dataset is NULL");
+        if (cfg && cfg->struct_size != sizeof(fiu_config)) fail(FIU_ERR_INVALID, "fiu_config.struct_size mismatch");
+        check(hipSetDevice(ds->device), "hipSetDevice");
+        StaticWorkspace ws(static_block(ds->device));
+        search(const_cast<fiu_dataset *>(ds), ws, order, dist_step, dist_offset, outputs, cfg, out_combos, out_mi,
+               out_count, stats);
+    });
+}
+
+size_t fiu_search_static_bytes(void) { return kStaticBytes; }
 
 size_t fiu_ct_size(uint16_t order)
 {

@@ -73,4 +73,12 @@ hipError_t launch_batch(const BatchJob *jobs, const BatchSeg *segs0, const Batch
                         const BatchSeg *segs3, const int32_t *merge_list, BatchHeader *hdr, int blocks0, int blocks1,
                         int blocks2, int blocks3, int merge_blocks, hipStream_t s);
 
+// Searches in a fixed device block: the binomial table C(x, j), x <= n,
+// j <= m (saturated), a fill of n 32-bit words, and the top list unpacked to
+// ascending SNP indices (binom up to row `order`) and float MI.
+hipError_t launch_binom_table(uint64_t *binom, uint64_t n, int m, hipStream_t s);
+hipError_t launch_fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t s);
+hipError_t launch_unpack_top(const Cand *top, uint32_t count, int order, int n, const uint64_t *binom,
+                             int32_t *combos, float *mi, hipStream_t s);
+
 }  // namespace fiu

@@ -2042,6 +2042,53 @@ k_mfma3(const DevData D, const ScanParams P, TopState *st, const Fp4Image F, int
 }
 
 // ---------------------------------------------------------------------------
+// Small helpers of the searches that run in a fixed device block (no
+// allocation, no memset): the binomial table, fills, result unpacking.
+// ---------------------------------------------------------------------------
+// binom[j * (n + 1) + x] = C(x, j), saturated at UINT64_MAX, for j <= m.
+// C(x - j + i, i) = C(x - j + i - 1, i - 1) (x - j + i) / i is exact at every
+// step; with r = q i + rem the product splits as q v + (rem v) / i, where
+// rem v is a multiple of i and below 2^35, so no 128-bit division is needed.
+// The values grow with i (x >= j), so a saturated step stays saturated.
+__global__ void k_binom_table(uint64_t *binom, uint64_t n, int m)
+{
+    const uint64_t id = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x, stride = n + 1;
+    if (id >= (uint64_t)(m + 1) * stride) return;
+    const int j = (int)(id / stride);
+    const uint64_t x = id % stride;
+    uint64_t r = 0;
+    if ((uint64_t)j <= x) {
+        r = 1;
+        for (int i = 1; i <= j && r != UINT64_MAX; ++i) {
+            const uint64_t v = x - (uint64_t)j + (uint64_t)i, q = r / (uint64_t)i, rem = r % (uint64_t)i;
+            const uint64_t lo = q * v, part = rem * v / (uint64_t)i;
+            r = (__umul64hi(q, v) != 0 || lo > UINT64_MAX - part) ? UINT64_MAX : lo + part;
+        }
+    }
+    binom[id] = r;
+}
+
+__global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v)
+{
+    const uint64_t id = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id < n) p[id] = v;
+}
+
+// Top-N entries -> ascending SNP indices (order per entry, the key's
+// combination by lexicographic unranking) and float MI, as the host's
+// unpack_results does.
+__global__ void k_unpack_top(const Cand *top, uint32_t count, int order, int n, const uint64_t *binom, int32_t *combos,
+                             float *mi)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    int c[kMaxOrder];
+    unrank(top[i].combo, n, order, binom, c);
+    for (int q = 0; q < order; ++q) combos[(size_t)i * order + q] = c[q];
+    mi[i] = top[i].mi;
+}
+
+// ---------------------------------------------------------------------------
 // Top-N merge (one workgroup): current list + candidates -> best N, unique.
 // ---------------------------------------------------------------------------
 // Sorts buf[0, L) (L a power of two, padded with empty entries) by key,
@@ -2951,6 +2998,30 @@ hipError_t launch_merge_big(TopState *st, Cand *top, uint32_t n_top, const Cand 
         for (uint32_t jj = k >> 1; jj > 0; jj >>= 1)
             hipLaunchKernelGGL(k_big_bitonic, dim3(blocks), dim3(256), 0, s, work, L, k, jj);
     hipLaunchKernelGGL(k_big_finish, dim3(1), dim3(1024), 0, s, work, L, top, N, st, chunk_id, redo);
+    return hipGetLastError();
+}
+
+hipError_t launch_binom_table(uint64_t *binom, uint64_t n, int m, hipStream_t s)
+{
+    const uint64_t total = (uint64_t)(m + 1) * (n + 1);
+    hipLaunchKernelGGL(k_binom_table, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, binom, n, m);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t s)
+{
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_unpack_top(const Cand *top, uint32_t count, int order, int n, const uint64_t *binom,
+                             int32_t *combos, float *mi, hipStream_t s)
+{
+    if (!count) return hipSuccess;
+    if (order > kMaxOrder) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_unpack_top, dim3((count + 255) / 256), dim3(256), 0, s, top, count, order, n, binom, combos,
+                       mi);
     return hipGetLastError();
 }
 

@@ -185,6 +185,31 @@ fiu_status fiu_search(const fiu_dataset *ds, uint16_t order,
                       int32_t *out_combos, float *out_mi, uint32_t *out_count,
                       fiu_search_stats *stats);
 
+/* Search that allocates nothing.
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory -- code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc.  Same arguments, results, errors and algorithm choice as
+ * fiu_search; the search runs in one fixed block of device memory per GPU
+ * (fiu_search_static_bytes(), under 9 MiB), which the library holds until
+ * the process ends, and calls no hipMalloc / hipFree (and no host heap
+ * allocation).  Limits, refused with FIU_ERR_UNSUPPORTED: candidate_cap <=
+ * 2^18 (default 2^18); (order + 1) * (snps + 1) <= 2^19 (binomial table),
+ * e.g. 131,071 SNPs at order 3, 52,427 at order 9; orders up to 9; outputs
+ * up to 4,096; 2^14 scan launches (chunks grow to fit).  The dataset's pair
+ * table and fp4 image are used when an earlier fiu_search built them;
+ * otherwise the kernels that need neither run
+ * (same results).  stats->scan_ms is 0 (no timing events).  Every call on one
+ * GPU works in that GPU's block, which keeps the last call's contents (each
+ * call clears what must start at zero): calls that overlap on one GPU race,
+ * and must be serialised by the caller. */
+fiu_status fiu_search_static(const fiu_dataset *ds, uint16_t order,
+                             int32_t dist_step, int32_t dist_offset,
+                             uint32_t outputs, const fiu_config *cfg,
+                             int32_t *out_combos, float *out_mi,
+                             uint32_t *out_count, fiu_search_stats *stats);
+/* Bytes of the fixed block fiu_search_static uses on each GPU. */
+size_t fiu_search_static_bytes(void);
+
 /* ------------------------------------------------------------------------ */
 /* Batch operators (one GPU launch for many tables)                          */
 /* ------------------------------------------------------------------------ */
