@@ -369,21 +369,23 @@ def percentile(xs, q):
     return float(np.percentile(np.asarray(xs, float), q)) if xs else None
 
 
-def bench_serve(count=300, load=0.7):
-    """Scan-request stream: `count` requests as above at serving scale (seed
-    1000 + request id), each carrying host tables.  Capacity of each form
-    with every request queued at once, then a Poisson stream at `load` x the
-    served capacity through both forms (plain: one worker, one request at a
-    time, FIFO).  Latency = completion - arrival."""
+def bench_serve(count=3000, distinct=300, load=0.7):
+    """Scan-request stream: `count` requests over `distinct` tables at
+    serving scale (table i: seed 1000 + i; request r uses table r mod
+    `distinct`), each request carrying its host tables.  Capacity of each
+    form with every request queued at once, then a Poisson stream at `load`
+    x the served capacity through both forms (plain: one worker, one request
+    at a time, FIFO).  Latency = completion - arrival."""
     import fiuncho_amd as fa
-    specs = uneven_specs(count, 1000, SERVE_SCALE)
-    reqs = []
+    specs = uneven_specs(distinct, 1000, SERVE_SCALE)
+    tables = []
     for sp in specs:
         words, cw, tw = make_words(sp)
-        reqs.append(dict(order=sp["order"], outputs=BATCH_OUTPUTS, words=words, snps=sp["snps"],
-                         cases_words=cw, ctrls_words=tw, cases=sp["cases"], ctrls=sp["ctrls"]))
-    work = sum(combo_samples(sp) for sp in specs)
-    print(f"[bench] serve: {count} requests generated", file=sys.stderr, flush=True)
+        tables.append(dict(order=sp["order"], outputs=BATCH_OUTPUTS, words=words, snps=sp["snps"],
+                           cases_words=cw, ctrls_words=tw, cases=sp["cases"], ctrls=sp["ctrls"]))
+    stream = [i % distinct for i in range(count)]
+    work = sum(combo_samples(specs[i]) for i in stream)
+    print(f"[bench] serve: {distinct} tables generated, {count} requests", file=sys.stderr, flush=True)
 
     def plain_one(r):
         d = fa.Dataset.from_words(r["words"], r["snps"], r["cases_words"], r["ctrls_words"], r["cases"], r["ctrls"])
@@ -391,16 +393,25 @@ def bench_serve(count=300, load=0.7):
         d.close()
         return out
 
+    def same_as_plain(res):
+        return all(a[0].tolist() == plain_res[i][0].tolist() and np.array_equal(a[1], plain_res[i][1])
+                   for i, a in zip(stream, res))
+
     # capacity, every request queued at t = 0
     t0 = time.perf_counter()
-    plain_res = [plain_one(r) for r in reqs]
+    plain_res = {}
+    for i in stream:
+        out = plain_one(tables[i])
+        plain_res.setdefault(i, out)
     t_plain = time.perf_counter() - t0
+    print(f"[bench] serve: plain capacity run {t_plain:.2f} s", file=sys.stderr, flush=True)
     srv = fa.Server()
     t0 = time.perf_counter()
-    ids = [srv.submit(**r) for r in reqs]
-    served_res = [srv.wait(i) for i in ids]
+    ids = [srv.submit(**tables[i]) for i in stream]
+    served_res = [srv.wait(x) for x in ids]
     t_served = time.perf_counter() - t0
-    same = all(a[0].tolist() == b[0].tolist() and np.array_equal(a[1], b[1]) for a, b in zip(plain_res, served_res))
+    same = same_as_plain(served_res)
+    del served_res
     print(f"[bench] serve: capacity plain {count / t_plain:.1f} req/s, served {count / t_served:.1f} req/s",
           file=sys.stderr, flush=True)
     # Poisson arrivals at `load` x the served capacity
@@ -410,35 +421,37 @@ def bench_serve(count=300, load=0.7):
 
     t0 = time.perf_counter()
     sub = []
-    for r, a in zip(reqs, arrivals):
+    for i, a in zip(stream, arrivals):
         dt = a - (time.perf_counter() - t0)
         if dt > 0:
             time.sleep(dt)
-        sub.append(srv.submit(**r))
-    stream_res = [srv.wait(i) for i in sub]
+        sub.append(srv.submit(**tables[i]))
+    stream_res = [srv.wait(x) for x in sub]
     lat_served = [x[2].latency_ms for x in stream_res]
     t_stream_served = time.perf_counter() - t0
-    same = same and all(a[0].tolist() == b[0].tolist() and np.array_equal(a[1], b[1])
-                        for a, b in zip(plain_res, stream_res))
+    same = same and same_as_plain(stream_res)
+    del stream_res
     srv.close()
     print(f"[bench] serve: served stream p50 {percentile(lat_served, 50):.1f} ms, "
           f"p99 {percentile(lat_served, 99):.1f} ms", file=sys.stderr, flush=True)
 
     t0 = time.perf_counter()
     lat_plain = []
-    for r, a in zip(reqs, arrivals):
+    for i, a in zip(stream, arrivals):
         dt = a - (time.perf_counter() - t0)
         if dt > 0:
             time.sleep(dt)
-        plain_one(r)
+        plain_one(tables[i])
         lat_plain.append((time.perf_counter() - t0 - a) * 1e3)
     t_stream_plain = time.perf_counter() - t0
     print(f"[bench] serve: plain stream p50 {percentile(lat_plain, 50):.1f} ms, "
           f"p99 {percentile(lat_plain, 99):.1f} ms", file=sys.stderr, flush=True)
-    orders = [sp["order"] for sp in specs]
-    return {"workload": f"{count} requests (seeds 1000..{999 + count}): order 2 with 500..4,000 SNPs, order 3 "
-                        "with 100..600 SNPs, order 4 (every 16th) with 30..80 SNPs; 1,000..20,000 samples "
-                        "(order 4: ..5,000), case fraction 0.3..0.7, top-50; host tables uploaded per request",
+    orders = [specs[i]["order"] for i in stream]
+    return {"workload": f"{count} requests over {distinct} tables (table i: seed 1000 + i, request r uses table "
+                        f"r mod {distinct}): order 2 with 500..4,000 SNPs, order 3 with 100..600 SNPs, order 4 "
+                        "(every 16th table) with 30..80 SNPs; 1,000..20,000 samples (order 4: ..5,000), case "
+                        "fraction 0.3..0.7, top-50; host tables uploaded per request",
+            "requests": count, "distinct_tables": distinct,
             "orders": {str(k): orders.count(k) for k in (2, 3, 4)},
             "combo_samples": work,
             "capacity_req_per_s": {"plain": count / t_plain, "served": count / t_served},
@@ -523,7 +536,8 @@ def main():
     ap.add_argument("--no-ingest", action="store_true", help="skip the .raw read / device packing timing")
     ap.add_argument("--no-high", action="store_true", help="skip the order-4 to order-6 scans")
     ap.add_argument("--batch-datasets", type=int, default=64)
-    ap.add_argument("--serve-requests", type=int, default=300)
+    ap.add_argument("--serve-requests", type=int, default=3000)
+    ap.add_argument("--serve-tables", type=int, default=300)
     ap.add_argument("--only-extra", action="store_true",
                     help="run only the batch / serve comparisons (no headline scan)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
@@ -564,7 +578,7 @@ def main():
         if not args.no_batch:
             extra["batched"] = bench_batch(args.batch_datasets)
         if not args.no_serve:
-            extra["serving"] = bench_serve(args.serve_requests)
+            extra["serving"] = bench_serve(args.serve_requests, min(args.serve_tables, args.serve_requests))
         if not args.no_ingest:
             spec = fa.synth_spec(N_SNPS, N_CASES, N_CTRLS, seed=1, planted=PLANTED, penetrance_seed=2)
             words, geno, _, _ = fa.synth_generate(spec, words=True, geno=True)
@@ -663,7 +677,7 @@ def main():
     if not args.no_batch:
         line["batched"] = bench_batch(args.batch_datasets)
     if not args.no_serve:
-        line["serving"] = bench_serve(args.serve_requests)
+        line["serving"] = bench_serve(args.serve_requests, min(args.serve_tables, args.serve_requests))
     if not (args.no_cpu and args.no_ingest):
         with tempfile.TemporaryDirectory(dir="/dev/shm" if os.path.isdir("/dev/shm") else None) as tmp:
             path = os.path.join(tmp, "bench.raw")

@@ -726,7 +726,6 @@ class Workspace {
                          TopState *&state, uint32_t *&redo, uint32_t *&hist) = 0;
     // outputs above kMaxOutputs: the merge's sort buffer of `entries`
     virtual Cand *merge_work(fiu_dataset *d, size_t entries) = 0;
-    virtual void zero(uint32_t *p, size_t n, hipStream_t s) = 0;
     // orders 10-12: the count slots of the persistent all-cells kernel
     virtual uint32_t *big_cells(fiu_dataset *d, int order) = 0;
     // timing event i, or null (the search is not timed)
@@ -763,10 +762,6 @@ class DatasetWorkspace final : public Workspace {
         d->redo.alloc(std::max<uint64_t>(chunks, 1));
         d->hist.alloc(kHistBins);
         cand = d->cand.p, top = d->top.p, state = d->state.p, redo = d->redo.p, hist = d->hist.p;
-    }
-    void zero(uint32_t *p, size_t n, hipStream_t s) override
-    {
-        check(hipMemsetAsync(p, 0, n * sizeof(uint32_t), s), "clear");
     }
     uint32_t *big_cells(fiu_dataset *d, int order) override
     {
@@ -954,9 +949,15 @@ void search(fiu_dataset *d, Workspace &ws, int order, int32_t step, int32_t offs
     // combination it samples with a popcount kernel), 1/256 ahead of a
     // matrix-core scan, which costs 5-12x less per combination
     const double sample_frac = algo == FIU_ALGO_MFMA ? 1.0 / 256.0 : 1.0 / 16.0;
-    // (orders 7-8 cost about 100x more per combination: a smaller floor)
-    const uint64_t sample_cap = std::max<uint64_t>(order >= kMinAnyOrder ? 4096 : 65536,
-                                                   (uint64_t)((double)n_pref * per_avg * sample_frac));
+    // The floor: the scan pushes about outputs x combinations / sample
+    // candidates above the starting threshold (twice that with the bins'
+    // width), kept near the size of one chunk's candidate buffer; 16,384
+    // combinations at least (on 1,000 SNPs x 4,000 samples, top-10: 4,096
+    // lets 3x more candidates through the scan and the merge, 65,536 makes
+    // the sampling pass cost as much as the scan).
+    const double combos_est = (double)n_pref * per_avg;
+    const uint64_t sample_floor = (uint64_t)std::max(16384.0, 2.0 * (double)outputs * combos_est / (double)cap);
+    const uint64_t sample_cap = std::max<uint64_t>(sample_floor, (uint64_t)(combos_est * sample_frac));
     const int64_t sample_prefixes =
         plan.count == 0 || !sample
             ? 0
@@ -974,10 +975,8 @@ void search(fiu_dataset *d, Workspace &ws, int order, int32_t step, int32_t offs
         return p;
     };
     Cand *mwork = big_merge ? ws.merge_work(d, pow2_at_least((uint64_t)outputs + cap)) : nullptr;
-    TopState init{};
-    init.tau = -INFINITY;
-    init.tau_combo = ~0ull;
-    check(hipMemcpyAsync(state, &init, sizeof(init), hipMemcpyHostToDevice, s), "state init");
+    // state reset and (when sampling) histogram cleared in one launch
+    check(launch_search_begin(state, sample_prefixes > 0 ? hist : nullptr, s), "search start launch");
 
     ScanParams P{};
     P.step = step;
@@ -1035,7 +1034,6 @@ void search(fiu_dataset *d, Workspace &ws, int order, int32_t step, int32_t offs
         ++st.launches;
     };
     if (sample_prefixes > 0) {
-        ws.zero(hist, kHistBins, s);
         ScanParams H = P;
         H.j0 = (int64_t)n_pref - sample_prefixes;
         H.j1 = (int64_t)n_pref;
@@ -1178,7 +1176,6 @@ class StaticWorkspace final : public Workspace {
         redo = (uint32_t *)(base_ + kOffRedo);
         hist = (uint32_t *)(base_ + kOffHist);
     }
-    void zero(uint32_t *p, size_t n, hipStream_t s) override { check(launch_fill_u32(p, n, 0u, s), "clear"); }
     uint32_t *big_cells(fiu_dataset *, int order) override
     {
         fail(FIU_ERR_UNSUPPORTED, "fiu_search_static: orders above 9 need " + std::to_string(big_cells_bytes(order)) +

@@ -16,6 +16,8 @@ hipError_t launch_scan_direct(int order, const DevData &D, const ScanParams &P, 
 hipError_t launch_scan_marginal(int order, const DevData &D, const ScanParams &P, TopState *st, hipStream_t s,
                                 int pw = kMarginalPW);
 hipError_t launch_tau_from_hist(const uint32_t *hist, uint32_t N, float delta, TopState *st, hipStream_t s);
+// A search's start: state reset, sampling histogram (when set) cleared.
+hipError_t launch_search_begin(TopState *st, uint32_t *hist, hipStream_t s);
 hipError_t launch_pair_table(const DevData &D, const ScanParams &P, uint32_t *pair_out, hipStream_t s);
 // Order 2 on the matrix cores: the pair triangle of T x T 64-SNP tiles is cut
 // into super-blocks of 8 x 8 tiles; a launch scores super-blocks [s0, s1)
@@ -74,10 +76,9 @@ hipError_t launch_batch(const BatchJob *jobs, const BatchSeg *segs0, const Batch
                         int blocks2, int blocks3, int merge_blocks, hipStream_t s);
 
 // Searches in a fixed device block: the binomial table C(x, j), x <= n,
-// j <= m (saturated), a fill of n 32-bit words, and the top list unpacked to
-// ascending SNP indices (binom up to row `order`) and float MI.
+// j <= m (saturated), and the top list unpacked to ascending SNP indices
+// (binom up to row `order`) and float MI.
 hipError_t launch_binom_table(uint64_t *binom, uint64_t n, int m, hipStream_t s);
-hipError_t launch_fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t s);
 hipError_t launch_unpack_top(const Cand *top, uint32_t count, int order, int n, const uint64_t *binom,
                              int32_t *combos, float *mi, hipStream_t s);
 

@@ -377,18 +377,44 @@ __device__ __forceinline__ void hist_end(uint32_t *lds_hist, uint32_t *hist)
 // holding the N-th best sampled screening value, less the screening margin.
 // The sampled combinations are scanned again afterwards, and at least N of
 // them have an exact value above it, so the final N-th best is never below.
-__global__ void k_tau_from_hist(const uint32_t *hist, uint32_t N, float delta, TopState *st)
+// One workgroup of kTauThreads: thread t sums bins [kTauPer t, kTauPer (t +
+// 1)), a suffix scan over the threads gives the count at or above each
+// thread's range, and the thread whose range holds the crossing walks its
+// own bins (the histogram is read once, in parallel).
+constexpr int kTauThreads = 256, kTauPer = kHistBins / kTauThreads;
+static_assert(kHistBins % kTauThreads == 0, "histogram split");
+__global__ void __launch_bounds__(kTauThreads) k_tau_from_hist(const uint32_t *hist, uint32_t N, float delta,
+                                                                TopState *st)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    uint64_t cum = 0;
-    for (int b = kHistBins - 1; b >= 0; --b) {
-        cum += hist[b];
-        if (cum >= N && N > 0) {
+    __shared__ uint64_t part[kTauThreads];
+    const int t = threadIdx.x;
+    uint32_t h[kTauPer];
+    uint64_t mine = 0;
+#pragma unroll
+    for (int i = 0; i < kTauPer; ++i) {
+        h[i] = hist[t * kTauPer + i];
+        mine += h[i];
+    }
+    part[t] = mine;
+    __syncthreads();
+    // inclusive suffix sums: part[t] = sum over threads t' >= t
+    for (int d = 1; d < kTauThreads; d <<= 1) {
+        const uint64_t add = t + d < kTauThreads ? part[t + d] : 0;
+        __syncthreads();
+        part[t] += add;
+        __syncthreads();
+    }
+    const uint64_t above = t + 1 < kTauThreads ? part[t + 1] : 0;   // bins past this thread's range
+    if (N == 0 || above >= N || above + mine < N) return;
+    uint64_t cum = above;
+    for (int i = kTauPer - 1; i >= 0; --i) {
+        cum += h[i];
+        if (cum >= N) {
             union {
                 uint32_t u;
                 float f;
             } x;
-            x.u = (uint32_t)b << 20;
+            x.u = (uint32_t)(t * kTauPer + i) << 20;
             st->tau = x.f - delta;
             st->tau_combo = ~0ull;
             return;
@@ -2042,8 +2068,9 @@ k_mfma3(const DevData D, const ScanParams P, TopState *st, const Fp4Image F, int
 }
 
 // ---------------------------------------------------------------------------
-// Small helpers of the searches that run in a fixed device block (no
-// allocation, no memset): the binomial table, fills, result unpacking.
+// Small helpers of the searches (the ones in a fixed device block use no
+// allocation and no memset): the binomial table, the search start, result
+// unpacking.
 // ---------------------------------------------------------------------------
 // binom[j * (n + 1) + x] = C(x, j), saturated at UINT64_MAX, for j <= m.
 // C(x - j + i, i) = C(x - j + i - 1, i - 1) (x - j + i) / i is exact at every
@@ -2068,10 +2095,19 @@ __global__ void k_binom_table(uint64_t *binom, uint64_t n, int m)
     binom[id] = r;
 }
 
-__global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v)
+// A search's start: the top-N state reset (empty list, threshold -inf)
+// and, when `hist` is set, the sampling histogram cleared (a kernel rather
+// than copies and fills, so the start is one launch).
+__global__ void k_search_begin(TopState *st, uint32_t *hist)
 {
-    const uint64_t id = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (id < n) p[id] = v;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        TopState init{};
+        init.tau = -INFINITY;
+        init.tau_combo = ~0ull;
+        *st = init;
+    }
+    if (hist)
+        for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < kHistBins; b += gridDim.x * blockDim.x) hist[b] = 0;
 }
 
 // Top-N entries -> ascending SNP indices (order per entry, the key's
@@ -2840,7 +2876,7 @@ hipError_t launch_batch(const BatchJob *jobs, const BatchSeg *segs0, const Batch
 
 hipError_t launch_tau_from_hist(const uint32_t *hist, uint32_t N, float delta, TopState *st, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_tau_from_hist, dim3(1), dim3(64), 0, s, hist, N, delta, st);
+    hipLaunchKernelGGL(k_tau_from_hist, dim3(1), dim3(kTauThreads), 0, s, hist, N, delta, st);
     return hipGetLastError();
 }
 
@@ -3008,10 +3044,9 @@ hipError_t launch_binom_table(uint64_t *binom, uint64_t n, int m, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t s)
+hipError_t launch_search_begin(TopState *st, uint32_t *hist, hipStream_t s)
 {
-    if (!n) return hipSuccess;
-    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
+    hipLaunchKernelGGL(k_search_begin, dim3(hist ? 4 : 1), dim3(256), 0, s, st, hist);
     return hipGetLastError();
 }
 
