@@ -1861,16 +1861,6 @@ __device__ __forceinline__ void mfma3_tile(const DevData &D, const ScanParams &P
     const bool a_lane = (lane >> 1) == w;                                       // lanes 2w, 2w + 1 (< 8)
     const uint32_t ua = 16u * (uint32_t)(4 * ((lr & 1) * np + a) + ls);        // group 12, row x = lr
     v4f acc[2][2][2][2][2];   // [class][x][y][b block][z]
-#pragma unroll
-    for (int c = 0; c < 2; ++c)
-#pragma unroll
-        for (int x = 0; x < 2; ++x)
-#pragma unroll
-            for (int y = 0; y < 2; ++y)
-#pragma unroll
-                for (int bb = 0; bb < 2; ++bb)
-#pragma unroll
-                    for (int z = 0; z < 2; ++z) acc[c][x][y][bb][z] = (v4f){0.f, 0.f, 0.f, 0.f};
     // fragment reads: lane reads row rr of a group at segment kc (32 samples
     // 32 kc ..), stored at segment kc ^ m3_swz(rr); the a rows (m3_swz = 0)
     // are read by every lane of a 16-lane group at once (broadcast)
@@ -1878,6 +1868,14 @@ __device__ __forceinline__ void mfma3_tile(const DevData &D, const ScanParams &P
     const int fo = rr * 64 + ((kc ^ m3_swz(rr)) << 4), ao = kc << 4;
 #pragma unroll
     for (int cls = 0; cls < 2; ++cls) {
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+            for (int y = 0; y < 2; ++y)
+#pragma unroll
+                for (int bb = 0; bb < 2; ++bb)
+#pragma unroll
+                    for (int z = 0; z < 2; ++z) acc[cls][x][y][bb][z] = (v4f){0.f, 0.f, 0.f, 0.f};
         const int iters = D.quads[cls];
         const uint4 *img = F.base + F.class_off[cls];
         const int64_t qstride = 8 * (int64_t)np;   // uint4 per quad: 2 genotypes x n_pad x 4
@@ -1965,6 +1963,21 @@ __device__ __forceinline__ void mfma3_tile(const DevData &D, const ScanParams &P
             const bool more = steady || it + R - 1 < iters;
             if (more) issue_bc(it + R - 1);
             multiply(f);
+            if (steady) {
+                // issue order: the 8 fragment reads, the first mask, then each
+                // remaining mask word right behind a product (it issues while
+                // the product occupies the matrix core), the last 4 products
+                // (3 % faster than the compiler's order, which put all 16
+                // masks ahead of the products)
+                __builtin_amdgcn_sched_group_barrier(0x100, 8, 0);
+                __builtin_amdgcn_sched_group_barrier(0x002, 4, 0);
+#pragma unroll
+                for (int i = 0; i < 12; ++i) {
+                    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+                    __builtin_amdgcn_sched_group_barrier(0x002, 1, 0);
+                }
+                __builtin_amdgcn_sched_group_barrier(0x008, 4, 0);
+            }
             __builtin_amdgcn_s_setprio(0);
             if (more) issue_a(it + R - 1);
         };
@@ -2066,6 +2079,7 @@ k_mfma3(const DevData D, const ScanParams P, TopState *st, const Fp4Image F, int
     const int64_t L = blockIdx.x, s = L >> 3;
     mfma3_tile<R>(D, P, st, F, a0 + (int)(s % na), (s / na) * 8 + (L & 7), T, lds);
 }
+
 
 // ---------------------------------------------------------------------------
 // Small helpers of the searches (the ones in a fixed device block use no
@@ -2751,9 +2765,33 @@ __global__ void k_mi_batch(const uint32_t *tables, uint64_t n, uint64_t ct_size,
 // ---------------------------------------------------------------------------
 // Host launchers.
 // ---------------------------------------------------------------------------
-template <bool HIST>
-static hipError_t scan_direct(int order, const DevData &D, const ScanParams &P, TopState *st, hipStream_t s)
+// A dispatch holds at most 2^32 - 1 work-items per grid dimension: launches
+// over prefix ranges are split into ranges whose grids stay below that.
+constexpr int64_t kMaxGridBlocks = (int64_t)(0xffffffffu / (uint32_t)kBlock);
+
+// f(Q) for consecutive sub-ranges [Q.j0, Q.j1) of [P.j0, P.j1) of at most
+// kMaxGridBlocks x per_block prefixes each.
+template <class F>
+static hipError_t split_prefixes(const ScanParams &P, int64_t per_block, F &&f)
 {
+    const int64_t span = kMaxGridBlocks * per_block;
+    for (int64_t j = P.j0; j < P.j1; j += span) {
+        ScanParams Q = P;
+        Q.j0 = j;
+        Q.j1 = std::min<int64_t>(P.j1, j + span);
+        const hipError_t e = f(Q);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+template <bool HIST>
+static hipError_t scan_direct(int order, const DevData &D, const ScanParams &P0, TopState *st, hipStream_t s)
+{
+    if (P0.j1 - P0.j0 > (order >= 7 && order <= 9 ? 1 : 4) * kMaxGridBlocks && (order < 10 || order > 12))
+        return split_prefixes(P0, order >= 7 ? 1 : 4,
+                              [&](const ScanParams &Q) { return scan_direct<HIST>(order, D, Q, st, s); });
+    const ScanParams &P = P0;
     const int64_t waves = P.j1 - P.j0;
     if (waves <= 0) return hipSuccess;
     const dim3 grid((unsigned)((waves + 3) / 4)), block(kBlock);
@@ -2785,6 +2823,9 @@ hipError_t launch_scan_km(int order, const DevData &D, const ScanParams &P, TopS
     const int64_t prefixes = P.j1 - P.j0;
     if (prefixes <= 0) return hipSuccess;
     const int64_t per_block = kBlock / kWave;
+    if (prefixes > per_block * kMaxGridBlocks)
+        return split_prefixes(P, per_block,
+                              [&](const ScanParams &Q) { return launch_scan_km(order, D, Q, st, s, all_cells); });
     const dim3 grid((unsigned)((prefixes + per_block - 1) / per_block)), block(kBlock);
     const bool hist = P.hist != nullptr;
 #define FIU_KM(K, H, A) hipLaunchKernelGGL((k_scan_km<K, H, A>), grid, block, 0, s, D, P, st)
@@ -2820,6 +2861,9 @@ static hipError_t scan_marginal(int order, const DevData &D, const ScanParams &P
     const int64_t prefixes = P.j1 - P.j0;
     if (prefixes <= 0) return hipSuccess;
     const int64_t per_block = (kBlock / kWave) * PW;
+    if (prefixes > per_block * kMaxGridBlocks)
+        return split_prefixes(P, per_block,
+                              [&](const ScanParams &Q) { return scan_marginal<PW, HIST>(order, D, Q, st, s); });
     const dim3 grid((unsigned)((prefixes + per_block - 1) / per_block)), block(kBlock);
     switch (order) {
     case 2: hipLaunchKernelGGL((k_scan2m<PW, false, HIST>), grid, block, 0, s, D, P, st, (uint32_t *)nullptr); break;
@@ -2890,6 +2934,15 @@ hipError_t launch_mfma2(const DevData &D, const ScanParams &P, TopState *st, int
                         const uint4 *fp4, hipStream_t s)
 {
     if (s1 <= s0) return hipSuccess;
+    // a grid below 2^32 work-items: sub-launches of at most this many super-blocks
+    constexpr int64_t kMaxSuper = kMaxGridBlocks / (8 * 64) * 8;
+    if (s1 - s0 > kMaxSuper) {
+        for (int64_t a = s0; a < s1; a += kMaxSuper) {
+            const hipError_t e = launch_mfma2(D, P, st, a, std::min(s1, a + kMaxSuper), T, fp4, s);
+            if (e != hipSuccess) return e;
+        }
+        return hipSuccess;
+    }
     // 64 blocks per super-block, 8 super-blocks (one per XCD) per round
     const int64_t rounds = (s1 - s0 + 7) / 8;
     const dim3 grid((unsigned)(rounds * 8 * 64)), block(kBlock);
@@ -2924,7 +2977,15 @@ hipError_t launch_mfma3(const DevData &D, const ScanParams &P, TopState *st, con
     F.class_off[0] = 0;
     F.class_off[1] = fp4_image_units(D, 0);
     const int64_t tiles8 = ((int64_t)m * (m + 1) + 7) / 8, na = a1 - a0;
-    if (tiles8 * na * 8 > 0x7fffffffLL) return hipErrorInvalidValue;
+    if (tiles8 * na * 8 > kMaxGridBlocks) {
+        // a grid below 2^32 work-items: fewer first SNPs per launch
+        const int64_t step = std::max<int64_t>(1, kMaxGridBlocks / (tiles8 * 8));
+        for (int64_t a = a0; a < a1; a += step) {
+            const hipError_t e = launch_mfma3(D, P, st, fp4, (int)a, (int)std::min<int64_t>(a1, a + step), T, ring, s);
+            if (e != hipSuccess) return e;
+        }
+        return hipSuccess;
+    }
     const dim3 grid((unsigned)(tiles8 * na * 8)), block(kBlock);
     if (ring == 3) hipLaunchKernelGGL((k_mfma3<3>), grid, block, 0, s, D, P, st, F, a0, (int)na, T);
     else if (ring == kM3Ring) hipLaunchKernelGGL((k_mfma3<kM3Ring>), grid, block, 0, s, D, P, st, F, a0, (int)na, T);
@@ -3000,27 +3061,42 @@ hipError_t launch_combine_batch(const DevData &D, const int32_t *combos, int k, 
 {
     uint64_t R = 1;
     for (int i = 0; i < k; ++i) R *= 3;
-    const uint64_t total = n * R * (cw + tw);
+    const uint64_t per = R * (cw + tw), total = n * per;
     if (!total) return hipSuccess;
-    hipLaunchKernelGGL(k_combine_batch, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, D.bits, D, combos, k,
-                       n, cw, tw, out);
+    // grids below 2^32 work-items: whole combinations per launch
+    const uint64_t max_n = std::max<uint64_t>(1, 0xffffff00ull / per);
+    if (per > 0xffffff00ull) return hipErrorInvalidValue;
+    for (uint64_t c = 0; c < n; c += max_n) {
+        const uint64_t m = std::min(n - c, max_n);
+        hipLaunchKernelGGL(k_combine_batch, dim3((unsigned)((m * per + 255) / 256)), dim3(256), 0, s, D.bits, D,
+                           combos + c * k, k, m, cw, tw, out + c * per);
+    }
     return hipGetLastError();
 }
 
 hipError_t launch_contingency_batch(const DevData &D, const int32_t *combos, int k, uint64_t n, uint64_t ct_size,
                                     uint32_t *out, hipStream_t s)
 {
-    if (!n) return hipSuccess;
-    hipLaunchKernelGGL(k_contingency_batch, dim3((unsigned)n), dim3(64), 0, s, D.bits, D, combos, k, ct_size, out);
+    // grids below 2^32 work-items (one 64-thread workgroup per combination)
+    constexpr uint64_t kMaxN = 0xffffffffull / 64;
+    for (uint64_t c = 0; c < n; c += kMaxN) {
+        const uint64_t m = std::min(n - c, kMaxN);
+        hipLaunchKernelGGL(k_contingency_batch, dim3((unsigned)m), dim3(64), 0, s, D.bits, D, combos + c * k, k,
+                           ct_size, out + c * 2 * ct_size);
+    }
     return hipGetLastError();
 }
 
 hipError_t launch_mi_batch(const uint32_t *tables, uint64_t n, uint64_t ct_size, double n64, double hy64, float *out,
                            hipStream_t s)
 {
-    if (!n) return hipSuccess;
-    hipLaunchKernelGGL(k_mi_batch, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, tables, n, ct_size, n64, hy64,
-                       out);
+    // grids below 2^32 work-items
+    constexpr uint64_t kMaxN = 0xffffff00ull;
+    for (uint64_t c = 0; c < n; c += kMaxN) {
+        const uint64_t m = std::min(n - c, kMaxN);
+        hipLaunchKernelGGL(k_mi_batch, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, tables + c * 2 * ct_size, m,
+                           ct_size, n64, hy64, out + c);
+    }
     return hipGetLastError();
 }
 

@@ -78,3 +78,18 @@ def test_mfma3_distribution_and_chunks():
     oc, om, _ = O.search(od, 3, 64, threads=2, mi_mode=2)
     assert c.tolist() == oc.tolist() and np.array_equal(m, om)
     assert s.last_stats.redo_chunks > 0
+
+
+def test_mfma3_launch_beyond_the_grid_limit():
+    # one chunk of 4,200 first SNPs needs 553 x 4,198 x 8 = 18.6M workgroups of
+    # 256 threads, above the 2^32 work-items a dispatch holds: the launch is
+    # split over first-SNP ranges, and the result equals the default chunks'
+    print("case: mfma3 4200 SNPs in one chunk", flush=True)
+    snps = 4200
+    d, _ = synth(snps, 64, 64, 91)
+    dist = fa.Distribution(snps, 2)
+    ref = fa.GpuSearch(algorithm=fa.ALGO_MFMA).run_arrays(d, 3, dist, 50)
+    s = fa.GpuSearch(algorithm=fa.ALGO_MFMA, chunk_combos=1 << 40, sample_threshold=False)
+    c, m = s.run_arrays(d, 3, dist, 50)
+    assert s.last_stats.combinations == math.comb(snps, 3)
+    assert c.tolist() == ref[0].tolist() and np.array_equal(m, ref[1])
