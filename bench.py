@@ -50,7 +50,7 @@ VALU_PEAK = 256 * 64 * 2.4e9          # int32 VALU lane-ops per second (MI355X)
 # FETCH_SIZE per dispatch x 2.0 (the factor a plain 1 GiB copy shows) +
 # WRITE_SIZE x 1.0, divided by the triples of those dispatches
 # (profiles/k_scan3m_order3_10000x16000.md, profiles/k_mfma3_order3_10000x16000.md).
-TRAFFIC_BYTES_PER_TRIPLE = {"marginal": 1563.0, "mfma": 45.8}
+TRAFFIC_BYTES_PER_TRIPLE = {"marginal": 1563.0, "mfma": 47.7}
 
 
 def alg_ops_per_combo(order, cases, ctrls):
