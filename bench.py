@@ -231,17 +231,24 @@ def bench_batch(count=64):
 
 
 def bench_tune():
-    """The autotuner over its default shape classes (orders 2 and 3); the
-    table it loads is used by the searches that follow (batched, served)."""
+    """The autotuner over its default shape classes (orders 2 and 3), then
+    the batched popcount kernels' unit sizes over its batch classes (orders
+    4 and 5); the table it loads is used by the searches that follow
+    (batched, served)."""
     from fiuncho_amd import tune
     t0 = time.perf_counter()
-    entries = tune.autotune()
+    tune.autotune()
     dt = time.perf_counter() - t0
-    print(f"[bench] tune: {len(entries)} classes in {dt:.1f} s", file=sys.stderr, flush=True)
+    t1 = time.perf_counter()
+    entries = tune.autotune_batch()
+    dtb = time.perf_counter() - t1
+    print(f"[bench] tune: {len(entries)} classes in {dt:.1f} s, batch units in {dtb:.1f} s", file=sys.stderr,
+          flush=True)
     names = {1: "direct", 2: "marginal", 3: "mfma"}
-    return {"seconds": dt, "classes": [
+    return {"seconds": dt, "batch_seconds": dtb, "classes": [
         {"order": e.order, "snps": [e.snps_lo, e.snps_hi], "samples": [e.samples_lo, e.samples_hi],
-         "algorithm": names.get(e.algorithm, e.algorithm), "variant": e.variant, "ms": e.ms} for e in entries]}
+         "algorithm": names.get(e.algorithm, e.algorithm), "variant": e.variant, "ms": e.ms,
+         "batch_unit": e.batch_unit} for e in entries]}
 
 
 def bench_config1(repeats=20):

@@ -52,7 +52,7 @@ class Config(C.Structure):
 
 
 class TuneEntry(C.Structure):
-    _fields_ = [("order", C.c_uint16), ("reserved", C.c_uint16), ("snps_lo", C.c_uint32),
+    _fields_ = [("order", C.c_uint16), ("batch_unit", C.c_uint16), ("snps_lo", C.c_uint32),
                 ("snps_hi", C.c_uint32), ("samples_lo", C.c_uint32), ("samples_hi", C.c_uint32),
                 ("algorithm", C.c_int32), ("variant", C.c_int32), ("ms", C.c_float)]
 

@@ -1250,6 +1250,7 @@ fiu_status fiu_set_tuning(const fiu_tune_entry *entries, size_t n)
             if (!valid_variant(e.algorithm, e.order, e.variant)) fail(FIU_ERR_INVALID, "tuning entry with an unknown variant");
             if (e.snps_lo >= e.snps_hi || e.samples_lo >= e.samples_hi)
                 fail(FIU_ERR_INVALID, "tuning entry with an empty range");
+            if (e.batch_unit > 6) fail(FIU_ERR_INVALID, "tuning entry with a batch unit above 6");
         }
         std::lock_guard<std::mutex> lk(g_tuning_mu);
         g_tuning.assign(entries, entries + n);

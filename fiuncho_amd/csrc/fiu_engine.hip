@@ -356,10 +356,12 @@ void Engine::plan_job(Job *j)
     // the matrix cores take order 2 (any distribution: the tiles mask other
     // shares) and whole order-3 scans, as the single search does by default
     const int T = d->dev.n_pad / 64;
-    // a tuning entry for this shape class picks the algorithm (the batched
-    // kernels have one variant each)
+    // a tuning entry for this shape class picks the algorithm and the
+    // popcount kinds' unit size
     fiu_tune_entry te{};
-    const int tuned_algo = tuned(order, d->snps, (uint64_t)d->cases + d->ctrls, te) ? te.algorithm : FIU_ALGO_AUTO;
+    const bool has_te = tuned(order, d->snps, (uint64_t)d->cases + d->ctrls, te);
+    const int tuned_algo = has_te ? te.algorithm : FIU_ALGO_AUTO;
+    const double unit_instr = has_te && te.batch_unit ? 40000.0 * std::ldexp(1.0, (int)te.batch_unit - 3) : 40000.0;
     if ((tuned_algo == FIU_ALGO_AUTO || tuned_algo == FIU_ALGO_MFMA) && mfma_dataset(d) && d->fp4_ready &&
         (order == 2 || (order == 3 && d->pair_ready && j->step == 1))) {
         j->kind = order == 2 ? kKindMfma2 : kKindMfma3;
@@ -390,8 +392,9 @@ void Engine::plan_job(Job *j)
     const double epi = marginal ? (order == 3 ? 600.0 : order >= 4 ? 400.0 : 200.0) : 300.0;
     const double per_tile = (double)cells * 8.0 * quads + epi;   // wave instructions per prefix and tile
     const int n_tiles = std::max(1, d->dev.n_pad / 64);
-    // about 40k wave instructions (tens of microseconds) per unit
-    j->tiles = (int)std::clamp(std::lround(40000.0 / (j->pw * per_tile)), 1L, (long)n_tiles);
+    // about 40k wave instructions (tens of microseconds) per unit, or the
+    // tuned size
+    j->tiles = (int)std::clamp(std::lround(unit_instr / (j->pw * per_tile)), 1L, (long)n_tiles);
     // marginal orders 4-5 count the prefix rows once per unit: at least 4
     // tiles (256 last SNPs) per unit keep that pass a few per cent
     if (split) j->tiles = std::min(n_tiles, std::max(j->tiles, 4));

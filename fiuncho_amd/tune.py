@@ -7,9 +7,12 @@ LDS staging depth (order 3) or operand format (order 2), the popcount scans
 with their prefixes per wavefront -- on the whole scan, requires every
 candidate to return the identical top-N list, and keeps the fastest.  The
 table is loaded into the library, where AUTO searches and batched scans of
-that class use it.
+that class use it.  ``autotune_batch`` then times ``fiu_search_batch`` over a
+few datasets of each batch class with every unit size of the batched
+popcount kernels (the entry's ``batch_unit``) and keeps the fastest.
 
     entries = tune.autotune()          # times the default classes, loads the table
+    entries = tune.autotune_batch()    # adds the batched kernels' unit sizes
     tune.save(entries, "tuning.json")  # later: tune.load("tuning.json")
 """
 from __future__ import annotations
@@ -39,6 +42,7 @@ class Entry:
     algorithm: int
     variant: int = 0
     ms: float = 0.0
+    batch_unit: int = 0   # batched popcount kinds: 0 = default unit, k = default x 2^(k-3)
 
 
 @dataclass
@@ -83,7 +87,7 @@ def set_table(entries: Sequence[Entry]) -> None:
         t = arr[i]
         t.order, t.snps_lo, t.snps_hi = e.order, e.snps_lo, e.snps_hi
         t.samples_lo, t.samples_hi = e.samples_lo, e.samples_hi
-        t.algorithm, t.variant, t.ms = e.algorithm, e.variant, e.ms
+        t.algorithm, t.variant, t.ms, t.batch_unit = e.algorithm, e.variant, e.ms, e.batch_unit
     check(lib().fiu_set_tuning(arr, len(entries)))
 
 
@@ -92,8 +96,8 @@ def get_table() -> List[Entry]:
     check(lib().fiu_get_tuning(None, 0, C.byref(n)))
     arr = (_lib.TuneEntry * max(n.value, 1))()
     check(lib().fiu_get_tuning(arr, n.value, C.byref(n)))
-    return [Entry(t.order, t.snps_lo, t.snps_hi, t.samples_lo, t.samples_hi, t.algorithm, t.variant, t.ms)
-            for t in arr[:n.value]]
+    return [Entry(t.order, t.snps_lo, t.snps_hi, t.samples_lo, t.samples_hi, t.algorithm, t.variant, t.ms,
+                  t.batch_unit) for t in arr[:n.value]]
 
 
 def clear() -> None:
@@ -164,3 +168,83 @@ def autotune(classes: Iterable[ShapeClass] = DEFAULT_CLASSES, outputs: int = 50,
         raise
     set_table(entries if apply else saved)
     return entries
+
+
+@dataclass
+class BatchClass:
+    """A class of batched scans: [snps_lo, snps_hi) x [samples_lo, samples_hi)
+    at `order`, timed on `count` datasets of `snps` x `samples` with
+    `algorithm` (a popcount kind: the unit size has no effect on the
+    matrix-core kinds)."""
+    order: int
+    snps_lo: int
+    snps_hi: int
+    samples_lo: int
+    samples_hi: int
+    snps: int
+    samples: int
+    count: int = 4
+    algorithm: int = ALGO_MARGINAL
+
+
+# Orders 4 and 5, the batched scans that run on the popcount kinds.
+DEFAULT_BATCH_CLASSES: List[BatchClass] = [
+    BatchClass(4, 0, 512, 0, 8192, 200, 3000),
+    BatchClass(4, 0, 512, 8192, INF, 160, 16000),
+    BatchClass(5, 0, 128, 0, INF, 50, 4000),
+]
+BATCH_UNITS = (0, 1, 2, 4, 5, 6)
+
+
+def autotune_batch(classes: Iterable[BatchClass] = DEFAULT_BATCH_CLASSES, units: Sequence[int] = BATCH_UNITS,
+                   outputs: int = 50, repeats: int = 2, seed: int = 11, apply: bool = True, log=None) -> List[Entry]:
+    """Time fiu_search_batch over each class's datasets with every unit size;
+    the fastest goes into the class's table entry (added when the table has
+    none for it).  Returns the whole table (loaded when `apply`).  Raises if
+    two unit sizes disagree on a result."""
+    from . import Dataset, Distribution, search_batch, synth_spec
+    saved = get_table()
+    table = list(saved)
+    try:
+        for bc in classes:
+            same = [e for e in table if (e.order, e.snps_lo, e.snps_hi, e.samples_lo, e.samples_hi) ==
+                    (bc.order, bc.snps_lo, bc.snps_hi, bc.samples_lo, bc.samples_hi)]
+            base = same[0] if same else Entry(bc.order, bc.snps_lo, bc.snps_hi, bc.samples_lo, bc.samples_hi,
+                                              bc.algorithm)
+            rest = [e for e in table if e not in same]
+            dss = []
+            for i in range(bc.count):
+                cases = bc.samples // 2
+                dss.append(Dataset.synth(synth_spec(bc.snps, cases, bc.samples - cases, seed=seed + i)))
+            scans = [(d, bc.order, Distribution(bc.snps, bc.order - 1), outputs) for d in dss]
+            best = None
+            ref = None
+            for u in units:
+                entry = Entry(base.order, base.snps_lo, base.snps_hi, base.samples_lo, base.samples_hi,
+                              base.algorithm, base.variant, base.ms, u)
+                set_table(rest + [entry])
+                t = float("inf")
+                res = None
+                for _ in range(repeats):
+                    t0 = time.perf_counter()
+                    res, _ = search_batch(scans)
+                    t = min(t, time.perf_counter() - t0)
+                got = [(c.tolist(), m.tobytes()) for c, m, *_ in res]
+                if ref is None:
+                    ref = got
+                elif got != ref:
+                    raise RuntimeError(f"batch unit {u} disagrees on class {bc}")
+                if log:
+                    log(f"[tune] batch order {bc.order} {bc.count} x {bc.snps} x {bc.samples}: unit {u} "
+                        f"{t * 1e3:.2f} ms")
+                if best is None or t < best[0]:
+                    best = (t, u)
+            for d in dss:
+                d.close()
+            table = rest + [Entry(base.order, base.snps_lo, base.snps_hi, base.samples_lo, base.samples_hi,
+                                  base.algorithm, base.variant, base.ms, best[1])]
+    except BaseException:
+        set_table(saved)
+        raise
+    set_table(table if apply else saved)
+    return table

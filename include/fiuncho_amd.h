@@ -292,10 +292,14 @@ fiu_status fiu_write_results(const char *path, const int32_t *combos,
  *   FIU_ALGO_MFMA, order 2: 0 = fp4 operands (default), 1 = int8 operands.
  * The table is written by an autotuner that times the candidates on the
  * device (fiuncho_amd.tune); it is process-wide and starts empty.  Batched
- * scans use its algorithm; their kernels have one variant each. */
+ * scans use its algorithm and, for the popcount kinds (orders 2-5 off the
+ * matrix cores), its batch_unit: the work of one unit of the persistent
+ * kernels, 0 = the default (about 40,000 wave instructions: that many
+ * 64-SNP tiles of last SNPs per prefix group), k = 1..6 = the default times
+ * 2^(k-3) (a quarter to 8 times). */
 typedef struct {
     uint16_t order;
-    uint16_t reserved;
+    uint16_t batch_unit;              /* batched popcount kinds' unit size, 0..6 */
     uint32_t snps_lo, snps_hi;        /* SNPs in [lo, hi) */
     uint32_t samples_lo, samples_hi;  /* cases + ctrls in [lo, hi) */
     int32_t algorithm;                /* fiu_algorithm (not AUTO) */

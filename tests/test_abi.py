@@ -106,13 +106,15 @@ def test_tuning_table_roundtrip_and_validation():
     E = tune.Entry
     good = [E(2, 0, 1000, 0, 4096, fa.ALGO_MARGINAL, 8, 1.5), E(3, 100, 200, 10, 20, fa.ALGO_MFMA, 3),
             E(4, 0, 50, 0, 100, fa.ALGO_DIRECT, 0), E(5, 0, 50, 0, 100, fa.ALGO_MARGINAL, 0),
-            E(7, 0, 50, 0, 100, fa.ALGO_DIRECT, 0), E(6, 0, 50, 0, 100, fa.ALGO_DIRECT, 0)]
+            E(7, 0, 50, 0, 100, fa.ALGO_DIRECT, 0), E(6, 0, 50, 0, 100, fa.ALGO_DIRECT, 0),
+            E(4, 50, 500, 0, 100, fa.ALGO_MARGINAL, 0, 0.0, 6)]
     try:
         tune.set_table(good)
         got = tune.get_table()
         assert [(e.order, e.snps_lo, e.snps_hi, e.samples_lo, e.samples_hi, e.algorithm, e.variant) for e in got] == \
                [(e.order, e.snps_lo, e.snps_hi, e.samples_lo, e.samples_hi, e.algorithm, e.variant) for e in good]
         assert abs(got[0].ms - 1.5) < 1e-6
+        assert [e.batch_unit for e in got] == [e.batch_unit for e in good]
         bad = [E(3, 0, 10, 0, 10, fa.ALGO_MARGINAL, 8),     # 8 prefixes per wave: order 2 only
                E(2, 0, 10, 0, 10, fa.ALGO_MFMA, 3),         # ring depth: order 3 only
                E(3, 0, 10, 0, 10, fa.ALGO_MFMA, 5),         # no such staging variant
@@ -123,7 +125,8 @@ def test_tuning_table_roundtrip_and_validation():
                E(4, 0, 10, 0, 10, fa.ALGO_MARGINAL, 4),     # prefixes per wave: orders 2-3
                E(2, 10, 10, 0, 10, fa.ALGO_DIRECT, 0),      # empty SNP range
                E(2, 0, 10, 0, 10, fa.ALGO_AUTO, 0),         # AUTO is not a choice
-               E(1, 0, 10, 0, 10, fa.ALGO_DIRECT, 0)]
+               E(1, 0, 10, 0, 10, fa.ALGO_DIRECT, 0),
+               E(4, 0, 10, 0, 10, fa.ALGO_MARGINAL, 0, 0.0, 7)]     # batch unit 0..6
         for b in bad:
             try:
                 tune.set_table([b])

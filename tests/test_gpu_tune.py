@@ -79,3 +79,29 @@ def test_autotune_steers_auto_searches_and_batches():
     assert s.last_stats.algorithm == fa.ALGO_DIRECT and s.last_stats.variant == 0
     oc, om, _ = O.search(od, 2, 10, threads=2, mi_mode=2)
     assert c.tolist() == oc.tolist() and np.array_equal(m, om)
+
+
+def test_batch_unit_sizes_keep_results():
+    # the batched popcount kinds' unit size (a tuning entry's batch_unit)
+    # changes how the work is cut, not what is found
+    from fiuncho_amd import tune
+    print("case: batch unit sizes, order 4", flush=True)
+    dss = [synth(snps, 900, 700, 140 + snps)[0] for snps in (60, 75, 90)]
+    scans = [(d, 4, fa.Distribution(d.snps, 3), 40) for d in dss]
+    plain = [fa.GpuSearch().run_arrays(*sc) for sc in scans]
+    try:
+        for unit in (1, 4, 6):
+            tune.set_table([tune.Entry(4, 0, 1000, 0, 100000, fa.ALGO_MARGINAL, 0, 0.0, unit)])
+            got, _ = fa.search_batch(scans)
+            for (c, m), (pc, pm) in zip(got, plain):
+                assert c.tolist() == pc.tolist() and np.array_equal(m, pm), unit
+        # the tuner times every unit size of a class and keeps one
+        table = tune.autotune_batch([tune.BatchClass(4, 0, 1000, 0, 100000, 70, 1600, count=2)], units=(0, 2, 5),
+                                    repeats=1)
+        entry = [e for e in table if e.order == 4]
+        assert len(entry) == 1 and entry[0].batch_unit in (0, 2, 5)
+        assert entry[0].algorithm == fa.ALGO_MARGINAL
+    finally:
+        tune.clear()
+        for d in dss:
+            d.close()
