@@ -455,7 +455,7 @@ bool valid_variant(int algo, int order, int variant)
     switch (algo) {
     case FIU_ALGO_MARGINAL:
         return order <= 3 && (variant == 2 || variant == kMarginalPW || (order == 2 && variant == 8));
-    case FIU_ALGO_MFMA: return order == 3 ? (variant == 3 || variant == 4) : variant == 1;
+    case FIU_ALGO_MFMA: return order == 3 ? (variant >= 3 && variant <= 6) : variant == 1;
     default: return false;
     }
 }
@@ -908,7 +908,11 @@ void search(fiu_dataset *d, Workspace &ws, int order, int32_t step, int32_t offs
     st.algorithm = algo;
     st.variant = variant;
     const int pw = algo == FIU_ALGO_MARGINAL && variant ? variant : kMarginalPW;
-    const int ring = algo == FIU_ALGO_MFMA && order == 3 && variant ? variant : 4;
+    // order-3 matrix-core variants: 3 and 4 the 64 x 32 tile with rings of 3
+    // and 4 stages, 5 and 6 the 64 x 64 tile (default; its counts wait as
+    // 16-bit pairs: classes below 2^16 samples, else the 64 x 32 tile)
+    int ring = algo == FIU_ALGO_MFMA && order == 3 && variant ? variant : 6;
+    if (ring >= 5 && (d->cases >= (1u << 16) || d->ctrls >= (1u << 16))) ring = 4;
     // matrix-core scans walk super-blocks of 8 x 8 tiles of 64 x 64 pairs;
     // fp4 operands accumulate in fp32, exact while every count is below 2^24;
     // they come from the dataset's fp4 image (int8 when it is not there)

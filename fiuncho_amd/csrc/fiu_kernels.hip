@@ -300,6 +300,34 @@ __device__ __forceinline__ void push_candidates(bool want, float mi, uint64_t co
     }
 }
 
+// Screening MI, then exact MI and the key test for the survivors (one lane's
+// combination; the push is separate).
+template <int CELLS, class T, class K>
+__device__ __forceinline__ void screen_exact(const T *a0, const T *a1, K &&key, float tau, uint64_t tau_combo,
+                                             const ScanParams &P, bool &want, float &mf, uint64_t &combo)
+{
+    const float ms = mi_screen<CELLS>(a0, a1, P.hy, P.inv_n);
+    if (ms >= tau - P.delta) {
+        if constexpr (std::is_same<T, uint32_t>::value) {
+            mf = (float)mi_exact<CELLS>(a0, a1, P.n64, P.hy64);
+        } else {
+            uint32_t u0[CELLS], u1[CELLS];
+#pragma unroll
+            for (int x = 0; x < CELLS; ++x) u0[x] = (uint32_t)a0[x], u1[x] = (uint32_t)a1[x];
+            mf = (float)mi_exact<CELLS>(u0, u1, P.n64, P.hy64);
+        }
+        combo = key();
+        want = better(mf, combo, tau, tau_combo);
+    }
+}
+
+// Whether a valid combination passes the bound (always, without it).
+template <int CELLS, class T>
+__device__ __forceinline__ bool bound_ok(const T *a0, const T *a1, float tau, const ScanParams &P)
+{
+    return !P.bounded || bound_passes(mi_bound<CELLS>(a0, a1, P.frac_c, P.chi_scale), tau, P.delta);
+}
+
 // Epilogue shared by every scan kernel: screening MI, exact MI for the
 // survivors, key test against the current N-th best, push.  key() gives the
 // combination's key; it is only called for the few combinations that reach
@@ -311,21 +339,7 @@ __device__ __forceinline__ void score_and_push(bool valid, const T *a0, const T 
     bool want = false;
     float mf = 0.0f;
     uint64_t combo = ~0ull;
-    if (valid && (!P.bounded || bound_passes(mi_bound<CELLS>(a0, a1, P.frac_c, P.chi_scale), tau, P.delta))) {
-        const float ms = mi_screen<CELLS>(a0, a1, P.hy, P.inv_n);
-        if (ms >= tau - P.delta) {
-            if constexpr (std::is_same<T, uint32_t>::value) {
-                mf = (float)mi_exact<CELLS>(a0, a1, P.n64, P.hy64);
-            } else {
-                uint32_t u0[CELLS], u1[CELLS];
-#pragma unroll
-                for (int x = 0; x < CELLS; ++x) u0[x] = (uint32_t)a0[x], u1[x] = (uint32_t)a1[x];
-                mf = (float)mi_exact<CELLS>(u0, u1, P.n64, P.hy64);
-            }
-            combo = key();
-            want = better(mf, combo, tau, tau_combo);
-        }
-    }
+    if (valid && bound_ok<CELLS>(a0, a1, tau, P)) screen_exact<CELLS>(a0, a1, key, tau, tau_combo, P, want, mf, combo);
     push_candidates(want, mf, combo, P, st);
 }
 
@@ -1831,6 +1845,112 @@ constexpr int kM3Stage = 13 * kM3Group;
 constexpr int kM3Ring = 4;   // default stages resident: the one read + up to 3 loading
 __device__ __forceinline__ int m3_swz(int r) { return (r >> 1) & 3; }
 
+// Values of the order-3 matrix-core epilogue that depend only on the first
+// SNP a and the lane's last SNP c: c's singles, the pair (a, c) and the rank
+// of the prefix (a, a + 1).
+struct M3Lane {
+    int c;
+    bool c_ok;
+    const uint32_t *sa;
+    uint32_t sc[2][2], fac[8];
+    int64_t rank_a;
+};
+
+__device__ __forceinline__ M3Lane m3_lane(const DevData &D, const ScanParams &P, int a, int c)
+{
+    M3Lane L;
+    const int n = D.n;
+    L.c = c;
+    L.c_ok = c < n && c > a;
+    L.sa = D.singles + (size_t)a * 6;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) L.fac[e] = 0;
+    L.sc[0][0] = L.sc[0][1] = L.sc[1][0] = L.sc[1][1] = 0;
+    if (L.c_ok) {
+        const uint32_t *sp = D.singles + (size_t)c * 6;
+        L.sc[0][0] = sp[0], L.sc[0][1] = sp[1], L.sc[1][0] = sp[3], L.sc[1][1] = sp[4];
+        const uint4 *pac = (const uint4 *)(P.pair + pair_index(a, c, n) * 8);
+        const uint4 u0 = pac[0], u1 = pac[1];
+        L.fac[0] = u0.x, L.fac[1] = u0.y, L.fac[2] = u0.z, L.fac[3] = u0.w;
+        L.fac[4] = u1.x, L.fac[5] = u1.y, L.fac[6] = u1.z, L.fac[7] = u1.w;
+    }
+    L.rank_a = (int64_t)a * (2 * (int64_t)n - a - 1) / 2;   // rank of the prefix (a, a+1)
+    return L;
+}
+
+// Triple (a, b, L.c): the 8 matrix-core counts cnt(cls, x, y, z), x, y, z in
+// {0, 1}, with the pair table, the singles and the class totals give the 27
+// cells m0 / m1 of each class (inclusion-exclusion in FP32: the counts are
+// exact integers below 2^24, and so is every step); returns whether the
+// triple is one of the scan's.  m3_triple: then the shared epilogue.
+template <class C>
+__device__ __forceinline__ bool m3_cells(const DevData &D, const ScanParams &P, const M3Lane &L, int a, int b, C &&cnt,
+                                         float (&m0)[27], float (&m1)[27])
+{
+    const int n = D.n, c = L.c;
+    const int64_t rel = L.rank_a + (b - a - 1) - P.offset;
+    const bool valid = L.c_ok && b > a && c > b && rel >= 0 && (P.step == 1 || rel % P.step == 0);
+    uint32_t fab[8] = {0, 0, 0, 0, 0, 0, 0, 0}, fbc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    uint32_t sb[6] = {0, 0, 0, 0, 0, 0};
+    if (valid) {
+        const uint4 *pab = (const uint4 *)(P.pair + pair_index(a, b, n) * 8);
+        const uint4 *pbc = (const uint4 *)(P.pair + pair_index(b, c, n) * 8);
+        const uint4 p0 = pab[0], p1 = pab[1], q0 = pbc[0], q1 = pbc[1];
+        fab[0] = p0.x, fab[1] = p0.y, fab[2] = p0.z, fab[3] = p0.w;
+        fab[4] = p1.x, fab[5] = p1.y, fab[6] = p1.z, fab[7] = p1.w;
+        fbc[0] = q0.x, fbc[1] = q0.y, fbc[2] = q0.z, fbc[3] = q0.w;
+        fbc[4] = q1.x, fbc[5] = q1.y, fbc[6] = q1.z, fbc[7] = q1.w;
+        const uint32_t *sp = D.singles + (size_t)b * 6;
+#pragma unroll
+        for (int e = 0; e < 6; ++e) sb[e] = sp[e];
+    }
+#pragma unroll
+    for (int cls = 0; cls < 2; ++cls) {
+        float *mm = cls ? m1 : m0;
+        // mm[x*9 + y*3 + z], x, y, z in {0, 1, 2 = any}
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+            for (int y = 0; y < 2; ++y) {
+#pragma unroll
+                for (int z = 0; z < 2; ++z) mm[x * 9 + y * 3 + z] = cnt(cls, x, y, z);
+                mm[x * 9 + y * 3 + 2] = (float)fab[cls * 4 + x * 2 + y];
+            }
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+            for (int z = 0; z < 2; ++z) mm[x * 9 + 6 + z] = (float)L.fac[cls * 4 + x * 2 + z];
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int z = 0; z < 2; ++z) mm[18 + y * 3 + z] = (float)fbc[cls * 4 + y * 2 + z];
+        mm[0 * 9 + 8] = (float)L.sa[cls * 3 + 0];
+        mm[1 * 9 + 8] = (float)L.sa[cls * 3 + 1];
+        mm[18 + 0 * 3 + 2] = (float)sb[cls * 3 + 0];
+        mm[18 + 1 * 3 + 2] = (float)sb[cls * 3 + 1];
+        mm[18 + 6 + 0] = (float)L.sc[cls][0];
+        mm[18 + 6 + 1] = (float)L.sc[cls][1];
+        mm[26] = (float)D.total[cls];
+        mobius3(mm);
+    }
+    return valid;
+}
+
+__device__ __forceinline__ uint64_t m3_key(const ScanParams &P, int n, int a, int b, int c)
+{
+    const int pre[2] = {a, b};
+    return prefix_key(P.binom, n, pre, 2) + (uint64_t)c;
+}
+
+template <class C>
+__device__ __forceinline__ void m3_triple(const DevData &D, const ScanParams &P, TopState *st, const M3Lane &L, int a,
+                                          int b, float tau, uint64_t tau_combo, C &&cnt)
+{
+    float m0[27], m1[27];
+    const bool valid = m3_cells(D, P, L, a, b, cnt, m0, m1);
+    score_and_push<27>(valid, m0, m1, [&] { return m3_key(P, D.n, a, b, L.c); }, tau, tau_combo, P, st);
+}
+
 // One workgroup's work item: first SNP a, tile u (see k_mfma3); returns at
 // once for items outside the triangle.  Every wave of the workgroup calls it
 // with the same arguments.  R: stages of the LDS ring (R - 1 in flight).
@@ -1990,81 +2110,17 @@ __device__ __forceinline__ void mfma3_tile(const DevData &D, const ScanParams &P
     }
     // epilogue: lane holds column c = c0 + wc*16 + lane%16 and, per b block
     // bb, rows b = b0 + wb*32 + 16 bb + 4(lane/16) + j, j < 4
-    const int n = D.n;
     const float tau = st->tau;
     const uint64_t tau_combo = st->tau_combo;
-    const int c = c0 + wc * 16 + rr;
-    const bool c_ok = c < n && c > a;
-    const uint32_t *pair = P.pair;
-    const uint32_t *sa = D.singles + (size_t)a * 6;
-    uint32_t sc[2][2] = {{0, 0}, {0, 0}}, fac[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    if (c_ok) {
-        const uint32_t *sp = D.singles + (size_t)c * 6;
-        sc[0][0] = sp[0], sc[0][1] = sp[1], sc[1][0] = sp[3], sc[1][1] = sp[4];
-        const uint4 *pac = (const uint4 *)(pair + pair_index(a, c, n) * 8);
-        const uint4 u0 = pac[0], u1 = pac[1];
-        fac[0] = u0.x, fac[1] = u0.y, fac[2] = u0.z, fac[3] = u0.w;
-        fac[4] = u1.x, fac[5] = u1.y, fac[6] = u1.z, fac[7] = u1.w;
-    }
-    const int64_t rank_a = (int64_t)a * (2 * (int64_t)n - a - 1) / 2;   // rank of the prefix (a, a+1)
+    const M3Lane L = m3_lane(D, P, a, c0 + wc * 16 + rr);
     // eight instances (bb, j), so that every accumulator index is a constant
     static_for<8>([&](auto BJ) __attribute__((always_inline)) {
-            constexpr int bb = decltype(BJ)::value >> 2, j = decltype(BJ)::value & 3;
-            const int b = b0 + wb * 32 + bb * 16 + 4 * (lane >> 4) + j;
-            const int64_t rel = rank_a + (b - a - 1) - P.offset;
-            const bool valid = c_ok && b > a && c > b && rel >= 0 && (P.step == 1 || rel % P.step == 0);
-            uint32_t fab[8] = {0, 0, 0, 0, 0, 0, 0, 0}, fbc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-            uint32_t sb[6] = {0, 0, 0, 0, 0, 0};
-            if (valid) {
-                const uint4 *pab = (const uint4 *)(pair + pair_index(a, b, n) * 8);
-                const uint4 *pbc = (const uint4 *)(pair + pair_index(b, c, n) * 8);
-                const uint4 p0 = pab[0], p1 = pab[1], q0 = pbc[0], q1 = pbc[1];
-                fab[0] = p0.x, fab[1] = p0.y, fab[2] = p0.z, fab[3] = p0.w;
-                fab[4] = p1.x, fab[5] = p1.y, fab[6] = p1.z, fab[7] = p1.w;
-                fbc[0] = q0.x, fbc[1] = q0.y, fbc[2] = q0.z, fbc[3] = q0.w;
-                fbc[4] = q1.x, fbc[5] = q1.y, fbc[6] = q1.z, fbc[7] = q1.w;
-                const uint32_t *sp = D.singles + (size_t)b * 6;
-#pragma unroll
-                for (int e = 0; e < 6; ++e) sb[e] = sp[e];
-            }
-            // counts in FP32: the products are exact integers below 2^24, and
-            // so is every step of the inclusion-exclusion
-            float m0[27], m1[27];
-#pragma unroll
-            for (int cls = 0; cls < 2; ++cls) {
-                float *mm = cls ? m1 : m0;
-                // mm[x*9 + y*3 + z], x, y, z in {0, 1, 2 = any}
-#pragma unroll
-                for (int x = 0; x < 2; ++x)
-#pragma unroll
-                    for (int y = 0; y < 2; ++y) {
-#pragma unroll
-                        for (int z = 0; z < 2; ++z) mm[x * 9 + y * 3 + z] = acc[cls][x][y][bb][z][j];
-                        mm[x * 9 + y * 3 + 2] = (float)fab[cls * 4 + x * 2 + y];
-                    }
-#pragma unroll
-                for (int x = 0; x < 2; ++x)
-#pragma unroll
-                    for (int z = 0; z < 2; ++z) mm[x * 9 + 6 + z] = (float)fac[cls * 4 + x * 2 + z];
-#pragma unroll
-                for (int y = 0; y < 2; ++y)
-#pragma unroll
-                    for (int z = 0; z < 2; ++z) mm[18 + y * 3 + z] = (float)fbc[cls * 4 + y * 2 + z];
-                mm[0 * 9 + 8] = (float)sa[cls * 3 + 0];
-                mm[1 * 9 + 8] = (float)sa[cls * 3 + 1];
-                mm[18 + 0 * 3 + 2] = (float)sb[cls * 3 + 0];
-                mm[18 + 1 * 3 + 2] = (float)sb[cls * 3 + 1];
-                mm[18 + 6 + 0] = (float)sc[cls][0];
-                mm[18 + 6 + 1] = (float)sc[cls][1];
-                mm[26] = (float)D.total[cls];
-                mobius3(mm);
-            }
-            auto key = [&] {
-                const int pre[2] = {a, b};
-                return prefix_key(P.binom, n, pre, 2) + (uint64_t)c;
-            };
-            score_and_push<27>(valid, m0, m1, key, tau, tau_combo, P, st);
-        });
+        constexpr int bb = decltype(BJ)::value >> 2, j = decltype(BJ)::value & 3;
+        m3_triple(D, P, st, L, a, b0 + wb * 32 + bb * 16 + 4 * (lane >> 4) + j, tau, tau_combo,
+                  [&](int cls, int x, int y, int z) __attribute__((always_inline)) {
+                      return acc[cls][x][y][bb][z][j];
+                  });
+    });
 }
 
 template <int R>
@@ -2078,6 +2134,219 @@ k_mfma3(const DevData D, const ScanParams P, TopState *st, const Fp4Image F, int
     // rows in its L2 (only the a rows differ)
     const int64_t L = blockIdx.x, s = L >> 3;
     mfma3_tile<R>(D, P, st, F, a0 + (int)(s % na), (s / na) * 8 + (L & 7), T, lds);
+}
+
+// Wide tile: one workgroup takes first SNP a and a whole 64 x 64 tile of
+// (b, c); wave w multiplies all 64 b rows by the 16 c rows c0 + 16 w ...: 32
+// products per 128-sample stage against 12 fragment reads and 16 masks (16,
+// 8 and 16 per 16 products in mfma3_tile), and 16 KB staged per 128 products
+// (12 KB per 64).  One class's counts fill the 128 accumulators; the first
+// class's wait as 16-bit pairs (64 registers) while the second runs, so
+// every class must hold fewer than 2^16 samples.  Stage layout: groups 0..7
+// the b rows (y, 16-row block bb), 8..15 the c rows (z, block), 16 the a
+// rows, each stored as in mfma3_tile.
+constexpr int kM3wStage = 17 * kM3Group;
+
+// Counts of b block bb = i / 4, (x, y) = (i / 2 % 2, i % 2) as 16-bit pairs:
+// word z * 2 + j / 2 holds f(x, y, z) of rows 4 (lane / 16) + j, in the low
+// (j even) or high half (every count is below 2^16).
+__device__ __forceinline__ v4i pack_counts(const v4f (&acc)[2][2][4][2], int i)
+{
+    auto pk = [](float lo, float hi) __attribute__((always_inline)) {
+        return (int)((uint32_t)lo | (uint32_t)hi << 16);
+    };
+    const int bb = i >> 2, x = (i >> 1) & 1, y = i & 1;
+    const v4f &v0 = acc[x][y][bb][0], &v1 = acc[x][y][bb][1];
+    return (v4i){pk(v0[0], v0[1]), pk(v0[2], v0[3]), pk(v1[0], v1[1]), pk(v1[2], v1[3])};
+}
+
+template <int R>
+__device__ __forceinline__ void mfma3w_tile(const DevData &D, const ScanParams &P, TopState *st, const Fp4Image &F,
+                                            int a, int64_t u, int T, int8_t (*lds)[kM3wStage])
+{
+    if (a >= D.n - 2) return;
+    const int ta = (a + 1) >> 6, m = T - ta;
+    if (u >= (int64_t)m * (m + 1) / 2) return;
+    int r, cc;
+    tri_tile(u, m, r, cc);
+    const int b0 = (ta + r) * 64, c0 = (ta + cc) * 64;
+    if (c0 >= D.n) return;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int np = D.n_pad;
+    // staging: wave w loads b groups 2w, 2w + 1 (y = w / 2, blocks 2 (w % 2)
+    // and the next), c groups 8 + 2w, 9 + 2w (z = w / 2, same blocks) and,
+    // lanes 2w, 2w + 1, the a rows; the second group of each pair is the
+    // next 16 rows (+1,024 bytes)
+    const int lr = lane >> 2, ls = (lane & 3) ^ m3_swz(lr);
+    const uint32_t ub = 16u * (uint32_t)(4 * ((w >> 1) * np + b0 + 32 * (w & 1) + lr) + ls);
+    const uint32_t uc = 16u * (uint32_t)(4 * ((w >> 1) * np + c0 + 32 * (w & 1) + lr) + ls);
+    const bool a_lane = (lane >> 1) == w;
+    const uint32_t ua = 16u * (uint32_t)(4 * ((lr & 1) * np + a) + ls);
+    v4f acc[2][2][4][2];   // [x][y][b block][z], the class being scanned
+    // the counts (pack_counts; 16 words per class) in private memory: the
+    // first class's out of the registers the second class needs, both for
+    // the epilogue's loop over the b blocks.  The pointer passes through an empty
+    // asm statement, so the compiler keeps plain 16-byte private stores and
+    // loads (a volatile access would be uncached and waited for one by one)
+    v4i stash_mem[32];
+    __attribute__((address_space(5))) v4i *stash = (__attribute__((address_space(5))) v4i *)stash_mem;
+    asm volatile("" : "+v"(stash));
+    const int rr = lane & 15, kc = lane >> 4;
+    const int fo = rr * 64 + ((kc ^ m3_swz(rr)) << 4), ao = kc << 4;
+#pragma unroll
+    for (int cls = 0; cls < 2; ++cls) {
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+            for (int y = 0; y < 2; ++y)
+#pragma unroll
+                for (int bb = 0; bb < 4; ++bb)
+#pragma unroll
+                    for (int z = 0; z < 2; ++z) acc[x][y][bb][z] = (v4f){0.f, 0.f, 0.f, 0.f};
+        const int iters = D.quads[cls];
+        const uint4 *img = F.base + F.class_off[cls];
+        const int64_t qstride = 8 * (int64_t)np;
+        auto quad_rsrc = [&](int q) __attribute__((always_inline)) {
+            return __builtin_amdgcn_make_buffer_rsrc((void *)(img + q * qstride), (short)0, 128 * np, 0x00020000);
+        };
+        auto issue_bc = [&](int q) __attribute__((always_inline)) {
+            const __amdgpu_buffer_rsrc_t rq = quad_rsrc(q);
+            int8_t *s = lds[q % R];
+            dma16(rq, ub, s + (2 * w) * kM3Group);
+            dma16(rq, ub + 1024u, s + (2 * w + 1) * kM3Group);
+            dma16(rq, uc, s + (8 + 2 * w) * kM3Group);
+            dma16(rq, uc + 1024u, s + (9 + 2 * w) * kM3Group);
+        };
+        auto issue_a = [&](int q) __attribute__((always_inline)) {
+            if (a_lane) dma16(quad_rsrc(q), ua, lds[q % R] + 16 * kM3Group);
+        };
+#pragma unroll
+        for (int q = 0; q < R - 1; ++q)
+            if (q < iters) {
+                issue_bc(q);
+                issue_a(q);
+            }
+        auto stage = [&](int it, bool steady) __attribute__((always_inline)) {
+            // 5 loads per stage (the a rows' load counts for every lane)
+            static_assert(R >= 2 && 5 * (R - 2) < 16, "ring depth");
+            if (steady) wait_vmcnt<5 * (R - 2)>();
+            else wait_stages<R - 2, 5>(iters - 1 - it);
+            stage_barrier();
+            __builtin_amdgcn_s_setprio(1);
+            const int8_t *sb = lds[it % R];
+            // x outer, b fragment, z inner: the x = 0 products need the a
+            // rows, both c fragments and then the b fragments one by one, so
+            // the b reads and the x = 1 masks issue between products, in
+            // segments the compiler keeps in order (12 reads in a row
+            // overfill the LDS command queue)
+            v4i ma[2], fc[2], fb[8], mc[2][2];
+#pragma unroll
+            for (int x = 0; x < 2; ++x) ma[x] = *(const v4i *)(sb + 16 * kM3Group + x * 64 + ao);
+#pragma unroll
+            for (int z = 0; z < 2; ++z) fc[z] = *(const v4i *)(sb + (8 + 4 * z + w) * kM3Group + fo);
+#pragma unroll
+            for (int k = 0; k < 2; ++k) fb[k] = *(const v4i *)(sb + k * kM3Group + fo);
+            const bool more = steady || it + R - 1 < iters;
+            if (more) issue_bc(it + R - 1);
+#pragma unroll
+            for (int z = 0; z < 2; ++z) mc[0][z] = fc[z] & ma[0];   // nibble 0x2 & 0x2 = 1.0, else 0
+            __builtin_amdgcn_sched_barrier(0);
+            auto product = [&](int x, int k, int z) __attribute__((always_inline)) {
+                const v8i b8 = {fb[k][0], fb[k][1], fb[k][2], fb[k][3], 0, 0, 0, 0};
+                const v8i c8 = {mc[x][z][0], mc[x][z][1], mc[x][z][2], mc[x][z][3], 0, 0, 0, 0};
+                acc[x][k >> 2][k & 3][z] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(
+                    b8, c8, acc[x][k >> 2][k & 3][z], 4, 4, 0, 0, 0, 0);
+            };
+            // b fragment k (y = k / 4, block k % 4): the read of fragment k +
+            // 2, one word of the x = 1 masks, the two x = 0 products
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                if (k < 6) fb[k + 2] = *(const v4i *)(sb + (k + 2) * kM3Group + fo);
+                mc[1][k >> 2][k & 3] = fc[k >> 2][k & 3] & ma[1][k & 3];
+                product(0, k, 0);
+                product(0, k, 1);
+                __builtin_amdgcn_sched_barrier(0);
+            }
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                product(1, k, 0);
+                product(1, k, 1);
+            }
+            // the a rows' load (two lanes, a branch) at the stage's priority:
+            // 2 % faster than after the drop
+            if (more) issue_a(it + R - 1);
+            __builtin_amdgcn_s_setprio(0);
+        };
+        int it = 0;
+        for (; it + R - 1 < iters; ++it) stage(it, true);
+        for (; it < iters; ++it) stage(it, false);
+        stage_barrier();
+#pragma unroll
+        for (int i = 0; i < 16; ++i) stash[16 * cls + i] = pack_counts(acc, i);
+    }
+    // epilogue: lane holds column c = c0 + 16 w + lane % 16 and, per b block
+    // bb, rows b = b0 + 16 bb + 4 (lane / 16) + j, j < 4
+    const float tau = st->tau;
+    const uint64_t tau_combo = st->tau_combo;
+    const M3Lane L = m3_lane(D, P, a, c0 + 16 * w + rr);
+    // first pass: the bound of each of the 16 instances (bb, j); a lane's
+    // instances that pass are marked in `pend` (bit 4 bb + j).  Second pass:
+    // each round scores one marked instance per lane, its counts read back
+    // from the stash, so the screening MI (54 logarithms) runs about as often
+    // as the most marked lane has marks, not once per instance in which any
+    // lane of the wave passes the bound (most instances, at a top-100
+    // threshold)
+    uint32_t pend = 0;
+    static_for<4>([&](auto BB) __attribute__((always_inline)) {
+        constexpr int bb = decltype(BB)::value;
+        v4i lo[4];
+#pragma unroll
+        for (int xy = 0; xy < 4; ++xy) lo[xy] = stash[4 * bb + xy];
+        static_for<4>([&](auto J) __attribute__((always_inline)) {
+            constexpr int j = decltype(J)::value;
+            float m0[27], m1[27];
+            const bool valid = m3_cells(
+                D, P, L, a, b0 + 16 * bb + 4 * (lane >> 4) + j,
+                [&](int cls, int x, int y, int z) __attribute__((always_inline)) {
+                    return cls ? acc[x][y][bb][z][j]
+                               : (float)((uint32_t)lo[x * 2 + y][z * 2 + (j >> 1)] >> (16 * (j & 1)) & 0xffffu);
+                },
+                m0, m1);
+            if (valid && bound_ok<27>(m0, m1, tau, P)) pend |= 1u << (4 * bb + j);
+        });
+    });
+    const __attribute__((address_space(5))) uint32_t *sw = (const __attribute__((address_space(5))) uint32_t *)stash;
+    while (__ballot(pend != 0)) {
+        bool want = false;
+        float mf = 0.0f;
+        uint64_t combo = ~0ull;
+        if (pend) {
+            const int i = __builtin_ctz(pend), bb = i >> 2, j = i & 3;
+            pend &= pend - 1;
+            const int b = b0 + 16 * bb + 4 * (lane >> 4) + j;
+            float m0[27], m1[27];
+            m3_cells(
+                D, P, L, a, b,
+                [&](int cls, int x, int y, int z) __attribute__((always_inline)) {
+                    const uint32_t v = sw[4 * (16 * cls + 4 * bb + x * 2 + y) + z * 2 + (j >> 1)];
+                    return (float)(v >> (16 * (j & 1)) & 0xffffu);
+                },
+                m0, m1);
+            screen_exact<27>(m0, m1, [&] { return m3_key(P, D.n, a, b, L.c); }, tau, tau_combo, P, want, mf, combo);
+        }
+        push_candidates(want, mf, combo, P, st);
+    }
+}
+
+template <int R>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 2)))
+k_mfma3w(const DevData D, const ScanParams P, TopState *st, const Fp4Image F, int a0, int na, int T)
+{
+    __shared__ __attribute__((aligned(1024))) int8_t lds[R][kM3wStage];
+    // block order as in k_mfma3 (XCD L % 8 takes tiles u = 8 i + L % 8)
+    const int64_t L = blockIdx.x, s = L >> 3;
+    mfma3w_tile<R>(D, P, st, F, a0 + (int)(s % na), (s / na) * 8 + (L & 7), T, lds);
 }
 
 
@@ -2976,7 +3245,11 @@ hipError_t launch_mfma3(const DevData &D, const ScanParams &P, TopState *st, con
     F.base = fp4;
     F.class_off[0] = 0;
     F.class_off[1] = fp4_image_units(D, 0);
-    const int64_t tiles8 = ((int64_t)m * (m + 1) + 7) / 8, na = a1 - a0;
+    // work items per first SNP: 64 x 32 half tiles, or whole 64 x 64 tiles
+    // (rings 5 and 6: the wide tile with 3 and 4 stages)
+    const bool wide = ring >= 5;
+    const int64_t items = wide ? (int64_t)m * (m + 1) / 2 : (int64_t)m * (m + 1);
+    const int64_t tiles8 = (items + 7) / 8, na = a1 - a0;
     if (tiles8 * na * 8 > kMaxGridBlocks) {
         // a grid below 2^32 work-items: fewer first SNPs per launch
         const int64_t step = std::max<int64_t>(1, kMaxGridBlocks / (tiles8 * 8));
@@ -2989,6 +3262,8 @@ hipError_t launch_mfma3(const DevData &D, const ScanParams &P, TopState *st, con
     const dim3 grid((unsigned)(tiles8 * na * 8)), block(kBlock);
     if (ring == 3) hipLaunchKernelGGL((k_mfma3<3>), grid, block, 0, s, D, P, st, F, a0, (int)na, T);
     else if (ring == kM3Ring) hipLaunchKernelGGL((k_mfma3<kM3Ring>), grid, block, 0, s, D, P, st, F, a0, (int)na, T);
+    else if (ring == 5) hipLaunchKernelGGL((k_mfma3w<3>), grid, block, 0, s, D, P, st, F, a0, (int)na, T);
+    else if (ring == 6) hipLaunchKernelGGL((k_mfma3w<4>), grid, block, 0, s, D, P, st, F, a0, (int)na, T);
     else return hipErrorInvalidValue;
     return hipGetLastError();
 }

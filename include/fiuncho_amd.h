@@ -288,7 +288,11 @@ fiu_status fiu_write_results(const char *path, const int32_t *combos,
  * allows them (else the default choice).  Variants:
  *   FIU_ALGO_MARGINAL: prefixes per wavefront, 2, 4 (default) or, order 2
  *                      only, 8;
- *   FIU_ALGO_MFMA, order 3: LDS staging depth, 3 or 4 (default) stages;
+ *   FIU_ALGO_MFMA, order 3: tile and LDS staging depth: 3, 4 = 64 x 32
+ *                      (b, c) per workgroup with 3 or 4 stages, 5, 6 =
+ *                      64 x 64 with 3 or 4 stages (6 is the default; 5 and 6
+ *                      fall back to 4 when a class holds 2^16 samples or
+ *                      more);
  *   FIU_ALGO_MFMA, order 2: 0 = fp4 operands (default), 1 = int8 operands.
  * The table is written by an autotuner that times the candidates on the
  * device (fiuncho_amd.tune); it is process-wide and starts empty.  Batched

@@ -34,7 +34,8 @@ def test_mfma3_vs_oracle(snps, cases, ctrls, outputs, seed):
     print(f"case: mfma3 {snps}x{cases}+{ctrls}", flush=True)
     d, od = synth(snps, cases, ctrls, seed)
     oc, om, _ = O.search(od, 3, outputs, threads=4, mi_mode=2)
-    for variant in (0, 3):   # default (4-stage ring), 3-stage ring
+    # default (64 x 64 tile, 4-stage ring), 64 x 32 with 3 and 4 stages, 64 x 64 with 3
+    for variant in (0, 3, 4, 5):
         s = fa.GpuSearch(algorithm=fa.ALGO_MFMA, variant=variant)
         c, m = s.run_arrays(d, 3, fa.Distribution(snps, 2), outputs)
         assert s.last_stats.algorithm == fa.ALGO_MFMA
@@ -80,16 +81,17 @@ def test_mfma3_distribution_and_chunks():
     assert s.last_stats.redo_chunks > 0
 
 
-def test_mfma3_launch_beyond_the_grid_limit():
-    # one chunk of 4,200 first SNPs needs 553 x 4,198 x 8 = 18.6M workgroups of
-    # 256 threads, above the 2^32 work-items a dispatch holds: the launch is
-    # split over first-SNP ranges, and the result equals the default chunks'
-    print("case: mfma3 4200 SNPs in one chunk", flush=True)
-    snps = 4200
+@pytest.mark.parametrize("snps,variant", [(4200, 4), (5300, 6)])
+def test_mfma3_launch_beyond_the_grid_limit(snps, variant):
+    # one chunk of all first SNPs needs more than 2^32 work-items (4,200 SNPs,
+    # 64 x 32 tiles: 553 x 4,198 x 8 = 18.6M workgroups of 256 threads;
+    # 5,300 SNPs, 64 x 64 tiles: 436 x 5,298 x 8 = 18.5M): the launch is split
+    # over first-SNP ranges, and the result equals the default chunks'
+    print(f"case: mfma3 {snps} SNPs in one chunk, variant {variant}", flush=True)
     d, _ = synth(snps, 64, 64, 91)
     dist = fa.Distribution(snps, 2)
-    ref = fa.GpuSearch(algorithm=fa.ALGO_MFMA).run_arrays(d, 3, dist, 50)
-    s = fa.GpuSearch(algorithm=fa.ALGO_MFMA, chunk_combos=1 << 40, sample_threshold=False)
+    ref = fa.GpuSearch(algorithm=fa.ALGO_MFMA, variant=variant).run_arrays(d, 3, dist, 50)
+    s = fa.GpuSearch(algorithm=fa.ALGO_MFMA, variant=variant, chunk_combos=1 << 40, sample_threshold=False)
     c, m = s.run_arrays(d, 3, dist, 50)
     assert s.last_stats.combinations == math.comb(snps, 3)
     assert c.tolist() == ref[0].tolist() and np.array_equal(m, ref[1])
