@@ -455,7 +455,7 @@ bool valid_variant(int algo, int order, int variant)
     switch (algo) {
     case FIU_ALGO_MARGINAL:
         return order <= 3 && (variant == 2 || variant == kMarginalPW || (order == 2 && variant == 8));
-    case FIU_ALGO_MFMA: return order == 3 ? (variant >= 3 && variant <= 6) : variant == 1;
+    case FIU_ALGO_MFMA: return order == 3 ? (variant >= 3 && variant <= 6) : (variant == 1 || variant == 2);
     default: return false;
     }
 }
@@ -920,6 +920,10 @@ void search(fiu_dataset *d, Workspace &ws, int order, int32_t step, int32_t offs
     const int64_t n_super = mfma2_superblocks(T);
     const bool int8 = (cfg && (cfg->flags & FIU_FLAG_MFMA_INT8)) || (order == 2 && variant == 1);
     const bool mfma_fp4 = algo == FIU_ALGO_MFMA && !int8 && small_classes && ws.fp4_image(d, s);
+    // order-2 matrix-core variants: 0 the 64 x 128 tile (default; 16-bit
+    // counts between the classes: classes below 2^16 samples, else 64 x 64),
+    // 1 int8 operands, 2 the 64 x 64 tile
+    const bool m2_wide = order == 2 && variant != 2 && d->cases < (1u << 16) && d->ctrls < (1u << 16);
 
     // chunk plan: even chunks of about `target` combinations (at most
     // max_chunks of them); before the first, a sampling pass over the last
@@ -1010,7 +1014,8 @@ void search(fiu_dataset *d, Workspace &ws, int order, int32_t step, int32_t offs
         }
         if (e0 && e1) check(hipEventRecord(e0, s), "event");
         if (algo == FIU_ALGO_MFMA && order == 2) {
-            check(launch_mfma2(d->dev, P, state, c.j0, c.j1, T, mfma_fp4 ? d->fp4.p : nullptr, s), "scan launch");
+            check(launch_mfma2(d->dev, P, state, c.j0, c.j1, T, mfma_fp4 ? d->fp4.p : nullptr, m2_wide, s),
+                  "scan launch");
         } else if (algo == FIU_ALGO_MFMA) {
             check(launch_mfma3(d->dev, P, state, d->fp4.p, (int)c.j0, (int)c.j1, T, ring, s), "scan launch");
         } else if (algo == FIU_ALGO_MARGINAL && order >= 4) {

@@ -30,12 +30,16 @@ int64_t mfma2_superblocks(int T);
 uint64_t fp4_image_units(const DevData &D, int cls);
 hipError_t launch_fp4_image(const DevData &D, uint4 *out, hipStream_t s);
 // Order 3 on the matrix cores (fp4): first SNPs [a0, a1), every (b, c) past
-// them, with the pair table in P.pair and the dataset's fp4 image; `ring`
-// LDS stages (3 or 4).
+// them, with the pair table in P.pair and the dataset's fp4 image; `ring`:
+// 3 or 4 LDS stages of 64 x 32 (b, c) tiles, 5 or 6 (3 or 4 stages) of 64 x
+// 64 tiles (classes below 2^16 samples).
 hipError_t launch_mfma3(const DevData &D, const ScanParams &P, TopState *st, const uint4 *fp4, int a0, int a1, int T,
                         int ring, hipStream_t s);
+// Order 2 on the matrix cores: super-blocks [s0, s1) of 8 x 8 tiles, fp4
+// operands from the image `fp4` (int8 when null); `wide`: 64 x 128 pairs per
+// workgroup (fp4, classes below 2^16 samples), else 64 x 64.
 hipError_t launch_mfma2(const DevData &D, const ScanParams &P, TopState *st, int64_t s0, int64_t s1, int T,
-                        const uint4 *fp4, hipStream_t s);
+                        const uint4 *fp4, bool wide, hipStream_t s);
 hipError_t launch_merge(TopState *st, Cand *top, const Cand *cand, uint32_t N, uint32_t cap, uint32_t chunk_id,
                         uint32_t *redo, hipStream_t s);
 // Top-N merge for N above kMaxOutputs, in global memory: the list's n_top

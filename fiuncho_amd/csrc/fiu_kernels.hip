@@ -1457,6 +1457,17 @@ k_scan3m(const DevData D, const ScanParams P, TopState *st)
 //     accumulation of 0/1 products -- exact while a count stays below 2^24
 //     (checked on the host); half the bytes per product and twice the rate.
 // ---------------------------------------------------------------------------
+// f(integral_constant<int, 0>) ... f(integral_constant<int, N - 1>), inline.
+template <class F, int... I>
+__device__ __forceinline__ void static_for_impl(F &&f, std::integer_sequence<int, I...>)
+{
+    (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, class F> __device__ __forceinline__ void static_for(F &&f)
+{
+    static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
+
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v8i __attribute__((ext_vector_type(8)));
 typedef int v16i __attribute__((ext_vector_type(16)));
@@ -1807,6 +1818,232 @@ k_mfma2(const DevData D, const ScanParams P, TopState *st, const Fp4Image F4, in
     mfma2_tile<FP4>(D, P, st, F4, su, (int)(k & 63), T, lds);
 }
 
+// Wide order-2 tile: one workgroup takes 64 a x 128 b (the b tiles 2j and 2j
+// + 1 of a super-block row); wave w multiplies all 64 a rows by the 32 b rows
+// b0 + 32 w ...: 16 products per 128-sample stage against 12 fragment reads
+// (8 per 8 in mfma2_tile), and 24 KB staged per 64 products (16 KB per 32).
+// One class's counts fill the 128 accumulators; both classes wait as 16-bit
+// pairs in private memory (every class holds fewer than 2^16 samples), the
+// epilogue scores them in two passes as mfma3w_tile does.  Stage layout: 384
+// rows of 64 bytes, rows g * 64 + i the a rows (genotype g, SNP a0 + i), rows
+// 128 + h * 128 + j the b rows; segments swizzled as in mfma2_tile.
+constexpr int kM2wRing = 3;
+constexpr int kM2wStage = 384 * 64;
+
+// Counts of instance row block (ab, r) as 16-bit pairs: word (g * 2 + h) * 2
+// + q / 2 holds f(g, h) of row 8 r + 4 (lane / 32) + q of a block ab, in the
+// low (q even) or high half.
+__device__ __forceinline__ v4i pack_counts2(const v16f (&acc)[2][2][2], int ab, int r, int half)
+{
+    auto pk = [](float lo, float hi) __attribute__((always_inline)) {
+        return (int)((uint32_t)lo | (uint32_t)hi << 16);
+    };
+    const int g = half;   // words 4 half .. 4 half + 3: g = half, h = 0, 1
+    const v16f &v0 = acc[ab][g][0], &v1 = acc[ab][g][1];
+    return (v4i){pk(v0[r * 4 + 0], v0[r * 4 + 1]), pk(v0[r * 4 + 2], v0[r * 4 + 3]), pk(v1[r * 4 + 0], v1[r * 4 + 1]),
+                 pk(v1[r * 4 + 2], v1[r * 4 + 3])};
+}
+
+__device__ __forceinline__ void mfma2w_tile(const DevData &D, const ScanParams &P, TopState *st, const Fp4Image &F4,
+                                            int64_t su, int slot, int T, int8_t (*lds)[kM2wStage])
+{
+    const int S = (T + kMfmaSuper - 1) / kMfmaSuper;
+    int sa, sc;
+    tri_tile(su, S, sa, sc);
+    const int ta = sa * kMfmaSuper + (slot >> 2), tb = sc * kMfmaSuper + 2 * (slot & 3);
+    if (ta > tb + 1 || ta >= T || tb >= T) return;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int np = D.n_pad;
+    const int a0 = ta * kMfmaTile, b0 = tb * kMfmaTile;
+    // staging: wave w loads the 16-row groups 6 w .. 6 w + 5 of the 24 (group
+    // G: rows 16 G ..); lane l fills row l / 4 of the group, stored segment l
+    // % 4, so it loads the row's segment (l % 4) ^ m2_swz(row).  Offsets
+    // within a quad of the image (b rows past the padded SNPs, on an odd
+    // last tile, read zeros: the range check of the buffer)
+    const int lr = lane >> 2, sg = (lane & 3) ^ m2_swz(lr);
+    uint32_t vo[6];
+#pragma unroll
+    for (int t = 0; t < 6; ++t) {
+        const int G = 6 * w + t;
+        const int gen = G < 8 ? G >> 2 : (G - 8) >> 3;
+        const int snp = G < 8 ? a0 + 16 * (G & 3) : b0 + 16 * ((G - 8) & 7);
+        vo[t] = 16u * (uint32_t)(4 * (gen * np + snp + lr) + sg);
+    }
+    v16f acc[2][2][2];   // [a block][g][h], the class being scanned
+    v4i stash_mem[32];   // pack_counts2 words: class c, block (ab, r), half -> 16 c + 2 (4 ab + r) + half
+    __attribute__((address_space(5))) v4i *stash = (__attribute__((address_space(5))) v4i *)stash_mem;
+    asm volatile("" : "+v"(stash));
+    const int r32 = lane & 31;
+#pragma unroll
+    for (int cls = 0; cls < 2; ++cls) {
+#pragma unroll
+        for (int ab = 0; ab < 2; ++ab)
+#pragma unroll
+            for (int g = 0; g < 2; ++g)
+#pragma unroll
+                for (int h = 0; h < 2; ++h) acc[ab][g][h] = (v16f){0};
+        const int iters = D.quads[cls];
+        const uint4 *img = F4.base + F4.class_off[cls];   // quad q at + q * 8 * np
+        auto issue = [&](int q) __attribute__((always_inline)) {
+            const __amdgpu_buffer_rsrc_t rq =
+                __builtin_amdgcn_make_buffer_rsrc((void *)(img + (int64_t)q * 8 * np), (short)0, 128 * np, 0x00020000);
+            int8_t *s = lds[q % kM2wRing];
+#pragma unroll
+            for (int t = 0; t < 6; ++t) dma16(rq, vo[t], s + (6 * w + t) * 1024);
+        };
+#pragma unroll
+        for (int q = 0; q < kM2wRing - 1; ++q)
+            if (q < iters) issue(q);
+        auto stage = [&](int it, bool steady) __attribute__((always_inline)) {
+            // 6 loads per stage; stage it is complete once at most those of
+            // the later stages issued are pending, the barrier publishes every
+            // wave's part
+            static_assert(6 * (kM2wRing - 2) < 16, "ring depth");
+            if (steady) wait_vmcnt<6 * (kM2wRing - 2)>();
+            else wait_stages<kM2wRing - 2, 6>(iters - 1 - it);
+            stage_barrier();
+            __builtin_amdgcn_s_setprio(1);
+            const int8_t *sl = lds[it % kM2wRing];
+            // substep s (64 samples): the b fragments of genotypes 0 and 1
+            // and the a fragments (g, a block); reads of substep 1 issue
+            // between the products of substep 0 (12 reads in a row overfill
+            // the LDS command queue)
+            v4i fa[2][2][2], fb[2][2];   // [s][g][a block], [s][h]
+            auto rd = [&](int s, int row0) __attribute__((always_inline)) {
+                return *(const v4i *)(sl + (row0 + r32) * 64 + (((2 * s + (lane >> 5)) ^ m2_swz(r32)) << 4));
+            };
+#pragma unroll
+            for (int h = 0; h < 2; ++h) fb[0][h] = rd(0, 128 + h * 128 + 32 * w);
+#pragma unroll
+            for (int g = 0; g < 2; ++g)
+#pragma unroll
+                for (int ab = 0; ab < 2; ++ab) fa[0][g][ab] = rd(0, g * 64 + 32 * ab);
+            if (steady || it + kM2wRing - 1 < iters) issue(it + kM2wRing - 1);
+            __builtin_amdgcn_sched_barrier(0);
+            auto product = [&](int s, int ab, int g, int h) __attribute__((always_inline)) {
+                const v8i a8 = {fa[s][g][ab][0], fa[s][g][ab][1], fa[s][g][ab][2], fa[s][g][ab][3], 0, 0, 0, 0};
+                const v8i b8 = {fb[s][h][0], fb[s][h][1], fb[s][h][2], fb[s][h][3], 0, 0, 0, 0};
+                // formats 4/4 = e2m1; scale operands 0 select the unscaled product
+                acc[ab][g][h] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a8, b8, acc[ab][g][h], 4, 4, 0, 0, 0, 0);
+            };
+            // substep 0's 8 products, each pair with one read of substep 1
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int g = k >> 1, ab = k & 1;
+                if (k < 2) fb[1][k] = rd(1, 128 + k * 128 + 32 * w);
+                fa[1][k >> 1][k & 1] = rd(1, (k >> 1) * 64 + 32 * (k & 1));
+                product(0, ab, g, 0);
+                product(0, ab, g, 1);
+                __builtin_amdgcn_sched_barrier(0);
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                product(1, k & 1, k >> 1, 0);
+                product(1, k & 1, k >> 1, 1);
+            }
+            __builtin_amdgcn_s_setprio(0);
+        };
+        int it = 0;
+        for (; it + kM2wRing - 1 < iters; ++it) stage(it, true);
+        for (; it < iters; ++it) stage(it, false);
+        // the next class's first loads reuse the ring
+        stage_barrier();
+#pragma unroll
+        for (int ab = 0; ab < 2; ++ab)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+#pragma unroll
+                for (int half = 0; half < 2; ++half)
+                    stash[16 * cls + 2 * (4 * ab + r) + half] = pack_counts2(acc, ab, r, half);
+    }
+    // epilogue: lane holds column b = b0 + 32 w + lane % 32 and, per a block
+    // ab, rows a = a0 + 32 ab + 8 r + 4 (lane / 32) + q, r, q < 4.  First
+    // pass: the bound of each of the 32 instances (ab, r, q), the passing
+    // ones marked in `pend` (bit 16 ab + 4 r + q); second pass: one marked
+    // instance per lane per round, counts read back from the stash
+    const float tau = st->tau;
+    const uint64_t tau_combo = st->tau_combo;
+    const int b = b0 + 32 * w + r32;
+    const bool b_ok = b < D.n;
+    uint32_t sb[2][2] = {{0, 0}, {0, 0}};
+    if (b_ok) {
+        const uint32_t *sp = D.singles + (size_t)b * 6;
+        sb[0][0] = sp[0], sb[0][1] = sp[1], sb[1][0] = sp[3], sb[1][1] = sp[4];
+    }
+    // the 9 cells of pair (a, b) per class from the counts cnt(cls, g, h)
+    auto cells = [&](int a, auto &&cnt, float (&m0)[9], float (&m1)[9]) __attribute__((always_inline)) {
+        const int64_t rel = (int64_t)a - P.offset;
+        const bool valid = b_ok && a < b && rel >= 0 && (P.step == 1 || rel % P.step == 0);
+        const uint32_t *sa = D.singles + (size_t)a * 6;
+#pragma unroll
+        for (int cls = 0; cls < 2; ++cls) {
+            float *m = cls ? m1 : m0;
+            m[0] = cnt(cls, 0, 0);
+            m[1] = cnt(cls, 0, 1);
+            m[3] = cnt(cls, 1, 0);
+            m[4] = cnt(cls, 1, 1);
+            m[2] = (float)sa[cls * 3 + 0];
+            m[5] = (float)sa[cls * 3 + 1];
+            m[6] = (float)sb[cls][0];
+            m[7] = (float)sb[cls][1];
+            m[8] = (float)D.total[cls];
+            mobius2(m);
+        }
+        return valid;
+    };
+    uint32_t pend = 0;
+    static_for<8>([&](auto ABR) __attribute__((always_inline)) {
+        constexpr int ab = decltype(ABR)::value >> 2, r = decltype(ABR)::value & 3;
+        const v4i lo0 = stash[2 * (4 * ab + r)], lo1 = stash[2 * (4 * ab + r) + 1];
+        static_for<4>([&](auto Q) __attribute__((always_inline)) {
+            constexpr int q = decltype(Q)::value;
+            float m0[9], m1[9];
+            const bool valid = cells(
+                a0 + 32 * ab + 8 * r + 4 * (lane >> 5) + q,
+                [&](int cls, int g, int h) __attribute__((always_inline)) {
+                    if (cls) return acc[ab][g][h][r * 4 + q];
+                    const v4i &lo = g ? lo1 : lo0;
+                    return (float)((uint32_t)lo[h * 2 + (q >> 1)] >> (16 * (q & 1)) & 0xffffu);
+                },
+                m0, m1);
+            if (valid && bound_ok<9>(m0, m1, tau, P)) pend |= 1u << (16 * ab + 4 * r + q);
+        });
+    });
+    const __attribute__((address_space(5))) uint32_t *sw = (const __attribute__((address_space(5))) uint32_t *)stash;
+    while (__ballot(pend != 0)) {
+        bool want = false;
+        float mf = 0.0f;
+        uint64_t combo = ~0ull;
+        if (pend) {
+            const int i = __builtin_ctz(pend), ab = i >> 4, r = (i >> 2) & 3, q = i & 3;
+            pend &= pend - 1;
+            const int a = a0 + 32 * ab + 8 * r + 4 * (lane >> 5) + q;
+            float m0[9], m1[9];
+            cells(
+                a,
+                [&](int cls, int g, int h) __attribute__((always_inline)) {
+                    const uint32_t v = sw[4 * (16 * cls + 2 * (4 * ab + r) + g) + h * 2 + (q >> 1)];
+                    return (float)(v >> (16 * (q & 1)) & 0xffffu);
+                },
+                m0, m1);
+            screen_exact<9>(m0, m1, [&] { return prefix_key(P.binom, D.n, &a, 1) + (uint64_t)b; }, tau, tau_combo, P,
+                            want, mf, combo);
+        }
+        push_candidates(want, mf, combo, P, st);
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 2)))
+k_mfma2w(const DevData D, const ScanParams P, TopState *st, const Fp4Image F4, int64_t s0, int64_t s1, int T)
+{
+    __shared__ __attribute__((aligned(1024))) int8_t lds[kM2wRing][kM2wStage];
+    // block order: XCD v % 8 takes super-blocks s0 + 8 i + v % 8, 32 blocks each
+    const int64_t k = blockIdx.x >> 3, su = s0 + (k >> 5) * 8 + (blockIdx.x & 7);
+    if (su >= s1) return;
+    mfma2w_tile(D, P, st, F4, su, (int)(k & 31), T, lds);
+}
+
 // ---------------------------------------------------------------------------
 // Order 3 on the matrix cores.  For a fixed first SNP a, the eight counts
 // f(x, y, z), x, y, z in {0, 1}, of the triples (a, b, c) are sums over the
@@ -1822,16 +2059,6 @@ k_mfma2(const DevData D, const ScanParams P, TopState *st, const Fp4Image F4, in
 // ---------------------------------------------------------------------------
 typedef float v4f __attribute__((ext_vector_type(4)));
 
-// f(integral_constant<int, 0>) ... f(integral_constant<int, N - 1>), inline.
-template <class F, int... I>
-__device__ __forceinline__ void static_for_impl(F &&f, std::integer_sequence<int, I...>)
-{
-    (f(std::integral_constant<int, I>{}), ...);
-}
-template <int N, class F> __device__ __forceinline__ void static_for(F &&f)
-{
-    static_for_impl(f, std::make_integer_sequence<int, N>{});
-}
 // A stage (one 128-sample quad of one class) in LDS: 13 groups of 16 rows x
 // 64 bytes -- groups 0..7 the b rows (y, b), 8..11 the c rows (z, c), 12 the
 // two a rows (x).  Segment s (16 bytes, 32 samples) of row r of a group is
@@ -3200,26 +3427,29 @@ int64_t mfma2_superblocks(int T)
 }
 
 hipError_t launch_mfma2(const DevData &D, const ScanParams &P, TopState *st, int64_t s0, int64_t s1, int T,
-                        const uint4 *fp4, hipStream_t s)
+                        const uint4 *fp4, bool wide, hipStream_t s)
 {
     if (s1 <= s0) return hipSuccess;
+    wide = wide && fp4;
+    const int per = wide ? 32 : 64;   // blocks per super-block
     // a grid below 2^32 work-items: sub-launches of at most this many super-blocks
-    constexpr int64_t kMaxSuper = kMaxGridBlocks / (8 * 64) * 8;
-    if (s1 - s0 > kMaxSuper) {
-        for (int64_t a = s0; a < s1; a += kMaxSuper) {
-            const hipError_t e = launch_mfma2(D, P, st, a, std::min(s1, a + kMaxSuper), T, fp4, s);
+    const int64_t max_super = kMaxGridBlocks / (8 * per) * 8;
+    if (s1 - s0 > max_super) {
+        for (int64_t a = s0; a < s1; a += max_super) {
+            const hipError_t e = launch_mfma2(D, P, st, a, std::min(s1, a + max_super), T, fp4, wide, s);
             if (e != hipSuccess) return e;
         }
         return hipSuccess;
     }
-    // 64 blocks per super-block, 8 super-blocks (one per XCD) per round
+    // `per` blocks per super-block, 8 super-blocks (one per XCD) per round
     const int64_t rounds = (s1 - s0 + 7) / 8;
-    const dim3 grid((unsigned)(rounds * 8 * 64)), block(kBlock);
+    const dim3 grid((unsigned)(rounds * 8 * per)), block(kBlock);
     Fp4Image F;
     F.base = fp4;
     F.class_off[0] = 0;
     F.class_off[1] = fp4_image_units(D, 0);
-    if (fp4) hipLaunchKernelGGL((k_mfma2<true>), grid, block, 0, s, D, P, st, F, s0, s1, T);
+    if (wide) hipLaunchKernelGGL(k_mfma2w, grid, block, 0, s, D, P, st, F, s0, s1, T);
+    else if (fp4) hipLaunchKernelGGL((k_mfma2<true>), grid, block, 0, s, D, P, st, F, s0, s1, T);
     else hipLaunchKernelGGL((k_mfma2<false>), grid, block, 0, s, D, P, st, F, s0, s1, T);
     return hipGetLastError();
 }

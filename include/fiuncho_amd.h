@@ -293,7 +293,10 @@ fiu_status fiu_write_results(const char *path, const int32_t *combos,
  *                      64 x 64 with 3 or 4 stages (6 is the default; 5 and 6
  *                      fall back to 4 when a class holds 2^16 samples or
  *                      more);
- *   FIU_ALGO_MFMA, order 2: 0 = fp4 operands (default), 1 = int8 operands.
+ *   FIU_ALGO_MFMA, order 2: 0 = fp4 operands, 64 x 128 pairs per
+ *                      workgroup (default; 64 x 64 when a class holds 2^16
+ *                      samples or more), 1 = int8 operands, 2 = fp4
+ *                      operands, 64 x 64 pairs per workgroup.
  * The table is written by an autotuner that times the candidates on the
  * device (fiuncho_amd.tune); it is process-wide and starts empty.  Batched
  * scans use its algorithm and, for the popcount kinds (orders 2-5 off the

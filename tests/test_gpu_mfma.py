@@ -2,8 +2,9 @@
 
 The marginal pair counts come from indicator dot products on the matrix cores
 -- int8 operands with int32 accumulation, or fp4 (e2m1) operands with fp32
-accumulation, both exact at these sizes; everything downstream is the popcount
-scan's.  Bar: the same
+accumulation, both exact at these sizes; fp4 tiles of 64 x 128 pairs per
+workgroup (counts kept as 16-bit pairs between the classes, two-pass
+epilogue) or 64 x 64; everything downstream is the popcount scan's.  Bar: the same
 top-N (combinations, order and float MI bit for bit) as the oracle's FP64
 ranking and as the popcount scan.
 """
@@ -20,18 +21,22 @@ from .test_gpu_parity import synth
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("int8", [False, True])
+# variant 0: fp4 operands, 64 x 128 pairs per workgroup (the default); 1: int8
+# operands; 2: fp4 operands, 64 x 64 pairs per workgroup
+@pytest.mark.parametrize("variant", [0, 1, 2])
 @pytest.mark.parametrize("snps,cases,ctrls,outputs,seed", [
     (70, 129, 63, 40, 21),       # one partial tile, ragged classes
     (128, 512, 512, 100, 22),    # two full tiles per side
     (200, 1000, 777, 300, 26),   # several tiles, K of 8 and 7 stages
     (65, 40, 2000, 2080, 27),    # every pair, one SNP in the last tile
+    (150, 300, 280, 150, 28),    # 3 tiles: the last 128-SNP column half past the SNPs
+    (530, 700, 650, 500, 29),    # 9 tiles: two super-blocks per side, odd last column
 ])
-def test_mfma_vs_oracle(snps, cases, ctrls, outputs, seed, int8):
-    print(f"case: mfma {snps}x{cases}+{ctrls} int8={int8}", flush=True)
+def test_mfma_vs_oracle(snps, cases, ctrls, outputs, seed, variant):
+    print(f"case: mfma {snps}x{cases}+{ctrls} variant={variant}", flush=True)
     d, od = synth(snps, cases, ctrls, seed)
     oc, om, _ = O.search(od, 2, outputs, threads=4, mi_mode=2)
-    s = fa.GpuSearch(algorithm=fa.ALGO_MFMA, mfma_int8=int8)
+    s = fa.GpuSearch(algorithm=fa.ALGO_MFMA, variant=variant)
     c, m = s.run_arrays(d, 2, fa.Distribution(snps, 1), outputs)
     assert s.last_stats.algorithm == fa.ALGO_MFMA
     assert c.tolist() == oc.tolist() and np.array_equal(m, om)
@@ -42,8 +47,9 @@ def test_mfma_matches_popcount_scan():
     print("case: mfma vs popcount 1500 SNPs", flush=True)
     d, _ = synth(1500, 3000, 2500, 31, planted=(100, 1200))
     ref = fa.GpuSearch(algorithm=fa.ALGO_MARGINAL).run_arrays(d, 2, fa.Distribution(1500, 1), 500)
-    for int8 in (False, True):
-        c, m = fa.GpuSearch(algorithm=fa.ALGO_MFMA, mfma_int8=int8).run_arrays(d, 2, fa.Distribution(1500, 1), 500)
+    for int8, variant in ((False, 0), (True, 0), (False, 2)):
+        c, m = fa.GpuSearch(algorithm=fa.ALGO_MFMA, mfma_int8=int8, variant=variant).run_arrays(
+            d, 2, fa.Distribution(1500, 1), 500)
         assert c.tolist() == ref[0].tolist() and np.array_equal(m, ref[1])
         assert c[0].tolist() == [100, 1200]
 
