@@ -720,6 +720,10 @@ class Workspace {
     virtual bool fp4_image(fiu_dataset *d, hipStream_t s) = 0;
     virtual uint32_t max_candidates() const = 0;
     virtual uint64_t max_chunks() const = 0;
+    // whether kernels that keep counts in per-lane private memory (the 64 x
+    // 64 order-3 and 64 x 128 order-2 tiles) may run: the runtime backs that
+    // memory at their launch
+    virtual bool private_memory() const { return true; }
     // candidate buffer (cap entries), top list (max(outputs, kMaxOutputs)),
     // state, redo list (chunks entries), histogram (kHistBins)
     virtual void buffers(fiu_dataset *d, uint32_t cap, uint64_t chunks, uint32_t outputs, Cand *&cand, Cand *&top,
@@ -912,7 +916,7 @@ void search(fiu_dataset *d, Workspace &ws, int order, int32_t step, int32_t offs
     // and 4 stages, 5 and 6 the 64 x 64 tile (default; its counts wait as
     // 16-bit pairs: classes below 2^16 samples, else the 64 x 32 tile)
     int ring = algo == FIU_ALGO_MFMA && order == 3 && variant ? variant : 6;
-    if (ring >= 5 && (d->cases >= (1u << 16) || d->ctrls >= (1u << 16))) ring = 4;
+    if (ring >= 5 && (d->cases >= (1u << 16) || d->ctrls >= (1u << 16) || !ws.private_memory())) ring = 4;
     // matrix-core scans walk super-blocks of 8 x 8 tiles of 64 x 64 pairs;
     // fp4 operands accumulate in fp32, exact while every count is below 2^24;
     // they come from the dataset's fp4 image (int8 when it is not there)
@@ -923,7 +927,8 @@ void search(fiu_dataset *d, Workspace &ws, int order, int32_t step, int32_t offs
     // order-2 matrix-core variants: 0 the 64 x 128 tile (default; 16-bit
     // counts between the classes: classes below 2^16 samples, else 64 x 64),
     // 1 int8 operands, 2 the 64 x 64 tile
-    const bool m2_wide = order == 2 && variant != 2 && d->cases < (1u << 16) && d->ctrls < (1u << 16);
+    const bool m2_wide =
+        order == 2 && variant != 2 && d->cases < (1u << 16) && d->ctrls < (1u << 16) && ws.private_memory();
 
     // chunk plan: even chunks of about `target` combinations (at most
     // max_chunks of them); before the first, a sampling pass over the last
@@ -1173,6 +1178,9 @@ class StaticWorkspace final : public Workspace {
     }
     uint32_t max_candidates() const override { return (uint32_t)kStaticCand; }
     uint64_t max_chunks() const override { return kStaticChunks; }
+    // the tiles without private memory: no launch of this search needs more
+    // than the fixed block and the caller's arrays
+    bool private_memory() const override { return false; }
     void buffers(fiu_dataset *, uint32_t cap, uint64_t chunks, uint32_t outputs, Cand *&cand, Cand *&top,
                  TopState *&state, uint32_t *&redo, uint32_t *&hist) override
     {

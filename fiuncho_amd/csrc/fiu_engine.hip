@@ -362,7 +362,10 @@ void Engine::plan_job(Job *j)
     const bool has_te = tuned(order, d->snps, (uint64_t)d->cases + d->ctrls, te);
     const int tuned_algo = has_te ? te.algorithm : FIU_ALGO_AUTO;
     const double unit_instr = has_te && te.batch_unit ? 40000.0 * std::ldexp(1.0, (int)te.batch_unit - 3) : 40000.0;
+    // (64 x 128 / 64 x 64 tiles, their counts 16-bit between the classes:
+    // classes below 2^16 samples)
     if ((tuned_algo == FIU_ALGO_AUTO || tuned_algo == FIU_ALGO_MFMA) && mfma_dataset(d) && d->fp4_ready &&
+        d->cases < (1u << 16) && d->ctrls < (1u << 16) &&
         (order == 2 || (order == 3 && d->pair_ready && j->step == 1))) {
         j->kind = order == 2 ? kKindMfma2 : kKindMfma3;
         j->pw = 1;
@@ -574,9 +577,9 @@ bool Engine::launch_step()
             uint64_t u;
             if (j->kind == kKindMfma3) {
                 const int64_t m = T - (pr.first + 1) / 64;
-                u = (uint64_t)((m * (m + 1) + 7) / 8) * (uint64_t)(pr.second - pr.first);
+                u = (uint64_t)((m * (m + 1) / 2 + 7) / 8) * (uint64_t)(pr.second - pr.first);
             } else {
-                u = (uint64_t)((pr.second - pr.first + 7) / 8) * 64;
+                u = (uint64_t)((pr.second - pr.first + 7) / 8) * 32;
             }
             total += u;
         }
@@ -585,9 +588,9 @@ bool Engine::launch_step()
             uint64_t u;
             if (j->kind == kKindMfma3) {
                 const int64_t m = T - (pr.first + 1) / 64;
-                u = (uint64_t)((m * (m + 1) + 7) / 8) * (uint64_t)(pr.second - pr.first);
+                u = (uint64_t)((m * (m + 1) / 2 + 7) / 8) * (uint64_t)(pr.second - pr.first);
             } else {
-                u = (uint64_t)((pr.second - pr.first + 7) / 8) * 64;
+                u = (uint64_t)((pr.second - pr.first + 7) / 8) * 32;
             }
             BatchSeg &sg = msegs[nmseg++];
             sg.start = munits;

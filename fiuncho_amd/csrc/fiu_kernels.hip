@@ -2976,13 +2976,15 @@ k_batch_scan(const BatchJob *__restrict__ jobs, const BatchSeg *__restrict__ seg
 // takes units from queue b % 8; every queue has 1/8 of the workgroups, so all
 // are drained wherever they run).  Queue x's unit v lies in the segment whose
 // start is the last <= v; with l = v - start, order 3: first SNP j0 + l % na,
-// tile 8 (l / na) + x (the XCD-ordered work of k_mfma3); order 2: super-block
-// j0 + 8 (l / 64) + x, tile l % 64 (that of k_mfma2).
+// 64 x 64 tile 8 (l / na) + x (the XCD-ordered work of k_mfma3w); order 2:
+// super-block j0 + 8 (l / 32) + x, 64 x 128 tile l % 32 (that of k_mfma2w).
+// Every dataset of these kinds has classes below 2^16 samples.
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 2)))
 k_batch_mfma(const BatchJob *__restrict__ jobs, const BatchSeg *__restrict__ segs, BatchHeader *hdr)
 {
-    constexpr size_t kBytes = sizeof(Mfma2Lds<true>) > (size_t)kM3Ring * kM3Stage ? sizeof(Mfma2Lds<true>)
-                                                                                  : (size_t)kM3Ring * kM3Stage;
+    constexpr size_t kBytes = (size_t)kM2wRing * kM2wStage > (size_t)kM3Ring * kM3wStage
+                                  ? (size_t)kM2wRing * kM2wStage
+                                  : (size_t)kM3Ring * kM3wStage;
     __shared__ __attribute__((aligned(1024))) int8_t lds[kBytes];   // the order-2 or the order-3 tiles
     __shared__ uint32_t unit_sh;
     const int x = blockIdx.x & 7;
@@ -3008,11 +3010,11 @@ k_batch_mfma(const BatchJob *__restrict__ jobs, const BatchSeg *__restrict__ seg
         const int64_t j0 = segs[lo].j0, j1 = segs[lo].j1;
         if (J.kind == kKindMfma3) {
             const uint32_t na = (uint32_t)(j1 - j0);
-            mfma3_tile<kM3Ring>(D, P, st, F, (int)(j0 + l % na), (int64_t)(l / na) * 8 + x, J.T,
-                       (int8_t(*)[kM3Stage])lds);
+            mfma3w_tile<kM3Ring>(D, P, st, F, (int)(j0 + l % na), (int64_t)(l / na) * 8 + x, J.T,
+                                 (int8_t(*)[kM3wStage])lds);
         } else {
-            const int64_t su = j0 + (int64_t)(l / 64) * 8 + x;
-            if (su < j1) mfma2_tile<true>(D, P, st, F, su, (int)(l % 64), J.T, *(Mfma2Lds<true> *)lds);
+            const int64_t su = j0 + (int64_t)(l / 32) * 8 + x;
+            if (su < j1) mfma2w_tile(D, P, st, F, su, (int)(l % 32), J.T, (int8_t(*)[kM2wStage])lds);
         }
     }
 }

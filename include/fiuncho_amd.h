@@ -198,7 +198,10 @@ fiu_status fiu_search(const fiu_dataset *ds, uint16_t order,
  * up to 4,096; 2^14 scan launches (chunks grow to fit).  The dataset's pair
  * table and fp4 image are used when an earlier fiu_search built them;
  * otherwise the kernels that need neither run
- * (same results).  stats->scan_ms is 0 (no timing events).  Every call on one
+ * (same results).  The matrix-core scans run their tiles that keep no counts
+ * in per-lane private memory (order 3: variants 3 / 4, order 2: the 64 x 64
+ * tile), so no launch needs scratch memory from the runtime (same results).
+ * stats->scan_ms is 0 (no timing events).  Every call on one
  * GPU works in that GPU's block, which keeps the last call's contents (each
  * call clears what must start at zero): calls that overlap on one GPU race,
  * and must be serialised by the caller. */
