@@ -2373,6 +2373,10 @@ k_mfma3(const DevData D, const ScanParams P, TopState *st, const Fp4Image F, int
 // the b rows (y, 16-row block bb), 8..15 the c rows (z, block), 16 the a
 // rows, each stored as in mfma3_tile.
 constexpr int kM3wStage = 17 * kM3Group;
+// Words of the first class's 16 packed count words that stay in registers
+// through the second class's stages (the rest wait in private memory, whose
+// lines the operand stream evicts from L2).
+constexpr int kM3wKeep = 4;
 
 // Counts of b block bb = i / 4, (x, y) = (i / 2 % 2, i % 2) as 16-bit pairs:
 // word z * 2 + j / 2 holds f(x, y, z) of rows 4 (lane / 16) + j, in the low
@@ -2411,14 +2415,16 @@ __device__ __forceinline__ void mfma3w_tile(const DevData &D, const ScanParams &
     const bool a_lane = (lane >> 1) == w;
     const uint32_t ua = 16u * (uint32_t)(4 * ((lr & 1) * np + a) + ls);
     v4f acc[2][2][4][2];   // [x][y][b block][z], the class being scanned
-    // the counts (pack_counts; 16 words per class) in private memory: the
-    // first class's out of the registers the second class needs, both for
-    // the epilogue's loop over the b blocks.  The pointer passes through an empty
-    // asm statement, so the compiler keeps plain 16-byte private stores and
-    // loads (a volatile access would be uncached and waited for one by one)
-    v4i stash_mem[32];
+    // the first class's counts (pack_counts, 16 words) in private memory,
+    // out of the registers the second class needs; `marked`: both classes'
+    // counts of the instances the epilogue's first pass marks.  The pointers
+    // pass through an empty asm statement, so the compiler keeps plain
+    // 16-byte private stores and loads (a volatile access would be uncached
+    // and waited for one by one)
+    v4i stash_mem[16], marked_mem[32], keep[kM3wKeep];
     __attribute__((address_space(5))) v4i *stash = (__attribute__((address_space(5))) v4i *)stash_mem;
-    asm volatile("" : "+v"(stash));
+    __attribute__((address_space(5))) v4i *marked = (__attribute__((address_space(5))) v4i *)marked_mem;
+    asm volatile("" : "+v"(stash), "+v"(marked));
     const int rr = lane & 15, kc = lane >> 4;
     const int fo = rr * 64 + ((kc ^ m3_swz(rr)) << 4), ao = kc << 4;
 #pragma unroll
@@ -2509,8 +2515,13 @@ __device__ __forceinline__ void mfma3w_tile(const DevData &D, const ScanParams &
         for (; it + R - 1 < iters; ++it) stage(it, true);
         for (; it < iters; ++it) stage(it, false);
         stage_barrier();
+        if (cls == 0) {
 #pragma unroll
-        for (int i = 0; i < 16; ++i) stash[16 * cls + i] = pack_counts(acc, i);
+            for (int i = 0; i < 16; ++i) {
+                if (i < kM3wKeep) keep[i] = pack_counts(acc, i);
+                else stash[i] = pack_counts(acc, i);
+            }
+        }
     }
     // epilogue: lane holds column c = c0 + 16 w + lane % 16 and, per b block
     // bb, rows b = b0 + 16 bb + 4 (lane / 16) + j, j < 4
@@ -2518,18 +2529,20 @@ __device__ __forceinline__ void mfma3w_tile(const DevData &D, const ScanParams &
     const uint64_t tau_combo = st->tau_combo;
     const M3Lane L = m3_lane(D, P, a, c0 + 16 * w + rr);
     // first pass: the bound of each of the 16 instances (bb, j); a lane's
-    // instances that pass are marked in `pend` (bit 4 bb + j).  Second pass:
-    // each round scores one marked instance per lane, its counts read back
-    // from the stash, so the screening MI (54 logarithms) runs about as often
-    // as the most marked lane has marks, not once per instance in which any
-    // lane of the wave passes the bound (most instances, at a top-100
-    // threshold)
+    // instances that pass are marked in `pend` (bit 4 bb + j), their counts
+    // (f(x, y, z) of class 0 | class 1 << 16, word (x * 2 + y) * 2 + z) kept
+    // in `marked` in the same order.  Second pass: each round scores one
+    // marked instance per lane, so the screening MI (54 logarithms) runs
+    // about as often as the most marked lane has marks, not once per
+    // instance in which any lane of the wave passes the bound (most
+    // instances, at a top-100 threshold)
     uint32_t pend = 0;
+    int nmarked = 0;
     static_for<4>([&](auto BB) __attribute__((always_inline)) {
         constexpr int bb = decltype(BB)::value;
         v4i lo[4];
 #pragma unroll
-        for (int xy = 0; xy < 4; ++xy) lo[xy] = stash[4 * bb + xy];
+        for (int xy = 0; xy < 4; ++xy) lo[xy] = 4 * bb + xy < kM3wKeep ? keep[4 * bb + xy] : stash[4 * bb + xy];
         static_for<4>([&](auto J) __attribute__((always_inline)) {
             constexpr int j = decltype(J)::value;
             float m0[27], m1[27];
@@ -2540,10 +2553,25 @@ __device__ __forceinline__ void mfma3w_tile(const DevData &D, const ScanParams &
                                : (float)((uint32_t)lo[x * 2 + y][z * 2 + (j >> 1)] >> (16 * (j & 1)) & 0xffffu);
                 },
                 m0, m1);
-            if (valid && bound_ok<27>(m0, m1, tau, P)) pend |= 1u << (4 * bb + j);
+            if (valid && bound_ok<27>(m0, m1, tau, P)) {
+                pend |= 1u << (4 * bb + j);
+                int wd[8];
+#pragma unroll
+                for (int x = 0; x < 2; ++x)
+#pragma unroll
+                    for (int y = 0; y < 2; ++y)
+#pragma unroll
+                        for (int z = 0; z < 2; ++z)
+                            wd[(x * 2 + y) * 2 + z] =
+                                (int)((uint32_t)lo[x * 2 + y][z * 2 + (j >> 1)] >> (16 * (j & 1)) & 0xffffu |
+                                      (uint32_t)acc[x][y][bb][z][j] << 16);
+                marked[2 * nmarked] = (v4i){wd[0], wd[1], wd[2], wd[3]};
+                marked[2 * nmarked + 1] = (v4i){wd[4], wd[5], wd[6], wd[7]};
+                ++nmarked;
+            }
         });
     });
-    const __attribute__((address_space(5))) uint32_t *sw = (const __attribute__((address_space(5))) uint32_t *)stash;
+    int taken = 0;
     while (__ballot(pend != 0)) {
         bool want = false;
         float mf = 0.0f;
@@ -2552,12 +2580,15 @@ __device__ __forceinline__ void mfma3w_tile(const DevData &D, const ScanParams &
             const int i = __builtin_ctz(pend), bb = i >> 2, j = i & 3;
             pend &= pend - 1;
             const int b = b0 + 16 * bb + 4 * (lane >> 4) + j;
+            const v4i w0 = marked[2 * taken], w1 = marked[2 * taken + 1];
+            ++taken;
             float m0[27], m1[27];
             m3_cells(
                 D, P, L, a, b,
                 [&](int cls, int x, int y, int z) __attribute__((always_inline)) {
-                    const uint32_t v = sw[4 * (16 * cls + 4 * bb + x * 2 + y) + z * 2 + (j >> 1)];
-                    return (float)(v >> (16 * (j & 1)) & 0xffffu);
+                    const int e = (x * 2 + y) * 2 + z;
+                    const uint32_t v = (uint32_t)(e < 4 ? w0[e] : w1[e - 4]);
+                    return (float)(cls ? v >> 16 : v & 0xffffu);
                 },
                 m0, m1);
             screen_exact<27>(m0, m1, [&] { return m3_key(P, D.n, a, b, L.c); }, tau, tau_combo, P, want, mf, combo);
