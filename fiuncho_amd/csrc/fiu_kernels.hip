@@ -1829,6 +1829,9 @@ k_mfma2(const DevData D, const ScanParams P, TopState *st, const Fp4Image F4, in
 // 128 + h * 128 + j the b rows; segments swizzled as in mfma2_tile.
 constexpr int kM2wRing = 3;
 constexpr int kM2wStage = 384 * 64;
+// Words of the first class's 16 packed count words that stay in registers
+// through the second class's stages (the rest wait in private memory).
+constexpr int kM2wKeep = 8;
 
 // Counts of instance row block (ab, r) as 16-bit pairs: word (g * 2 + h) * 2
 // + q / 2 holds f(g, h) of row 8 r + 4 (lane / 32) + q of a block ab, in the
@@ -1871,9 +1874,15 @@ __device__ __forceinline__ void mfma2w_tile(const DevData &D, const ScanParams &
         vo[t] = 16u * (uint32_t)(4 * (gen * np + snp + lr) + sg);
     }
     v16f acc[2][2][2];   // [a block][g][h], the class being scanned
-    v4i stash_mem[32];   // pack_counts2 words: class c, block (ab, r), half -> 16 c + 2 (4 ab + r) + half
+    // the first class's counts (pack_counts2 words, block (ab, r), half ->
+    // 2 (4 ab + r) + half) in private memory, out of the registers the
+    // second class needs; `marked`: both classes' counts (f(g, h) of class 0
+    // | class 1 << 16, word g * 2 + h) of the instances the epilogue's first
+    // pass marks, in order
+    v4i stash_mem[16], marked_mem[32], keep[kM2wKeep];
     __attribute__((address_space(5))) v4i *stash = (__attribute__((address_space(5))) v4i *)stash_mem;
-    asm volatile("" : "+v"(stash));
+    __attribute__((address_space(5))) v4i *marked = (__attribute__((address_space(5))) v4i *)marked_mem;
+    asm volatile("" : "+v"(stash), "+v"(marked));
     const int r32 = lane & 31;
 #pragma unroll
     for (int cls = 0; cls < 2; ++cls) {
@@ -1955,13 +1964,17 @@ __device__ __forceinline__ void mfma2w_tile(const DevData &D, const ScanParams &
             for (int r = 0; r < 4; ++r)
 #pragma unroll
                 for (int half = 0; half < 2; ++half)
-                    stash[16 * cls + 2 * (4 * ab + r) + half] = pack_counts2(acc, ab, r, half);
+                    if (cls == 0) {
+                        const int i = 2 * (4 * ab + r) + half;
+                        if (i < kM2wKeep) keep[i] = pack_counts2(acc, ab, r, half);
+                        else stash[i] = pack_counts2(acc, ab, r, half);
+                    }
     }
     // epilogue: lane holds column b = b0 + 32 w + lane % 32 and, per a block
     // ab, rows a = a0 + 32 ab + 8 r + 4 (lane / 32) + q, r, q < 4.  First
     // pass: the bound of each of the 32 instances (ab, r, q), the passing
-    // ones marked in `pend` (bit 16 ab + 4 r + q); second pass: one marked
-    // instance per lane per round, counts read back from the stash
+    // ones marked in `pend` (bit 16 ab + 4 r + q) and their counts kept in
+    // `marked`; second pass: one marked instance per lane per round
     const float tau = st->tau;
     const uint64_t tau_combo = st->tau_combo;
     const int b = b0 + 32 * w + r32;
@@ -1993,9 +2006,11 @@ __device__ __forceinline__ void mfma2w_tile(const DevData &D, const ScanParams &
         return valid;
     };
     uint32_t pend = 0;
+    int nmarked = 0;
     static_for<8>([&](auto ABR) __attribute__((always_inline)) {
         constexpr int ab = decltype(ABR)::value >> 2, r = decltype(ABR)::value & 3;
-        const v4i lo0 = stash[2 * (4 * ab + r)], lo1 = stash[2 * (4 * ab + r) + 1];
+        constexpr int i0 = 2 * (4 * ab + r);
+        const v4i lo0 = i0 < kM2wKeep ? keep[i0] : stash[i0], lo1 = i0 + 1 < kM2wKeep ? keep[i0 + 1] : stash[i0 + 1];
         static_for<4>([&](auto Q) __attribute__((always_inline)) {
             constexpr int q = decltype(Q)::value;
             float m0[9], m1[9];
@@ -2007,10 +2022,22 @@ __device__ __forceinline__ void mfma2w_tile(const DevData &D, const ScanParams &
                     return (float)((uint32_t)lo[h * 2 + (q >> 1)] >> (16 * (q & 1)) & 0xffffu);
                 },
                 m0, m1);
-            if (valid && bound_ok<9>(m0, m1, tau, P)) pend |= 1u << (16 * ab + 4 * r + q);
+            if (valid && bound_ok<9>(m0, m1, tau, P)) {
+                pend |= 1u << (16 * ab + 4 * r + q);
+                int wd[4];
+#pragma unroll
+                for (int g = 0; g < 2; ++g)
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) {
+                        const v4i &lo = g ? lo1 : lo0;
+                        wd[g * 2 + h] = (int)((uint32_t)lo[h * 2 + (q >> 1)] >> (16 * (q & 1)) & 0xffffu |
+                                              (uint32_t)acc[ab][g][h][r * 4 + q] << 16);
+                    }
+                marked[nmarked++] = (v4i){wd[0], wd[1], wd[2], wd[3]};
+            }
         });
     });
-    const __attribute__((address_space(5))) uint32_t *sw = (const __attribute__((address_space(5))) uint32_t *)stash;
+    int taken = 0;
     while (__ballot(pend != 0)) {
         bool want = false;
         float mf = 0.0f;
@@ -2019,12 +2046,13 @@ __device__ __forceinline__ void mfma2w_tile(const DevData &D, const ScanParams &
             const int i = __builtin_ctz(pend), ab = i >> 4, r = (i >> 2) & 3, q = i & 3;
             pend &= pend - 1;
             const int a = a0 + 32 * ab + 8 * r + 4 * (lane >> 5) + q;
+            const v4i wd = marked[taken++];
             float m0[9], m1[9];
             cells(
                 a,
                 [&](int cls, int g, int h) __attribute__((always_inline)) {
-                    const uint32_t v = sw[4 * (16 * cls + 2 * (4 * ab + r) + g) + h * 2 + (q >> 1)];
-                    return (float)(v >> (16 * (q & 1)) & 0xffffu);
+                    const uint32_t v = (uint32_t)wd[g * 2 + h];
+                    return (float)(cls ? v >> 16 : v & 0xffffu);
                 },
                 m0, m1);
             screen_exact<9>(m0, m1, [&] { return prefix_key(P.binom, D.n, &a, 1) + (uint64_t)b; }, tau, tau_combo, P,
