@@ -48,11 +48,12 @@ N_SNPS, N_CASES, N_CTRLS = 10_000, 8_000, 8_000
 PLANTED = (17, 4242, 9001)
 ORDER, OUTPUTS = 3, 100
 VALU_PEAK = 256 * 64 * 2.4e9          # int32 VALU lane-ops per second (MI355X)
-# Bytes fetched beyond L2 per triple by the order-3 scan kernels: rocprofv3
+# Bytes moved beyond L2 per triple by the order-3 scan kernels: rocprofv3
 # FETCH_SIZE per dispatch x 2.0 (the factor a plain 1 GiB copy shows) +
 # WRITE_SIZE x 1.0, divided by the triples of those dispatches
-# (profiles/k_scan3m_order3_10000x16000.md, profiles/k_mfma3_order3_10000x16000.md).
-TRAFFIC_BYTES_PER_TRIPLE = {"marginal": 1563.0, "mfma": 47.7}
+# (profiles/k_scan3m_order3_10000x16000.md, profiles/k_mfma3w_order3_10000x16000.md;
+# k_mfma3, the 64 x 32 tile of variants 3 / 4, from its own earlier profile).
+TRAFFIC_BYTES_PER_TRIPLE = {"k_scan3m": 1563.0, "k_mfma3w": 90.4, "k_mfma3": 47.7}
 
 
 def alg_ops_per_combo(order, cases, ctrls):
@@ -633,7 +634,9 @@ def main():
     unit_samples = N_CASES + N_CTRLS
     value = combos * unit_samples / elapsed
     algo_name = {1: "direct", 2: "marginal", 3: "mfma"}.get(used_algo, str(used_algo))
-    per_triple = TRAFFIC_BYTES_PER_TRIPLE.get(algo_name)
+    kernel = {"mfma": "k_mfma3" if search.last_stats.variant in (3, 4) else "k_mfma3w",
+              "marginal": "k_scan3m"}.get(algo_name, "k_scan_direct<3>")
+    per_triple = TRAFFIC_BYTES_PER_TRIPLE.get(kernel)
     traffic = per_triple * combos / max(launches, 1) if per_triple else None
     ref_ops = combos * alg_ops_per_combo(ORDER, N_CASES, N_CTRLS)   # the reference's AND + popcount ops
     if algo_name == "mfma":
@@ -641,8 +644,7 @@ def main():
         flops = combos * 8 * 2 * unit_samples
         roofline = {"bound": "mfma", "achieved": flops / (scan_ms / 1e3) / 1e12, "peak": MFMA_FP4_PEAK / 1e12,
                     "unit": "TFLOP/s", "frac": flops / (scan_ms / 1e3) / MFMA_FP4_PEAK, "traffic": traffic,
-                    "kernel": "k_mfma3" if search.last_stats.variant in (3, 4) else "k_mfma3w",
-                    "operands": "fp4 (e2m1) x fp4, fp32 accumulation (exact)",
+                    "kernel": kernel, "operands": "fp4 (e2m1) x fp4, fp32 accumulation (exact)",
                     "flops_per_combo_sample": 16,
                     "reference_algorithm_valu_frac": ref_ops / (scan_ms / 1e3) / VALU_PEAK}
     else:
@@ -651,7 +653,7 @@ def main():
         executed = combos * 2 * (8 if algo_name == "marginal" else 27) * 2 * (cw + tw)
         roofline = {"bound": "valu", "achieved": ref_ops / (scan_ms / 1e3) / 1e12, "peak": VALU_PEAK / 1e12,
                     "unit": "TOP/s", "frac": ref_ops / (scan_ms / 1e3) / VALU_PEAK, "traffic": traffic,
-                    "kernel": "k_scan3m" if algo_name == "marginal" else "k_scan_direct<3>",
+                    "kernel": kernel,
                     "ops_per_combo_sample": alg_ops_per_combo(ORDER, N_CASES, N_CTRLS) / unit_samples,
                     "executed_frac": executed / (scan_ms / 1e3) / VALU_PEAK}
     roofline.update({"launches": launches, "avg_launch_ms": scan_ms / max(launches, 1)})

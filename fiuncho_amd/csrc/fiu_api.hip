@@ -455,7 +455,7 @@ bool valid_variant(int algo, int order, int variant)
     switch (algo) {
     case FIU_ALGO_MARGINAL:
         return order <= 3 && (variant == 2 || variant == kMarginalPW || (order == 2 && variant == 8));
-    case FIU_ALGO_MFMA: return order == 3 ? (variant >= 3 && variant <= 6) : (variant == 1 || variant == 2);
+    case FIU_ALGO_MFMA: return order == 3 ? (variant >= 3 && variant <= 6) : (variant >= 1 && variant <= 3);
     default: return false;
     }
 }
@@ -913,9 +913,14 @@ void search(fiu_dataset *d, Workspace &ws, int order, int32_t step, int32_t offs
     st.variant = variant;
     const int pw = algo == FIU_ALGO_MARGINAL && variant ? variant : kMarginalPW;
     // order-3 matrix-core variants: 3 and 4 the 64 x 32 tile with rings of 3
-    // and 4 stages, 5 and 6 the 64 x 64 tile (default; its counts wait as
-    // 16-bit pairs: classes below 2^16 samples, else the 64 x 32 tile)
-    int ring = algo == FIU_ALGO_MFMA && order == 3 && variant ? variant : 6;
+    // and 4 stages, 5 and 6 the 64 x 64 tile (its counts wait as 16-bit
+    // pairs: classes below 2^16 samples, else the 64 x 32 tile).  Default:
+    // 6 from 2,048 SNPs and 8,192 samples on (measured: the larger tile is
+    // 7 % faster on 10,000 x 16,000, slower on 2,000 x 4,000 and below, where
+    // its fewer workgroups and longer epilogue do not pay), else 4
+    const uint64_t samples = (uint64_t)d->cases + d->ctrls;
+    int ring = algo == FIU_ALGO_MFMA && order == 3 && variant ? variant
+                                                              : (n >= 2048 && samples >= 8192 ? 6 : 4);
     if (ring >= 5 && (d->cases >= (1u << 16) || d->ctrls >= (1u << 16) || !ws.private_memory())) ring = 4;
     // matrix-core scans walk super-blocks of 8 x 8 tiles of 64 x 64 pairs;
     // fp4 operands accumulate in fp32, exact while every count is below 2^24;
@@ -924,11 +929,14 @@ void search(fiu_dataset *d, Workspace &ws, int order, int32_t step, int32_t offs
     const int64_t n_super = mfma2_superblocks(T);
     const bool int8 = (cfg && (cfg->flags & FIU_FLAG_MFMA_INT8)) || (order == 2 && variant == 1);
     const bool mfma_fp4 = algo == FIU_ALGO_MFMA && !int8 && small_classes && ws.fp4_image(d, s);
-    // order-2 matrix-core variants: 0 the 64 x 128 tile (default; 16-bit
-    // counts between the classes: classes below 2^16 samples, else 64 x 64),
-    // 1 int8 operands, 2 the 64 x 64 tile
-    const bool m2_wide =
-        order == 2 && variant != 2 && d->cases < (1u << 16) && d->ctrls < (1u << 16) && ws.private_memory();
+    // order-2 matrix-core variants: 1 int8 operands, 2 fp4 with the 64 x 64
+    // tile, 3 fp4 with the 64 x 128 tile (16-bit counts between the classes:
+    // classes below 2^16 samples, else 64 x 64).  Default: 3 from 16,384
+    // SNPs and 8,192 samples on (measured: faster on 50,000 x 32,000, slower
+    // below), else 2
+    const bool m2_pick_wide = variant == 3 || (variant == 0 && n >= 16384 && samples >= 8192);
+    const bool m2_wide = order == 2 && m2_pick_wide && d->cases < (1u << 16) && d->ctrls < (1u << 16) &&
+                         ws.private_memory();
 
     // chunk plan: even chunks of about `target` combinations (at most
     // max_chunks of them); before the first, a sampling pass over the last

@@ -74,7 +74,7 @@ DEFAULT_CLASSES: List[ShapeClass] = (
 def candidates(order: int) -> List[Tuple[int, int]]:
     """(algorithm, variant) pairs the tuner times at `order`."""
     if order == 2:
-        return [(ALGO_MFMA, 0), (ALGO_MFMA, 1), (ALGO_MFMA, 2), (ALGO_MARGINAL, 2), (ALGO_MARGINAL, 4),
+        return [(ALGO_MFMA, 1), (ALGO_MFMA, 2), (ALGO_MFMA, 3), (ALGO_MARGINAL, 2), (ALGO_MARGINAL, 4),
                 (ALGO_MARGINAL, 8)]
     if order == 3:
         return [(ALGO_MFMA, 4), (ALGO_MFMA, 5), (ALGO_MFMA, 6), (ALGO_MARGINAL, 2), (ALGO_MARGINAL, 4)]

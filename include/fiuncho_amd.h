@@ -293,13 +293,14 @@ fiu_status fiu_write_results(const char *path, const int32_t *combos,
  *                      only, 8;
  *   FIU_ALGO_MFMA, order 3: tile and LDS staging depth: 3, 4 = 64 x 32
  *                      (b, c) per workgroup with 3 or 4 stages, 5, 6 =
- *                      64 x 64 with 3 or 4 stages (6 is the default; 5 and 6
- *                      fall back to 4 when a class holds 2^16 samples or
- *                      more);
- *   FIU_ALGO_MFMA, order 2: 0 = fp4 operands, 64 x 128 pairs per
- *                      workgroup (default; 64 x 64 when a class holds 2^16
- *                      samples or more), 1 = int8 operands, 2 = fp4
- *                      operands, 64 x 64 pairs per workgroup.
+ *                      64 x 64 with 3 or 4 stages (default: 6 from 2,048
+ *                      SNPs and 8,192 samples on, else 4; 5 and 6 fall back
+ *                      to 4 when a class holds 2^16 samples or more);
+ *   FIU_ALGO_MFMA, order 2: 1 = int8 operands, 2 = fp4 operands, 64 x 64
+ *                      pairs per workgroup, 3 = fp4 operands, 64 x 128 pairs
+ *                      per workgroup (64 x 64 when a class holds 2^16
+ *                      samples or more); default: 3 from 16,384 SNPs and
+ *                      8,192 samples on, else 2.
  * The table is written by an autotuner that times the candidates on the
  * device (fiuncho_amd.tune); it is process-wide and starts empty.  Batched
  * scans use its algorithm and, for the popcount kinds (orders 2-5 off the

@@ -116,7 +116,7 @@ def test_tuning_table_roundtrip_and_validation():
         assert abs(got[0].ms - 1.5) < 1e-6
         assert [e.batch_unit for e in got] == [e.batch_unit for e in good]
         bad = [E(3, 0, 10, 0, 10, fa.ALGO_MARGINAL, 8),     # 8 prefixes per wave: order 2 only
-               E(2, 0, 10, 0, 10, fa.ALGO_MFMA, 3),         # ring depth: order 3 only
+               E(2, 0, 10, 0, 10, fa.ALGO_MFMA, 4),         # ring depth: order 3 only
                E(3, 0, 10, 0, 10, fa.ALGO_MFMA, 7),         # no such tile / staging variant
                E(4, 0, 10, 0, 10, fa.ALGO_MFMA, 0),         # matrix cores: orders 2-3
                E(13, 0, 10, 0, 10, fa.ALGO_DIRECT, 0),      # orders 2-12 only

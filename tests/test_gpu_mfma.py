@@ -21,9 +21,9 @@ from .test_gpu_parity import synth
 pytestmark = pytest.mark.gpu
 
 
-# variant 0: fp4 operands, 64 x 128 pairs per workgroup (the default); 1: int8
-# operands; 2: fp4 operands, 64 x 64 pairs per workgroup
-@pytest.mark.parametrize("variant", [0, 1, 2])
+# variant 0: the default (fp4, 64 x 64 pairs per workgroup at these sizes); 1:
+# int8 operands; 2: fp4, 64 x 64; 3: fp4, 64 x 128
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
 @pytest.mark.parametrize("snps,cases,ctrls,outputs,seed", [
     (70, 129, 63, 40, 21),       # one partial tile, ragged classes
     (128, 512, 512, 100, 22),    # two full tiles per side
@@ -47,7 +47,7 @@ def test_mfma_matches_popcount_scan():
     print("case: mfma vs popcount 1500 SNPs", flush=True)
     d, _ = synth(1500, 3000, 2500, 31, planted=(100, 1200))
     ref = fa.GpuSearch(algorithm=fa.ALGO_MARGINAL).run_arrays(d, 2, fa.Distribution(1500, 1), 500)
-    for int8, variant in ((False, 0), (True, 0), (False, 2)):
+    for int8, variant in ((False, 0), (True, 0), (False, 2), (False, 3)):
         c, m = fa.GpuSearch(algorithm=fa.ALGO_MFMA, mfma_int8=int8, variant=variant).run_arrays(
             d, 2, fa.Distribution(1500, 1), 500)
         assert c.tolist() == ref[0].tolist() and np.array_equal(m, ref[1])

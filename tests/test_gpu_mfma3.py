@@ -34,8 +34,9 @@ def test_mfma3_vs_oracle(snps, cases, ctrls, outputs, seed):
     print(f"case: mfma3 {snps}x{cases}+{ctrls}", flush=True)
     d, od = synth(snps, cases, ctrls, seed)
     oc, om, _ = O.search(od, 3, outputs, threads=4, mi_mode=2)
-    # default (64 x 64 tile, 4-stage ring), 64 x 32 with 3 and 4 stages, 64 x 64 with 3
-    for variant in (0, 3, 4, 5):
+    # default (64 x 32, 4-stage ring, at these sizes), 64 x 32 with 3 stages, 64 x 64
+    # with 3 and 4 stages
+    for variant in (0, 3, 5, 6):
         s = fa.GpuSearch(algorithm=fa.ALGO_MFMA, variant=variant)
         c, m = s.run_arrays(d, 3, fa.Distribution(snps, 2), outputs)
         assert s.last_stats.algorithm == fa.ALGO_MFMA
