@@ -950,6 +950,13 @@ void search(fiu_dataset *d, Workspace &ws, int order, int32_t step, int32_t offs
     if (algo == FIU_ALGO_MFMA && order == 2) {
         // chunks of super-blocks (262,144 pairs each) instead of prefixes
         int64_t len = std::max<int64_t>(1, (int64_t)(target / (64 * 4096)));
+        // the 64 x 128 tile: launches of at most 64 super-blocks (2,048
+        // workgroups, four per slot of the device), whose workgroups on one
+        // XCD start close together and so share their row panels in L2 for
+        // longer (measured on 50,000 x 32,000: 64.0 ms per scan against 69.9
+        // with launches of 1,024 super-blocks; 16 per launch: 70.2, the
+        // launch gaps and tails eat the gain)
+        if (m2_wide && !(cfg && cfg->chunk_combos)) len = std::min<int64_t>(len, 64);
         len = std::max<int64_t>(len, (int64_t)(((uint64_t)n_super + max_chunks - 1) / max_chunks));
         plan.set_uniform(n_super, len);
     } else if (algo == FIU_ALGO_MFMA) {
