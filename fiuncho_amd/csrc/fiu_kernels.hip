@@ -2133,6 +2133,13 @@ __device__ __forceinline__ M3Lane m3_lane(const DevData &D, const ScanParams &P,
     return L;
 }
 
+// pair_index for 0 <= x < y < n: one 32 x 32 -> 64-bit product, no signed
+// division.
+__device__ __forceinline__ uint64_t pair_index_u(uint32_t x, uint32_t y, uint32_t n)
+{
+    return ((uint64_t)x * (2u * n - x - 1u) >> 1) + (y - x - 1u);
+}
+
 // Triple (a, b, L.c): the 8 matrix-core counts cnt(cls, x, y, z), x, y, z in
 // {0, 1}, with the pair table, the singles and the class totals give the 27
 // cells m0 / m1 of each class (inclusion-exclusion in FP32: the counts are
@@ -2143,19 +2150,25 @@ __device__ __forceinline__ bool m3_cells(const DevData &D, const ScanParams &P, 
                                          float (&m0)[27], float (&m1)[27])
 {
     const int n = D.n, c = L.c;
-    const int64_t rel = L.rank_a + (b - a - 1) - P.offset;
-    const bool valid = L.c_ok && b > a && c > b && rel >= 0 && (P.step == 1 || rel % P.step == 0);
-    uint32_t fab[8] = {0, 0, 0, 0, 0, 0, 0, 0}, fbc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    uint32_t sb[6] = {0, 0, 0, 0, 0, 0};
-    if (valid) {
-        const uint4 *pab = (const uint4 *)(P.pair + pair_index(a, b, n) * 8);
-        const uint4 *pbc = (const uint4 *)(P.pair + pair_index(b, c, n) * 8);
+    const bool in_range = L.c_ok && b > a && c > b;   // a < b < c < n
+    bool valid = in_range;
+    if (P.step != 1 || P.offset != 0) {   // a share of the prefixes (wave-uniform)
+        const int64_t rel = L.rank_a + (b - a - 1) - P.offset;
+        valid = valid && rel >= 0 && (P.step == 1 || rel % P.step == 0);
+    }
+    // a triple outside the range reads the rows of (a, a + 1, a + 2) instead
+    // (every tile has a < n - 2) and is dropped by `valid`: no select per value
+    const uint32_t b1 = in_range ? b : a + 1, c1 = in_range ? c : a + 2;
+    uint32_t fab[8], fbc[8], sb[6];
+    {
+        const uint4 *pab = (const uint4 *)(P.pair + pair_index_u(a, b1, n) * 8);
+        const uint4 *pbc = (const uint4 *)(P.pair + pair_index_u(b1, c1, n) * 8);
         const uint4 p0 = pab[0], p1 = pab[1], q0 = pbc[0], q1 = pbc[1];
         fab[0] = p0.x, fab[1] = p0.y, fab[2] = p0.z, fab[3] = p0.w;
         fab[4] = p1.x, fab[5] = p1.y, fab[6] = p1.z, fab[7] = p1.w;
         fbc[0] = q0.x, fbc[1] = q0.y, fbc[2] = q0.z, fbc[3] = q0.w;
         fbc[4] = q1.x, fbc[5] = q1.y, fbc[6] = q1.z, fbc[7] = q1.w;
-        const uint32_t *sp = D.singles + (size_t)b * 6;
+        const uint32_t *sp = D.singles + (size_t)b1 * 6;
 #pragma unroll
         for (int e = 0; e < 6; ++e) sb[e] = sp[e];
     }
