@@ -2100,14 +2100,22 @@ constexpr int kM3Stage = 13 * kM3Group;
 constexpr int kM3Ring = 4;   // default stages resident: the one read + up to 3 loading
 __device__ __forceinline__ int m3_swz(int r) { return (r >> 1) & 3; }
 
+// pair_index for 0 <= x < y < n: one 32 x 32 -> 64-bit product, no signed
+// division.
+__device__ __forceinline__ uint64_t pair_index_u(uint32_t x, uint32_t y, uint32_t n)
+{
+    return ((uint64_t)x * (2u * n - x - 1u) >> 1) + (y - x - 1u);
+}
+
 // Values of the order-3 matrix-core epilogue that depend only on the first
 // SNP a and the lane's last SNP c: c's singles, the pair (a, c) and the rank
 // of the prefix (a, a + 1).
 struct M3Lane {
     int c;
     bool c_ok;
-    const uint32_t *sa;
-    uint32_t sc[2][2], fac[8];
+    // as FP32 (exact): singles of a and c (class, genotype 0 / 1), the pair
+    // (a, c) (class, x, z), the class totals
+    float sa[2][2], sc[2][2], fac[8], tot[2];
     int64_t rank_a;
 };
 
@@ -2117,27 +2125,26 @@ __device__ __forceinline__ M3Lane m3_lane(const DevData &D, const ScanParams &P,
     const int n = D.n;
     L.c = c;
     L.c_ok = c < n && c > a;
-    L.sa = D.singles + (size_t)a * 6;
+    const uint32_t *sa = D.singles + (size_t)a * 6;
+    // an out-of-range c reads c = a + 1's rows (its triples are dropped)
+    const uint32_t c1 = L.c_ok ? c : a + 1;
+    const uint32_t *sp = D.singles + (size_t)c1 * 6;
+    const uint4 *pac = (const uint4 *)(P.pair + pair_index_u(a, c1, n) * 8);
+    const uint4 u0 = pac[0], u1 = pac[1];
 #pragma unroll
-    for (int e = 0; e < 8; ++e) L.fac[e] = 0;
-    L.sc[0][0] = L.sc[0][1] = L.sc[1][0] = L.sc[1][1] = 0;
-    if (L.c_ok) {
-        const uint32_t *sp = D.singles + (size_t)c * 6;
-        L.sc[0][0] = sp[0], L.sc[0][1] = sp[1], L.sc[1][0] = sp[3], L.sc[1][1] = sp[4];
-        const uint4 *pac = (const uint4 *)(P.pair + pair_index(a, c, n) * 8);
-        const uint4 u0 = pac[0], u1 = pac[1];
-        L.fac[0] = u0.x, L.fac[1] = u0.y, L.fac[2] = u0.z, L.fac[3] = u0.w;
-        L.fac[4] = u1.x, L.fac[5] = u1.y, L.fac[6] = u1.z, L.fac[7] = u1.w;
+    for (int cls = 0; cls < 2; ++cls) {
+#pragma unroll
+        for (int g = 0; g < 2; ++g) {
+            L.sa[cls][g] = (float)sa[cls * 3 + g];
+            L.sc[cls][g] = (float)sp[cls * 3 + g];
+        }
+        L.tot[cls] = (float)D.total[cls];
     }
+    const uint32_t f[8] = {u0.x, u0.y, u0.z, u0.w, u1.x, u1.y, u1.z, u1.w};
+#pragma unroll
+    for (int e = 0; e < 8; ++e) L.fac[e] = (float)f[e];
     L.rank_a = (int64_t)a * (2 * (int64_t)n - a - 1) / 2;   // rank of the prefix (a, a+1)
     return L;
-}
-
-// pair_index for 0 <= x < y < n: one 32 x 32 -> 64-bit product, no signed
-// division.
-__device__ __forceinline__ uint64_t pair_index_u(uint32_t x, uint32_t y, uint32_t n)
-{
-    return ((uint64_t)x * (2u * n - x - 1u) >> 1) + (y - x - 1u);
 }
 
 // Triple (a, b, L.c): the 8 matrix-core counts cnt(cls, x, y, z), x, y, z in
@@ -2187,18 +2194,18 @@ __device__ __forceinline__ bool m3_cells(const DevData &D, const ScanParams &P, 
 #pragma unroll
         for (int x = 0; x < 2; ++x)
 #pragma unroll
-            for (int z = 0; z < 2; ++z) mm[x * 9 + 6 + z] = (float)L.fac[cls * 4 + x * 2 + z];
+            for (int z = 0; z < 2; ++z) mm[x * 9 + 6 + z] = L.fac[cls * 4 + x * 2 + z];
 #pragma unroll
         for (int y = 0; y < 2; ++y)
 #pragma unroll
             for (int z = 0; z < 2; ++z) mm[18 + y * 3 + z] = (float)fbc[cls * 4 + y * 2 + z];
-        mm[0 * 9 + 8] = (float)L.sa[cls * 3 + 0];
-        mm[1 * 9 + 8] = (float)L.sa[cls * 3 + 1];
+        mm[0 * 9 + 8] = L.sa[cls][0];
+        mm[1 * 9 + 8] = L.sa[cls][1];
         mm[18 + 0 * 3 + 2] = (float)sb[cls * 3 + 0];
         mm[18 + 1 * 3 + 2] = (float)sb[cls * 3 + 1];
-        mm[18 + 6 + 0] = (float)L.sc[cls][0];
-        mm[18 + 6 + 1] = (float)L.sc[cls][1];
-        mm[26] = (float)D.total[cls];
+        mm[18 + 6 + 0] = L.sc[cls][0];
+        mm[18 + 6 + 1] = L.sc[cls][1];
+        mm[26] = L.tot[cls];
         mobius3(mm);
     }
     return valid;
@@ -2447,14 +2454,17 @@ __device__ __forceinline__ void mfma3w_tile(const DevData &D, const ScanParams &
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int np = D.n_pad;
     // staging: wave w loads b groups 2w, 2w + 1 (y = w / 2, blocks 2 (w % 2)
-    // and the next), c groups 8 + 2w, 9 + 2w (z = w / 2, same blocks) and,
-    // lanes 2w, 2w + 1, the a rows; the second group of each pair is the
-    // next 16 rows (+1,024 bytes)
+    // and the next; +1,024 bytes), its own c rows (groups 8 + w and 12 + w:
+    // z = 0, 1, SNPs c0 + 16 w ..) and, lanes 0 .. 7, its own copy of the a
+    // rows (bytes 128 w .. of group 16).  The c and a rows a wave reads are
+    // its own loads, complete after its own vmcnt wait: the next stage's c
+    // and a fragments and masks are read behind the current stage's x = 1
+    // products, before the barrier that publishes the shared b rows
     const int lr = lane >> 2, ls = (lane & 3) ^ m3_swz(lr);
     const uint32_t ub = 16u * (uint32_t)(4 * ((w >> 1) * np + b0 + 32 * (w & 1) + lr) + ls);
-    const uint32_t uc = 16u * (uint32_t)(4 * ((w >> 1) * np + c0 + 32 * (w & 1) + lr) + ls);
-    const bool a_lane = (lane >> 1) == w;
-    const uint32_t ua = 16u * (uint32_t)(4 * ((lr & 1) * np + a) + ls);
+    const uint32_t uc0 = 16u * (uint32_t)(4 * (c0 + 16 * w + lr) + ls), uc1 = uc0 + 64u * (uint32_t)np;
+    const bool a_lane = lane < 8;
+    const uint32_t ua = 16u * (uint32_t)(4 * ((lr & 1) * np + a) + ls);   // lanes 0 .. 7: row x = lr
     v4f acc[2][2][4][2];   // [x][y][b block][z], the class being scanned
     // the first class's counts (pack_counts, 16 words) in private memory,
     // out of the registers the second class needs; `marked`: both classes'
@@ -2489,11 +2499,11 @@ __device__ __forceinline__ void mfma3w_tile(const DevData &D, const ScanParams &
             int8_t *s = lds[q % R];
             dma16(rq, ub, s + (2 * w) * kM3Group);
             dma16(rq, ub + 1024u, s + (2 * w + 1) * kM3Group);
-            dma16(rq, uc, s + (8 + 2 * w) * kM3Group);
-            dma16(rq, uc + 1024u, s + (9 + 2 * w) * kM3Group);
+            dma16(rq, uc0, s + (8 + w) * kM3Group);
+            dma16(rq, uc1, s + (12 + w) * kM3Group);
         };
         auto issue_a = [&](int q) __attribute__((always_inline)) {
-            if (a_lane) dma16(quad_rsrc(q), ua, lds[q % R] + 16 * kM3Group);
+            if (a_lane) dma16(quad_rsrc(q), ua, lds[q % R] + 16 * kM3Group + 128 * w);
         };
 #pragma unroll
         for (int q = 0; q < R - 1; ++q)
@@ -2501,30 +2511,41 @@ __device__ __forceinline__ void mfma3w_tile(const DevData &D, const ScanParams &
                 issue_bc(q);
                 issue_a(q);
             }
-        auto stage = [&](int it, bool steady) __attribute__((always_inline)) {
-            // 5 loads per stage (the a rows' load counts for every lane)
-            static_assert(R >= 2 && 5 * (R - 2) < 16, "ring depth");
+        // 5 loads per stage (4 issued early in a stage, the a rows' at its
+        // end; the a load counts for every lane)
+        static_assert(R == 3 || R == 4, "the waits below count rings of 3 or 4 stages");
+        // the masks of stage q (x, z): this wave's c rows AND its a rows
+        auto masks = [&](int q, v4i (&mk)[2][2]) __attribute__((always_inline)) {
+            const int8_t *sq = lds[q % R];
+            v4i ma[2], fc[2];
+#pragma unroll
+            for (int x = 0; x < 2; ++x) ma[x] = *(const v4i *)(sq + 16 * kM3Group + 128 * w + x * 64 + ao);
+#pragma unroll
+            for (int z = 0; z < 2; ++z) fc[z] = *(const v4i *)(sq + (8 + 4 * z + w) * kM3Group + fo);
+#pragma unroll
+            for (int x = 0; x < 2; ++x)
+#pragma unroll
+                for (int z = 0; z < 2; ++z) mk[x][z] = fc[z] & ma[x];   // nibble 0x2 & 0x2 = 1.0, else 0
+        };
+        // stage 0's masks: its loads complete once at most those of stages
+        // 1 .. min(R - 1, iters) - 1 are pending
+        v4i mA[2][2], mB[2][2];
+        wait_stages<R - 2, 5>(iters - 1);
+        masks(0, mA);
+        auto stage = [&](int it, bool steady, v4i (&mc)[2][2], v4i (&mn)[2][2]) __attribute__((always_inline)) {
+            // this wave's loads of stage it are complete once at most those
+            // of the stages after it (min(iters - 1 - it, R - 2), 5 loads
+            // each) are pending; the barrier then publishes every wave's b rows
             if (steady) wait_vmcnt<5 * (R - 2)>();
             else wait_stages<R - 2, 5>(iters - 1 - it);
             stage_barrier();
             __builtin_amdgcn_s_setprio(1);
             const int8_t *sb = lds[it % R];
-            // x outer, b fragment, z inner: the x = 0 products need the a
-            // rows, both c fragments and then the b fragments one by one, so
-            // the b reads and the x = 1 masks issue between products, in
-            // segments the compiler keeps in order (12 reads in a row
-            // overfill the LDS command queue)
-            v4i ma[2], fc[2], fb[8], mc[2][2];
-#pragma unroll
-            for (int x = 0; x < 2; ++x) ma[x] = *(const v4i *)(sb + 16 * kM3Group + x * 64 + ao);
-#pragma unroll
-            for (int z = 0; z < 2; ++z) fc[z] = *(const v4i *)(sb + (8 + 4 * z + w) * kM3Group + fo);
+            v4i fb[8];
 #pragma unroll
             for (int k = 0; k < 2; ++k) fb[k] = *(const v4i *)(sb + k * kM3Group + fo);
             const bool more = steady || it + R - 1 < iters;
             if (more) issue_bc(it + R - 1);
-#pragma unroll
-            for (int z = 0; z < 2; ++z) mc[0][z] = fc[z] & ma[0];   // nibble 0x2 & 0x2 = 1.0, else 0
             __builtin_amdgcn_sched_barrier(0);
             auto product = [&](int x, int k, int z) __attribute__((always_inline)) {
                 const v8i b8 = {fb[k][0], fb[k][1], fb[k][2], fb[k][3], 0, 0, 0, 0};
@@ -2532,29 +2553,64 @@ __device__ __forceinline__ void mfma3w_tile(const DevData &D, const ScanParams &
                 acc[x][k >> 2][k & 3][z] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(
                     b8, c8, acc[x][k >> 2][k & 3][z], 4, 4, 0, 0, 0, 0);
             };
-            // b fragment k (y = k / 4, block k % 4): the read of fragment k +
-            // 2, one word of the x = 1 masks, the two x = 0 products
+            // x = 0: b fragment k (y = k / 4, block k % 4): the read of
+            // fragment k + 2, the two products
 #pragma unroll
             for (int k = 0; k < 8; ++k) {
                 if (k < 6) fb[k + 2] = *(const v4i *)(sb + (k + 2) * kM3Group + fo);
-                mc[1][k >> 2][k & 3] = fc[k >> 2][k & 3] & ma[1][k & 3];
                 product(0, k, 0);
                 product(0, k, 1);
                 __builtin_amdgcn_sched_barrier(0);
             }
+            // the next stage's masks from this wave's own loads: complete once
+            // at most those issued after them are pending (4 stages: stage it +
+            // 2's 5 and stage it + 3's 4 early ones; 3 stages: stage it + 2's 4
+            // early ones; where those stages exist)
+            const bool next = steady || it + 1 < iters;
+            if (next) {
+                if constexpr (R == 4) {
+                    if (steady || it + 3 < iters) wait_vmcnt<9>();
+                    else if (it + 2 < iters) wait_vmcnt<5>();
+                    else wait_vmcnt<0>();
+                } else {
+                    if (steady || it + 2 < iters) wait_vmcnt<4>();
+                    else wait_vmcnt<0>();
+                }
+                masks(it + 1, mn);
+            }
+            __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int k = 0; k < 8; ++k) {
                 product(1, k, 0);
                 product(1, k, 1);
             }
-            // the a rows' load (two lanes, a branch) at the stage's priority:
-            // 2 % faster than after the drop
+            if (steady) {
+                // the next stage's 4 reads, then its 16 masks one behind each
+                // of the x = 1 products' first 16
+                __builtin_amdgcn_sched_group_barrier(0x100, 4, 0);
+#pragma unroll
+                for (int i = 0; i < 16; ++i) {
+                    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+                    __builtin_amdgcn_sched_group_barrier(0x002, 1, 0);
+                }
+            }
             if (more) issue_a(it + R - 1);
             __builtin_amdgcn_s_setprio(0);
         };
+        // stages with at least R - 1 stages after them (and one more, the
+        // pair's second) run a body without branches; mask registers
+        // alternate between the two stages of a pair
         int it = 0;
-        for (; it + R - 1 < iters; ++it) stage(it, true);
-        for (; it < iters; ++it) stage(it, false);
+        for (; it + R < iters; it += 2) {
+            stage(it, true, mA, mB);
+            stage(it + 1, true, mB, mA);
+        }
+        for (; it < iters; it += 2) {
+            stage(it, false, mA, mB);
+            if (it + 1 < iters) stage(it + 1, false, mB, mA);
+        }
+        // the next class's first loads reuse the ring: every wave must be
+        // done reading this class's last stages
         stage_barrier();
         if (cls == 0) {
 #pragma unroll
