@@ -53,7 +53,7 @@ VALU_PEAK = 256 * 64 * 2.4e9          # int32 VALU lane-ops per second (MI355X)
 # WRITE_SIZE x 1.0, divided by the triples of those dispatches
 # (profiles/k_scan3m_order3_10000x16000.md, profiles/k_mfma3w_order3_10000x16000.md;
 # k_mfma3, the 64 x 32 tile of variants 3 / 4, from its own earlier profile).
-TRAFFIC_BYTES_PER_TRIPLE = {"k_scan3m": 1563.0, "k_mfma3w": 90.4, "k_mfma3": 47.7}
+TRAFFIC_BYTES_PER_TRIPLE = {"k_scan3m": 1563.0, "k_mfma3w": 89.7, "k_mfma3": 47.7}
 
 
 def alg_ops_per_combo(order, cases, ctrls):
