@@ -148,7 +148,7 @@ struct BatchJob {
     int32_t tiles;           // tiles per chunk
     Fp4Image F;              // matrix-core kinds: the dataset's fp4 image
     int32_t T;               // matrix-core kinds: 64-SNP tiles (n_pad / 64)
-    int32_t pad_;
+    int32_t wide;            // matrix-core kinds: the 64 x 64 (order 3) / 64 x 128 (order 2) tiles
 };
 
 // A contiguous prefix range [j0, j1) of one job in one launch; its units are

@@ -145,6 +145,7 @@ struct Job {
     uint32_t outputs = 0;
     // plan
     int kind = kKindDirect2, pw = 1, chunks = 1, tiles = 1;
+    bool wide = false;   // matrix-core kinds: the 64 x 64 / 64 x 128 tiles
     // units of work: prefixes (popcount kinds), first SNPs (order 3 on the
     // matrix cores) or super-blocks (order 2 on the matrix cores)
     int64_t n_pref = 0, next = 0, given = 0, seed = 1;
@@ -362,12 +363,14 @@ void Engine::plan_job(Job *j)
     const bool has_te = tuned(order, d->snps, (uint64_t)d->cases + d->ctrls, te);
     const int tuned_algo = has_te ? te.algorithm : FIU_ALGO_AUTO;
     const double unit_instr = has_te && te.batch_unit ? 40000.0 * std::ldexp(1.0, (int)te.batch_unit - 3) : 40000.0;
-    // (64 x 128 / 64 x 64 tiles, their counts 16-bit between the classes:
-    // classes below 2^16 samples)
     if ((tuned_algo == FIU_ALGO_AUTO || tuned_algo == FIU_ALGO_MFMA) && mfma_dataset(d) && d->fp4_ready &&
-        d->cases < (1u << 16) && d->ctrls < (1u << 16) &&
         (order == 2 || (order == 3 && d->pair_ready && j->step == 1))) {
         j->kind = order == 2 ? kKindMfma2 : kKindMfma3;
+        // the larger tiles from the sizes where the single search takes them
+        // (their counts wait as 16-bit pairs: classes below 2^16 samples)
+        const uint64_t samples = (uint64_t)d->cases + d->ctrls;
+        j->wide = d->cases < (1u << 16) && d->ctrls < (1u << 16) && samples >= 8192 &&
+                  (uint64_t)n >= (order == 3 ? 2048u : 16384u);
         j->pw = 1;
         j->n_pref = order == 2 ? mfma2_superblocks(T) : std::max(0, n - 2);
         // cost of one combination in the wave-instruction currency of the
@@ -501,6 +504,7 @@ void Engine::activate(Job *j, int slot)
     B.chunks = j->chunks;
     B.tiles = j->tiles;
     B.T = d->dev.n_pad / 64;
+    B.wide = j->wide ? 1 : 0;
     if (batch_mfma(j->kind)) {
         B.F.base = d->fp4.p;
         B.F.class_off[0] = 0;
@@ -576,10 +580,10 @@ bool Engine::launch_step()
         for (auto &pr : parts) {
             uint64_t u;
             if (j->kind == kKindMfma3) {
-                const int64_t m = T - (pr.first + 1) / 64;
-                u = (uint64_t)((m * (m + 1) / 2 + 7) / 8) * (uint64_t)(pr.second - pr.first);
+                const int64_t m = T - (pr.first + 1) / 64, items = j->wide ? m * (m + 1) / 2 : m * (m + 1);
+                u = (uint64_t)((items + 7) / 8) * (uint64_t)(pr.second - pr.first);
             } else {
-                u = (uint64_t)((pr.second - pr.first + 7) / 8) * 32;
+                u = (uint64_t)((pr.second - pr.first + 7) / 8) * (j->wide ? 32 : 64);
             }
             total += u;
         }
@@ -587,10 +591,10 @@ bool Engine::launch_step()
         for (auto &pr : parts) {
             uint64_t u;
             if (j->kind == kKindMfma3) {
-                const int64_t m = T - (pr.first + 1) / 64;
-                u = (uint64_t)((m * (m + 1) / 2 + 7) / 8) * (uint64_t)(pr.second - pr.first);
+                const int64_t m = T - (pr.first + 1) / 64, items = j->wide ? m * (m + 1) / 2 : m * (m + 1);
+                u = (uint64_t)((items + 7) / 8) * (uint64_t)(pr.second - pr.first);
             } else {
-                u = (uint64_t)((pr.second - pr.first + 7) / 8) * 32;
+                u = (uint64_t)((pr.second - pr.first + 7) / 8) * (j->wide ? 32 : 64);
             }
             BatchSeg &sg = msegs[nmseg++];
             sg.start = munits;

@@ -1832,6 +1832,9 @@ constexpr int kM2wStage = 384 * 64;
 // Words of the first class's 16 packed count words that stay in registers
 // through the second class's stages (the rest wait in private memory).
 constexpr int kM2wKeep = 8;
+// The same in the batched steps' persistent kernel, whose queue state takes
+// registers of its own.
+constexpr int kBatchM2wKeep = 4;
 
 // Counts of instance row block (ab, r) as 16-bit pairs: word (g * 2 + h) * 2
 // + q / 2 holds f(g, h) of row 8 r + 4 (lane / 32) + q of a block ab, in the
@@ -1847,6 +1850,7 @@ __device__ __forceinline__ v4i pack_counts2(const v16f (&acc)[2][2][2], int ab, 
                  pk(v1[r * 4 + 2], v1[r * 4 + 3])};
 }
 
+template <int KEEP>
 __device__ __forceinline__ void mfma2w_tile(const DevData &D, const ScanParams &P, TopState *st, const Fp4Image &F4,
                                             int64_t su, int slot, int T, int8_t (*lds)[kM2wStage])
 {
@@ -1879,7 +1883,7 @@ __device__ __forceinline__ void mfma2w_tile(const DevData &D, const ScanParams &
     // second class needs; `marked`: both classes' counts (f(g, h) of class 0
     // | class 1 << 16, word g * 2 + h) of the instances the epilogue's first
     // pass marks, in order
-    v4i stash_mem[16], marked_mem[32], keep[kM2wKeep];
+    v4i stash_mem[16], marked_mem[32], keep[KEEP > 0 ? KEEP : 1];
     __attribute__((address_space(5))) v4i *stash = (__attribute__((address_space(5))) v4i *)stash_mem;
     __attribute__((address_space(5))) v4i *marked = (__attribute__((address_space(5))) v4i *)marked_mem;
     asm volatile("" : "+v"(stash), "+v"(marked));
@@ -1966,7 +1970,7 @@ __device__ __forceinline__ void mfma2w_tile(const DevData &D, const ScanParams &
                 for (int half = 0; half < 2; ++half)
                     if (cls == 0) {
                         const int i = 2 * (4 * ab + r) + half;
-                        if (i < kM2wKeep) keep[i] = pack_counts2(acc, ab, r, half);
+                        if (i < KEEP) keep[i] = pack_counts2(acc, ab, r, half);
                         else stash[i] = pack_counts2(acc, ab, r, half);
                     }
     }
@@ -2010,7 +2014,7 @@ __device__ __forceinline__ void mfma2w_tile(const DevData &D, const ScanParams &
     static_for<8>([&](auto ABR) __attribute__((always_inline)) {
         constexpr int ab = decltype(ABR)::value >> 2, r = decltype(ABR)::value & 3;
         constexpr int i0 = 2 * (4 * ab + r);
-        const v4i lo0 = i0 < kM2wKeep ? keep[i0] : stash[i0], lo1 = i0 + 1 < kM2wKeep ? keep[i0 + 1] : stash[i0 + 1];
+        const v4i lo0 = i0 < KEEP ? keep[i0] : stash[i0], lo1 = i0 + 1 < KEEP ? keep[i0 + 1] : stash[i0 + 1];
         static_for<4>([&](auto Q) __attribute__((always_inline)) {
             constexpr int q = decltype(Q)::value;
             float m0[9], m1[9];
@@ -2069,7 +2073,7 @@ k_mfma2w(const DevData D, const ScanParams P, TopState *st, const Fp4Image F4, i
     // block order: XCD v % 8 takes super-blocks s0 + 8 i + v % 8, 32 blocks each
     const int64_t k = blockIdx.x >> 3, su = s0 + (k >> 5) * 8 + (blockIdx.x & 7);
     if (su >= s1) return;
-    mfma2w_tile(D, P, st, F4, su, (int)(k & 31), T, lds);
+    mfma2w_tile<kM2wKeep>(D, P, st, F4, su, (int)(k & 31), T, lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -3104,15 +3108,18 @@ k_batch_scan(const BatchJob *__restrict__ jobs, const BatchSeg *__restrict__ seg
 // takes units from queue b % 8; every queue has 1/8 of the workgroups, so all
 // are drained wherever they run).  Queue x's unit v lies in the segment whose
 // start is the last <= v; with l = v - start, order 3: first SNP j0 + l % na,
-// 64 x 64 tile 8 (l / na) + x (the XCD-ordered work of k_mfma3w); order 2:
-// super-block j0 + 8 (l / 32) + x, 64 x 128 tile l % 32 (that of k_mfma2w).
-// Every dataset of these kinds has classes below 2^16 samples.
+// tile 8 (l / na) + x (the XCD-ordered work of k_mfma3w / k_mfma3); order 2:
+// super-block j0 + 8 (l / 32) + x, 64 x 128 tile l % 32 (k_mfma2w), or 8 (l /
+// 64) + x, 64 x 64 tile l % 64 (k_mfma2).  The job's `wide` picks the tiles.
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 2)))
 k_batch_mfma(const BatchJob *__restrict__ jobs, const BatchSeg *__restrict__ segs, BatchHeader *hdr)
 {
-    constexpr size_t kBytes = (size_t)kM2wRing * kM2wStage > (size_t)kM3Ring * kM3wStage
-                                  ? (size_t)kM2wRing * kM2wStage
-                                  : (size_t)kM3Ring * kM3wStage;
+    constexpr size_t kWide = (size_t)kM2wRing * kM2wStage > (size_t)kM3Ring * kM3wStage
+                                 ? (size_t)kM2wRing * kM2wStage
+                                 : (size_t)kM3Ring * kM3wStage;
+    constexpr size_t kNarrow = sizeof(Mfma2Lds<true>) > (size_t)kM3Ring * kM3Stage ? sizeof(Mfma2Lds<true>)
+                                                                                 : (size_t)kM3Ring * kM3Stage;
+    constexpr size_t kBytes = kWide > kNarrow ? kWide : kNarrow;
     __shared__ __attribute__((aligned(1024))) int8_t lds[kBytes];   // the order-2 or the order-3 tiles
     __shared__ uint32_t unit_sh;
     const int x = blockIdx.x & 7;
@@ -3138,11 +3145,18 @@ k_batch_mfma(const BatchJob *__restrict__ jobs, const BatchSeg *__restrict__ seg
         const int64_t j0 = segs[lo].j0, j1 = segs[lo].j1;
         if (J.kind == kKindMfma3) {
             const uint32_t na = (uint32_t)(j1 - j0);
-            mfma3w_tile<kM3Ring>(D, P, st, F, (int)(j0 + l % na), (int64_t)(l / na) * 8 + x, J.T,
-                                 (int8_t(*)[kM3wStage])lds);
-        } else {
+            if (J.wide)
+                mfma3w_tile<kM3Ring>(D, P, st, F, (int)(j0 + l % na), (int64_t)(l / na) * 8 + x, J.T,
+                                     (int8_t(*)[kM3wStage])lds);
+            else
+                mfma3_tile<kM3Ring>(D, P, st, F, (int)(j0 + l % na), (int64_t)(l / na) * 8 + x, J.T,
+                                    (int8_t(*)[kM3Stage])lds);
+        } else if (J.wide) {
             const int64_t su = j0 + (int64_t)(l / 32) * 8 + x;
-            if (su < j1) mfma2w_tile(D, P, st, F, su, (int)(l % 32), J.T, (int8_t(*)[kM2wStage])lds);
+            if (su < j1) mfma2w_tile<kBatchM2wKeep>(D, P, st, F, su, (int)(l % 32), J.T, (int8_t(*)[kM2wStage])lds);
+        } else {
+            const int64_t su = j0 + (int64_t)(l / 64) * 8 + x;
+            if (su < j1) mfma2_tile<true>(D, P, st, F, su, (int)(l % 64), J.T, *(Mfma2Lds<true> *)lds);
         }
     }
 }
