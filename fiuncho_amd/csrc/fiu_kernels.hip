@@ -2557,46 +2557,57 @@ __device__ __forceinline__ void mfma3w_tile(const DevData &D, const ScanParams &
                 acc[x][k >> 2][k & 3][z] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(
                     b8, c8, acc[x][k >> 2][k & 3][z], 4, 4, 0, 0, 0, 0);
             };
+            // the next stage's c and a fragments come from this wave's own
+            // loads: complete once at most those issued after them are pending
+            // (4 stages: stage it + 2's 5 and stage it + 3's 4 early ones; 3
+            // stages: stage it + 2's 4 early ones; where those stages exist)
+            const bool next = steady || it + 1 < iters;
+            const int8_t *sn = lds[(it + 1) % R];
+            v4i na[2], nc[2];
             // x = 0: b fragment k (y = k / 4, block k % 4): the read of
-            // fragment k + 2, the two products
+            // fragment k + 2 (k < 6) or of the next stage's a and c fragments
+            // (k = 6, 7), the two products
 #pragma unroll
             for (int k = 0; k < 8; ++k) {
-                if (k < 6) fb[k + 2] = *(const v4i *)(sb + (k + 2) * kM3Group + fo);
+                if (k < 6) {
+                    fb[k + 2] = *(const v4i *)(sb + (k + 2) * kM3Group + fo);
+                } else if (next) {
+                    if (k == 6) {
+                        if constexpr (R == 4) {
+                            if (steady || it + 3 < iters) wait_vmcnt<9>();
+                            else if (it + 2 < iters) wait_vmcnt<5>();
+                            else wait_vmcnt<0>();
+                        } else {
+                            if (steady || it + 2 < iters) wait_vmcnt<4>();
+                            else wait_vmcnt<0>();
+                        }
+                    }
+                    const int x = k - 6;
+                    na[x] = *(const v4i *)(sn + 16 * kM3Group + 128 * w + x * 64 + ao);
+                    nc[x] = *(const v4i *)(sn + (8 + 4 * x + w) * kM3Group + fo);
+                }
                 product(0, k, 0);
                 product(0, k, 1);
                 __builtin_amdgcn_sched_barrier(0);
             }
-            // the next stage's masks from this wave's own loads: complete once
-            // at most those issued after them are pending (4 stages: stage it +
-            // 2's 5 and stage it + 3's 4 early ones; 3 stages: stage it + 2's 4
-            // early ones; where those stages exist)
-            const bool next = steady || it + 1 < iters;
-            if (next) {
-                if constexpr (R == 4) {
-                    if (steady || it + 3 < iters) wait_vmcnt<9>();
-                    else if (it + 2 < iters) wait_vmcnt<5>();
-                    else wait_vmcnt<0>();
-                } else {
-                    if (steady || it + 2 < iters) wait_vmcnt<4>();
-                    else wait_vmcnt<0>();
-                }
-                masks(it + 1, mn);
-            }
-            __builtin_amdgcn_sched_barrier(0);
+            // x = 1: two products, then two words of the next stage's masks
+            // (nibble 0x2 & 0x2 = 1.0, else 0), issued while the products run
 #pragma unroll
             for (int k = 0; k < 8; ++k) {
                 product(1, k, 0);
                 product(1, k, 1);
-            }
-            if (steady) {
-                // the next stage's 4 reads, then its 16 masks one behind each
-                // of the x = 1 products' first 16
-                __builtin_amdgcn_sched_group_barrier(0x100, 4, 0);
+                if (next) {
 #pragma unroll
-                for (int i = 0; i < 16; ++i) {
-                    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-                    __builtin_amdgcn_sched_group_barrier(0x002, 1, 0);
+                    for (int e = 0; e < 2; ++e) {
+                        const int i = 2 * k + e, x = i >> 3, z = (i >> 2) & 1, d = i & 3;
+                        // the empty asm holds the AND here: left free, the
+                        // compiler sinks all 16 behind the last product
+                        int v = nc[z][d] & na[x][d];
+                        asm volatile("" : "+v"(v));
+                        mn[x][z][d] = v;
+                    }
                 }
+                __builtin_amdgcn_sched_barrier(0);
             }
             if (more) issue_a(it + R - 1);
             __builtin_amdgcn_s_setprio(0);
