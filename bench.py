@@ -19,8 +19,8 @@ Also reported on the same JSON line:
                 durations from HIP events on the library's stream around every
                 scan launch;
   order2        50,000 SNPs x 32,000 samples at order 2: popcount vs int8 vs
-                fp4 matrix-core scans, 64 x 128 (default) and 64 x 64 pairs
-                per workgroup (identical results required);
+                fp4 matrix-core scans, 128 x 128 (default), 64 x 128 and
+                64 x 64 pairs per workgroup (identical results required);
   batched       64 uneven datasets, one search per call vs one batched call;
   serving       a stream of scan requests, plain vs the scan server;
   config1       the plain order-2 search of 1,000 SNPs x 4,000 samples;
@@ -329,8 +329,8 @@ def bench_order2(snps=50_000, half=16_000, outputs=1000):
     """Order-2 exhaustive scan, 50,000 SNPs x 32,000 samples (16,000 /
     16,000), MAF ~ U(0.05, 0.5) with 20% of SNPs ~ U(0.01, 0.05), top-1,000:
     the popcount scan against the matrix-core scans (int8 operands; fp4
-    operands with 64 x 128 pairs per workgroup, the default, and with 64 x
-    64), same dataset, identical results required."""
+    operands with 128 x 128 pairs per workgroup, the default, with 64 x 128
+    and with 64 x 64), same dataset, identical results required."""
     import fiuncho_amd as fa
     spec = fa.synth_spec(snps, half, half, seed=1, low_frac=0.2)
     d = fa.Dataset.synth(spec)
@@ -338,7 +338,8 @@ def bench_order2(snps=50_000, half=16_000, outputs=1000):
     out = {}
     res = {}
     for name, algo, variant in (("popcount", fa.ALGO_MARGINAL, 0), ("mfma_int8", fa.ALGO_MFMA, 1),
-                                ("mfma_64x64", fa.ALGO_MFMA, 2), ("mfma", fa.ALGO_MFMA, 0)):
+                                ("mfma_64x64", fa.ALGO_MFMA, 2), ("mfma_64x128", fa.ALGO_MFMA, 3),
+                                ("mfma", fa.ALGO_MFMA, 0)):
         s = fa.GpuSearch(algorithm=algo, variant=variant)
         s.run_arrays(d, 2, dist, outputs)   # warm
         best = None
@@ -361,14 +362,14 @@ def bench_order2(snps=50_000, half=16_000, outputs=1000):
     out["mfma_int8"]["mfma_tops"] = mfma_ops / (out["mfma_int8"]["scan_ms"] / 1e3) / 1e12
     out["mfma_int8"]["frac_of_peak"] = mfma_ops / (out["mfma_int8"]["scan_ms"] / 1e3) / MFMA_I8_PEAK
     out["mfma_int8"]["peak"] = "int8 dense 5.0 POP/s"
-    for k, tile in (("mfma_64x64", "64 x 64"), ("mfma", "64 x 128")):
+    for k, tile in (("mfma_64x64", "64 x 64"), ("mfma_64x128", "64 x 128"), ("mfma", "128 x 128")):
         out[k]["operands"] = "fp4 (e2m1) x fp4, fp32 accumulation (exact: counts < 2^24)"
         out[k]["pairs_per_workgroup"] = tile
         out[k]["mfma_tops"] = mfma_ops / (out[k]["scan_ms"] / 1e3) / 1e12
         out[k]["frac_of_peak"] = mfma_ops / (out[k]["scan_ms"] / 1e3) / MFMA_FP4_PEAK
         out[k]["peak"] = "fp4 dense 10.1 PFLOP/s"
     same = all(res["popcount"][0].tolist() == res[k][0].tolist() and np.array_equal(res["popcount"][1], res[k][1])
-               for k in ("mfma_int8", "mfma_64x64", "mfma"))
+               for k in ("mfma_int8", "mfma_64x64", "mfma_64x128", "mfma"))
     return {"workload": f"order-2 exhaustive scan, {snps:,} SNPs x {2 * half:,} samples ({half:,}/{half:,}), "
                         "20% of SNPs with MAF < 0.05, top-1,000",
             **out, "int8_kernel_speedup": out["popcount"]["scan_ms"] / out["mfma_int8"]["scan_ms"],

@@ -455,7 +455,7 @@ bool valid_variant(int algo, int order, int variant)
     switch (algo) {
     case FIU_ALGO_MARGINAL:
         return order <= 3 && (variant == 2 || variant == kMarginalPW || (order == 2 && variant == 8));
-    case FIU_ALGO_MFMA: return order == 3 ? (variant >= 3 && variant <= 6) : (variant >= 1 && variant <= 3);
+    case FIU_ALGO_MFMA: return order == 3 ? (variant >= 3 && variant <= 6) : (variant >= 1 && variant <= 4);
     default: return false;
     }
 }
@@ -930,13 +930,15 @@ void search(fiu_dataset *d, Workspace &ws, int order, int32_t step, int32_t offs
     const bool int8 = (cfg && (cfg->flags & FIU_FLAG_MFMA_INT8)) || (order == 2 && variant == 1);
     const bool mfma_fp4 = algo == FIU_ALGO_MFMA && !int8 && small_classes && ws.fp4_image(d, s);
     // order-2 matrix-core variants: 1 int8 operands, 2 fp4 with the 64 x 64
-    // tile, 3 fp4 with the 64 x 128 tile (16-bit counts between the classes:
-    // classes below 2^16 samples, else 64 x 64).  Default: 3 from 16,384
-    // SNPs and 8,192 samples on (measured: faster on 50,000 x 32,000, slower
-    // below), else 2
-    const bool m2_pick_wide = variant == 3 || (variant == 0 && n >= 16384 && samples >= 8192);
-    const bool m2_wide = order == 2 && m2_pick_wide && d->cases < (1u << 16) && d->ctrls < (1u << 16) &&
-                         ws.private_memory();
+    // tile, 3 fp4 with the 64 x 128 tile, 4 fp4 with the 128 x 128 tile (the
+    // last two: 16-bit counts between the classes, so classes below 2^16
+    // samples, else 64 x 64).  Default: 4 from 16,384 SNPs and 8,192 samples
+    // on (measured on 50,000 x 32,000: 59.4 ms per scan, 61.2 with 3, 83.6
+    // with 2; slower below), else 2
+    const bool m2_big = variant == 0 && n >= 16384 && samples >= 8192;
+    const int m2_pick = variant == 4 || m2_big ? 2 : variant == 3 ? 1 : 0;
+    const int m2_tile = order == 2 && d->cases < (1u << 16) && d->ctrls < (1u << 16) && ws.private_memory()
+                            ? m2_pick : 0;
 
     // chunk plan: even chunks of about `target` combinations (at most
     // max_chunks of them); before the first, a sampling pass over the last
@@ -956,7 +958,7 @@ void search(fiu_dataset *d, Workspace &ws, int order, int32_t step, int32_t offs
         // longer (measured on 50,000 x 32,000: 64.0 ms per scan against 69.9
         // with launches of 1,024 super-blocks; 16 per launch: 70.2, the
         // launch gaps and tails eat the gain)
-        if (m2_wide && !(cfg && cfg->chunk_combos)) len = std::min<int64_t>(len, 64);
+        if (m2_tile && !(cfg && cfg->chunk_combos)) len = std::min<int64_t>(len, 64);
         len = std::max<int64_t>(len, (int64_t)(((uint64_t)n_super + max_chunks - 1) / max_chunks));
         plan.set_uniform(n_super, len);
     } else if (algo == FIU_ALGO_MFMA) {
@@ -1034,7 +1036,7 @@ void search(fiu_dataset *d, Workspace &ws, int order, int32_t step, int32_t offs
         }
         if (e0 && e1) check(hipEventRecord(e0, s), "event");
         if (algo == FIU_ALGO_MFMA && order == 2) {
-            check(launch_mfma2(d->dev, P, state, c.j0, c.j1, T, mfma_fp4 ? d->fp4.p : nullptr, m2_wide, s),
+            check(launch_mfma2(d->dev, P, state, c.j0, c.j1, T, mfma_fp4 ? d->fp4.p : nullptr, m2_tile, s),
                   "scan launch");
         } else if (algo == FIU_ALGO_MFMA) {
             check(launch_mfma3(d->dev, P, state, d->fp4.p, (int)c.j0, (int)c.j1, T, ring, s), "scan launch");
@@ -1283,6 +1285,8 @@ fiu_status fiu_set_tuning(const fiu_tune_entry *entries, size_t n)
             if (e.snps_lo >= e.snps_hi || e.samples_lo >= e.samples_hi)
                 fail(FIU_ERR_INVALID, "tuning entry with an empty range");
             if (e.batch_unit > 6) fail(FIU_ERR_INVALID, "tuning entry with a batch unit above 6");
+            if (e.algorithm == FIU_ALGO_MFMA && e.batch_unit > 2)
+                fail(FIU_ERR_INVALID, "matrix-core tuning entry with a batch tile above 2");
         }
         std::lock_guard<std::mutex> lk(g_tuning_mu);
         g_tuning.assign(entries, entries + n);

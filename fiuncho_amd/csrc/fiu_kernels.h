@@ -36,10 +36,11 @@ hipError_t launch_fp4_image(const DevData &D, uint4 *out, hipStream_t s);
 hipError_t launch_mfma3(const DevData &D, const ScanParams &P, TopState *st, const uint4 *fp4, int a0, int a1, int T,
                         int ring, hipStream_t s);
 // Order 2 on the matrix cores: super-blocks [s0, s1) of 8 x 8 tiles, fp4
-// operands from the image `fp4` (int8 when null); `wide`: 64 x 128 pairs per
-// workgroup (fp4, classes below 2^16 samples), else 64 x 64.
+// operands from the image `fp4` (int8 when null); `tile`: pairs per
+// workgroup, 0 = 64 x 64, 1 = 64 x 128, 2 = 128 x 128 (1 and 2: fp4, classes
+// below 2^16 samples).
 hipError_t launch_mfma2(const DevData &D, const ScanParams &P, TopState *st, int64_t s0, int64_t s1, int T,
-                        const uint4 *fp4, bool wide, hipStream_t s);
+                        const uint4 *fp4, int tile, hipStream_t s);
 hipError_t launch_merge(TopState *st, Cand *top, const Cand *cand, uint32_t N, uint32_t cap, uint32_t chunk_id,
                         uint32_t *redo, hipStream_t s);
 // Top-N merge for N above kMaxOutputs, in global memory: the list's n_top

@@ -1818,17 +1818,23 @@ k_mfma2(const DevData D, const ScanParams P, TopState *st, const Fp4Image F4, in
     mfma2_tile<FP4>(D, P, st, F4, su, (int)(k & 63), T, lds);
 }
 
-// Wide order-2 tile: one workgroup takes 64 a x 128 b (the b tiles 2j and 2j
-// + 1 of a super-block row); wave w multiplies all 64 a rows by the 32 b rows
-// b0 + 32 w ...: 16 products per 128-sample stage against 12 fragment reads
-// (8 per 8 in mfma2_tile), and 24 KB staged per 64 products (16 KB per 32).
-// One class's counts fill the 128 accumulators; both classes wait as 16-bit
-// pairs in private memory (every class holds fewer than 2^16 samples), the
-// epilogue scores them in two passes as mfma3w_tile does.  Stage layout: 384
-// rows of 64 bytes, rows g * 64 + i the a rows (genotype g, SNP a0 + i), rows
-// 128 + h * 128 + j the b rows; segments swizzled as in mfma2_tile.
+// Wide order-2 tiles: one workgroup of AT * 4 waves takes AT * 64 a x 128 b
+// (the a tiles AT j .. and the b tiles 2 i, 2 i + 1 of a super-block), AT = 1
+// (k_mfma2w, 64 x 128) or 2 (k_mfma2x, 128 x 128); wave w multiplies the 64 a
+// rows a0 + 64 (w / 4) ... by the 32 b rows b0 + 32 (w % 4) ...: 16 products
+// per 128-sample stage against 12 fragment reads (8 per 8 in mfma2_tile), and
+// 24 KB (AT = 1) or 32 KB (AT = 2) staged per 16 AT * 4 products (16 KB per
+// 32).  One class's counts fill the 128 accumulators; both classes wait as
+// 16-bit pairs in private memory (every class holds fewer than 2^16 samples),
+// the epilogue scores them in two passes as mfma3w_tile does.  Stage layout:
+// (2 AT + 4) * 64 rows of 64 bytes, rows g * 64 AT + i the a rows (genotype
+// g, SNP a0 + i), rows 128 AT + h * 128 + j the b rows; segments swizzled as
+// in mfma2_tile.
 constexpr int kM2wRing = 3;
-constexpr int kM2wStage = 384 * 64;
+// the 128 x 128 tile runs one workgroup per CU: a deeper ring (128 KB)
+constexpr int kM2xRing = 4;
+template <int AT> constexpr int m2w_stage() { return (128 * AT + 256) * 64; }
+constexpr int kM2wStage = m2w_stage<1>();
 // Words of the first class's 16 packed count words that stay in registers
 // through the second class's stages (the rest wait in private memory).
 constexpr int kM2wKeep = 8;
@@ -1850,31 +1856,36 @@ __device__ __forceinline__ v4i pack_counts2(const v16f (&acc)[2][2][2], int ab, 
                  pk(v1[r * 4 + 2], v1[r * 4 + 3])};
 }
 
-template <int KEEP>
+template <int KEEP, int AT = 1, int R = kM2wRing>
 __device__ __forceinline__ void mfma2w_tile(const DevData &D, const ScanParams &P, TopState *st, const Fp4Image &F4,
-                                            int64_t su, int slot, int T, int8_t (*lds)[kM2wStage])
+                                            int64_t su, int slot, int T, int8_t (*lds)[m2w_stage<AT>()])
 {
+    static_assert(AT == 1 || AT == 2, "a tiles per workgroup");
+    // groups of 16 rows per stage and per wave
+    constexpr int kGroups = 8 * AT + 16, kPerWave = kGroups / (4 * AT);
     const int S = (T + kMfmaSuper - 1) / kMfmaSuper;
     int sa, sc;
     tri_tile(su, S, sa, sc);
-    const int ta = sa * kMfmaSuper + (slot >> 2), tb = sc * kMfmaSuper + 2 * (slot & 3);
+    const int ta = sa * kMfmaSuper + AT * (slot >> 2), tb = sc * kMfmaSuper + 2 * (slot & 3);
     if (ta > tb + 1 || ta >= T || tb >= T) return;
     const int tid = threadIdx.x, lane = tid & 63;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wa = w >> 2, wb = w & 3;   // the wave's a half (AT = 2) and b quarter
     const int np = D.n_pad;
     const int a0 = ta * kMfmaTile, b0 = tb * kMfmaTile;
-    // staging: wave w loads the 16-row groups 6 w .. 6 w + 5 of the 24 (group
-    // G: rows 16 G ..); lane l fills row l / 4 of the group, stored segment l
-    // % 4, so it loads the row's segment (l % 4) ^ m2_swz(row).  Offsets
-    // within a quad of the image (b rows past the padded SNPs, on an odd
-    // last tile, read zeros: the range check of the buffer)
+    // staging: wave w loads the 16-row groups kPerWave w .. of the kGroups
+    // (group G: rows 16 G ..); lane l fills row l / 4 of the group, stored
+    // segment l % 4, so it loads the row's segment (l % 4) ^ m2_swz(row).
+    // Offsets within a quad of the image; rows past the padded SNPs (an odd
+    // last tile) read other rows of the quad or, past its end, zeros (the
+    // buffer's range check): the epilogue drops their instances (a < b < n)
     const int lr = lane >> 2, sg = (lane & 3) ^ m2_swz(lr);
-    uint32_t vo[6];
+    uint32_t vo[kPerWave];
 #pragma unroll
-    for (int t = 0; t < 6; ++t) {
-        const int G = 6 * w + t;
-        const int gen = G < 8 ? G >> 2 : (G - 8) >> 3;
-        const int snp = G < 8 ? a0 + 16 * (G & 3) : b0 + 16 * ((G - 8) & 7);
+    for (int t = 0; t < kPerWave; ++t) {
+        const int G = kPerWave * w + t;
+        const int gen = G < 8 * AT ? G / (4 * AT) : (G - 8 * AT) >> 3;
+        const int snp = G < 8 * AT ? a0 + 16 * (G % (4 * AT)) : b0 + 16 * ((G - 8 * AT) & 7);
         vo[t] = 16u * (uint32_t)(4 * (gen * np + snp + lr) + sg);
     }
     v16f acc[2][2][2];   // [a block][g][h], the class being scanned
@@ -1901,23 +1912,23 @@ __device__ __forceinline__ void mfma2w_tile(const DevData &D, const ScanParams &
         auto issue = [&](int q) __attribute__((always_inline)) {
             const __amdgpu_buffer_rsrc_t rq =
                 __builtin_amdgcn_make_buffer_rsrc((void *)(img + (int64_t)q * 8 * np), (short)0, 128 * np, 0x00020000);
-            int8_t *s = lds[q % kM2wRing];
+            int8_t *s = lds[q % R];
 #pragma unroll
-            for (int t = 0; t < 6; ++t) dma16(rq, vo[t], s + (6 * w + t) * 1024);
+            for (int t = 0; t < kPerWave; ++t) dma16(rq, vo[t], s + (kPerWave * w + t) * 1024);
         };
 #pragma unroll
-        for (int q = 0; q < kM2wRing - 1; ++q)
+        for (int q = 0; q < R - 1; ++q)
             if (q < iters) issue(q);
         auto stage = [&](int it, bool steady) __attribute__((always_inline)) {
-            // 6 loads per stage; stage it is complete once at most those of
-            // the later stages issued are pending, the barrier publishes every
-            // wave's part
-            static_assert(6 * (kM2wRing - 2) < 16, "ring depth");
-            if (steady) wait_vmcnt<6 * (kM2wRing - 2)>();
-            else wait_stages<kM2wRing - 2, 6>(iters - 1 - it);
+            // kPerWave loads per stage; stage it is complete once at most
+            // those of the later stages issued are pending, the barrier
+            // publishes every wave's part
+            static_assert(kPerWave * (R - 2) < 16, "ring depth");
+            if (steady) wait_vmcnt<kPerWave * (R - 2)>();
+            else wait_stages<R - 2, kPerWave>(iters - 1 - it);
             stage_barrier();
             __builtin_amdgcn_s_setprio(1);
-            const int8_t *sl = lds[it % kM2wRing];
+            const int8_t *sl = lds[it % R];
             // substep s (64 samples): the b fragments of genotypes 0 and 1
             // and the a fragments (g, a block); reads of substep 1 issue
             // between the products of substep 0 (12 reads in a row overfill
@@ -1927,12 +1938,12 @@ __device__ __forceinline__ void mfma2w_tile(const DevData &D, const ScanParams &
                 return *(const v4i *)(sl + (row0 + r32) * 64 + (((2 * s + (lane >> 5)) ^ m2_swz(r32)) << 4));
             };
 #pragma unroll
-            for (int h = 0; h < 2; ++h) fb[0][h] = rd(0, 128 + h * 128 + 32 * w);
+            for (int h = 0; h < 2; ++h) fb[0][h] = rd(0, 128 * AT + h * 128 + 32 * wb);
 #pragma unroll
             for (int g = 0; g < 2; ++g)
 #pragma unroll
-                for (int ab = 0; ab < 2; ++ab) fa[0][g][ab] = rd(0, g * 64 + 32 * ab);
-            if (steady || it + kM2wRing - 1 < iters) issue(it + kM2wRing - 1);
+                for (int ab = 0; ab < 2; ++ab) fa[0][g][ab] = rd(0, g * 64 * AT + 64 * wa + 32 * ab);
+            if (steady || it + R - 1 < iters) issue(it + R - 1);
             __builtin_amdgcn_sched_barrier(0);
             auto product = [&](int s, int ab, int g, int h) __attribute__((always_inline)) {
                 const v8i a8 = {fa[s][g][ab][0], fa[s][g][ab][1], fa[s][g][ab][2], fa[s][g][ab][3], 0, 0, 0, 0};
@@ -1944,8 +1955,8 @@ __device__ __forceinline__ void mfma2w_tile(const DevData &D, const ScanParams &
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 const int g = k >> 1, ab = k & 1;
-                if (k < 2) fb[1][k] = rd(1, 128 + k * 128 + 32 * w);
-                fa[1][k >> 1][k & 1] = rd(1, (k >> 1) * 64 + 32 * (k & 1));
+                if (k < 2) fb[1][k] = rd(1, 128 * AT + k * 128 + 32 * wb);
+                fa[1][k >> 1][k & 1] = rd(1, (k >> 1) * 64 * AT + 64 * wa + 32 * (k & 1));
                 product(0, ab, g, 0);
                 product(0, ab, g, 1);
                 __builtin_amdgcn_sched_barrier(0);
@@ -1958,7 +1969,7 @@ __device__ __forceinline__ void mfma2w_tile(const DevData &D, const ScanParams &
             __builtin_amdgcn_s_setprio(0);
         };
         int it = 0;
-        for (; it + kM2wRing - 1 < iters; ++it) stage(it, true);
+        for (; it + R - 1 < iters; ++it) stage(it, true);
         for (; it < iters; ++it) stage(it, false);
         // the next class's first loads reuse the ring
         stage_barrier();
@@ -1974,14 +1985,14 @@ __device__ __forceinline__ void mfma2w_tile(const DevData &D, const ScanParams &
                         else stash[i] = pack_counts2(acc, ab, r, half);
                     }
     }
-    // epilogue: lane holds column b = b0 + 32 w + lane % 32 and, per a block
-    // ab, rows a = a0 + 32 ab + 8 r + 4 (lane / 32) + q, r, q < 4.  First
+    // epilogue: lane holds column b = b0 + 32 wb + lane % 32 and, per a block
+    // ab, rows a = a0 + 64 wa + 32 ab + 8 r + 4 (lane / 32) + q, r, q < 4.  First
     // pass: the bound of each of the 32 instances (ab, r, q), the passing
     // ones marked in `pend` (bit 16 ab + 4 r + q) and their counts kept in
     // `marked`; second pass: one marked instance per lane per round
     const float tau = st->tau;
     const uint64_t tau_combo = st->tau_combo;
-    const int b = b0 + 32 * w + r32;
+    const int b = b0 + 32 * wb + r32;
     const bool b_ok = b < D.n;
     uint32_t sb[2][2] = {{0, 0}, {0, 0}};
     if (b_ok) {
@@ -2019,7 +2030,7 @@ __device__ __forceinline__ void mfma2w_tile(const DevData &D, const ScanParams &
             constexpr int q = decltype(Q)::value;
             float m0[9], m1[9];
             const bool valid = cells(
-                a0 + 32 * ab + 8 * r + 4 * (lane >> 5) + q,
+                a0 + 64 * wa + 32 * ab + 8 * r + 4 * (lane >> 5) + q,
                 [&](int cls, int g, int h) __attribute__((always_inline)) {
                     if (cls) return acc[ab][g][h][r * 4 + q];
                     const v4i &lo = g ? lo1 : lo0;
@@ -2049,7 +2060,7 @@ __device__ __forceinline__ void mfma2w_tile(const DevData &D, const ScanParams &
         if (pend) {
             const int i = __builtin_ctz(pend), ab = i >> 4, r = (i >> 2) & 3, q = i & 3;
             pend &= pend - 1;
-            const int a = a0 + 32 * ab + 8 * r + 4 * (lane >> 5) + q;
+            const int a = a0 + 64 * wa + 32 * ab + 8 * r + 4 * (lane >> 5) + q;
             const v4i wd = marked[taken++];
             float m0[9], m1[9];
             cells(
@@ -2074,6 +2085,17 @@ k_mfma2w(const DevData D, const ScanParams P, TopState *st, const Fp4Image F4, i
     const int64_t k = blockIdx.x >> 3, su = s0 + (k >> 5) * 8 + (blockIdx.x & 7);
     if (su >= s1) return;
     mfma2w_tile<kM2wKeep>(D, P, st, F4, su, (int)(k & 31), T, lds);
+}
+
+// The 128 x 128 tile: 8 waves, 16 workgroups per super-block
+__global__ void __launch_bounds__(2 * kBlock) __attribute__((amdgpu_waves_per_eu(2, 2)))
+k_mfma2x(const DevData D, const ScanParams P, TopState *st, const Fp4Image F4, int64_t s0, int64_t s1, int T)
+{
+    __shared__ __attribute__((aligned(1024))) int8_t lds[kM2xRing][m2w_stage<2>()];
+    // block order: XCD v % 8 takes super-blocks s0 + 8 i + v % 8, 16 blocks each
+    const int64_t k = blockIdx.x >> 3, su = s0 + (k >> 4) * 8 + (blockIdx.x & 7);
+    if (su >= s1) return;
+    mfma2w_tile<kM2wKeep, 2, kM2xRing>(D, P, st, F4, su, (int)(k & 15), T, lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -3582,28 +3604,30 @@ int64_t mfma2_superblocks(int T)
 }
 
 hipError_t launch_mfma2(const DevData &D, const ScanParams &P, TopState *st, int64_t s0, int64_t s1, int T,
-                        const uint4 *fp4, bool wide, hipStream_t s)
+                        const uint4 *fp4, int tile, hipStream_t s)
 {
     if (s1 <= s0) return hipSuccess;
-    wide = wide && fp4;
-    const int per = wide ? 32 : 64;   // blocks per super-block
-    // a grid below 2^32 work-items: sub-launches of at most this many super-blocks
-    const int64_t max_super = kMaxGridBlocks / (8 * per) * 8;
+    if (!fp4) tile = 0;
+    const int per = tile == 2 ? 16 : tile == 1 ? 32 : 64;   // blocks per super-block
+    // a grid below 2^32 work-items (blocks of 2 kBlock for the 128 x 128
+    // tile): sub-launches of at most this many super-blocks
+    const int64_t max_super = kMaxGridBlocks / (8 * per * (tile == 2 ? 2 : 1)) * 8;
     if (s1 - s0 > max_super) {
         for (int64_t a = s0; a < s1; a += max_super) {
-            const hipError_t e = launch_mfma2(D, P, st, a, std::min(s1, a + max_super), T, fp4, wide, s);
+            const hipError_t e = launch_mfma2(D, P, st, a, std::min(s1, a + max_super), T, fp4, tile, s);
             if (e != hipSuccess) return e;
         }
         return hipSuccess;
     }
     // `per` blocks per super-block, 8 super-blocks (one per XCD) per round
     const int64_t rounds = (s1 - s0 + 7) / 8;
-    const dim3 grid((unsigned)(rounds * 8 * per)), block(kBlock);
+    const dim3 grid((unsigned)(rounds * 8 * per)), block(tile == 2 ? 2 * kBlock : kBlock);
     Fp4Image F;
     F.base = fp4;
     F.class_off[0] = 0;
     F.class_off[1] = fp4_image_units(D, 0);
-    if (wide) hipLaunchKernelGGL(k_mfma2w, grid, block, 0, s, D, P, st, F, s0, s1, T);
+    if (tile == 2) hipLaunchKernelGGL(k_mfma2x, grid, block, 0, s, D, P, st, F, s0, s1, T);
+    else if (tile == 1) hipLaunchKernelGGL(k_mfma2w, grid, block, 0, s, D, P, st, F, s0, s1, T);
     else if (fp4) hipLaunchKernelGGL((k_mfma2<true>), grid, block, 0, s, D, P, st, F, s0, s1, T);
     else hipLaunchKernelGGL((k_mfma2<false>), grid, block, 0, s, D, P, st, F, s0, s1, T);
     return hipGetLastError();

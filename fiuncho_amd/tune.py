@@ -9,10 +9,11 @@ candidate to return the identical top-N list, and keeps the fastest.  The
 table is loaded into the library, where AUTO searches and batched scans of
 that class use it.  ``autotune_batch`` then times ``fiu_search_batch`` over a
 few datasets of each batch class with every unit size of the batched
-popcount kernels (the entry's ``batch_unit``) and keeps the fastest.
+popcount kernels, or both batched tiles of the matrix-core kernels (the
+entry's ``batch_unit``), and keeps the fastest.
 
     entries = tune.autotune()          # times the default classes, loads the table
-    entries = tune.autotune_batch()    # adds the batched kernels' unit sizes
+    entries = tune.autotune_batch()    # adds the batched kernels' unit sizes / tiles
     tune.save(entries, "tuning.json")  # later: tune.load("tuning.json")
 """
 from __future__ import annotations
@@ -42,7 +43,8 @@ class Entry:
     algorithm: int
     variant: int = 0
     ms: float = 0.0
-    batch_unit: int = 0   # batched popcount kinds: 0 = default unit, k = default x 2^(k-3)
+    batch_unit: int = 0   # batched popcount kinds: 0 = default unit, k = default x 2^(k-3);
+                          # matrix cores: batched tile, 0 = the variant's, 1 = narrow, 2 = wide
 
 
 @dataclass
@@ -74,7 +76,7 @@ DEFAULT_CLASSES: List[ShapeClass] = (
 def candidates(order: int) -> List[Tuple[int, int]]:
     """(algorithm, variant) pairs the tuner times at `order`."""
     if order == 2:
-        return [(ALGO_MFMA, 1), (ALGO_MFMA, 2), (ALGO_MFMA, 3), (ALGO_MARGINAL, 2), (ALGO_MARGINAL, 4),
+        return [(ALGO_MFMA, 1), (ALGO_MFMA, 2), (ALGO_MFMA, 3), (ALGO_MFMA, 4), (ALGO_MARGINAL, 2), (ALGO_MARGINAL, 4),
                 (ALGO_MARGINAL, 8)]
     if order == 3:
         return [(ALGO_MFMA, 4), (ALGO_MFMA, 5), (ALGO_MFMA, 6), (ALGO_MARGINAL, 2), (ALGO_MARGINAL, 4)]
@@ -188,21 +190,26 @@ class BatchClass:
     algorithm: int = ALGO_MARGINAL
 
 
-# Orders 4 and 5, the batched scans that run on the popcount kinds.
+# Orders 4 and 5, the batched scans that run on the popcount kinds, and
+# orders 2 and 3 at the sizes where the matrix cores' two tiles compete.
 DEFAULT_BATCH_CLASSES: List[BatchClass] = [
     BatchClass(4, 0, 512, 0, 8192, 200, 3000),
     BatchClass(4, 0, 512, 8192, INF, 160, 16000),
     BatchClass(5, 0, 128, 0, INF, 50, 4000),
+    BatchClass(3, 2048, INF, 8192, INF, 2500, 16000, count=3, algorithm=ALGO_MFMA),
+    BatchClass(2, 16384, INF, 8192, INF, 20000, 16000, count=3, algorithm=ALGO_MFMA),
 ]
 BATCH_UNITS = (0, 1, 2, 4, 5, 6)
+BATCH_TILES = (1, 2)   # matrix-core classes: narrow, wide
 
 
 def autotune_batch(classes: Iterable[BatchClass] = DEFAULT_BATCH_CLASSES, units: Sequence[int] = BATCH_UNITS,
                    outputs: int = 50, repeats: int = 2, seed: int = 11, apply: bool = True, log=None) -> List[Entry]:
-    """Time fiu_search_batch over each class's datasets with every unit size;
-    the fastest goes into the class's table entry (added when the table has
-    none for it).  Returns the whole table (loaded when `apply`).  Raises if
-    two unit sizes disagree on a result."""
+    """Time fiu_search_batch over each class's datasets with every unit size
+    (popcount kinds) or both tiles (a class whose entry is on the matrix
+    cores); the fastest goes into the class's table entry (added when the
+    table has none for it).  Returns the whole table (loaded when `apply`).
+    Raises if two candidates disagree on a result."""
     from . import Dataset, Distribution, search_batch, synth_spec
     saved = get_table()
     table = list(saved)
@@ -220,7 +227,7 @@ def autotune_batch(classes: Iterable[BatchClass] = DEFAULT_BATCH_CLASSES, units:
             scans = [(d, bc.order, Distribution(bc.snps, bc.order - 1), outputs) for d in dss]
             best = None
             ref = None
-            for u in units:
+            for u in (BATCH_TILES if base.algorithm == ALGO_MFMA else units):
                 entry = Entry(base.order, base.snps_lo, base.snps_hi, base.samples_lo, base.samples_hi,
                               base.algorithm, base.variant, base.ms, u)
                 set_table(rest + [entry])
@@ -236,7 +243,8 @@ def autotune_batch(classes: Iterable[BatchClass] = DEFAULT_BATCH_CLASSES, units:
                 elif got != ref:
                     raise RuntimeError(f"batch unit {u} disagrees on class {bc}")
                 if log:
-                    log(f"[tune] batch order {bc.order} {bc.count} x {bc.snps} x {bc.samples}: unit {u} "
+                    log(f"[tune] batch order {bc.order} {bc.count} x {bc.snps} x {bc.samples}: "
+                        f"{'tile' if base.algorithm == ALGO_MFMA else 'unit'} {u} "
                         f"{t * 1e3:.2f} ms")
                 if best is None or t < best[0]:
                     best = (t, u)

@@ -298,19 +298,23 @@ fiu_status fiu_write_results(const char *path, const int32_t *combos,
  *                      to 4 when a class holds 2^16 samples or more);
  *   FIU_ALGO_MFMA, order 2: 1 = int8 operands, 2 = fp4 operands, 64 x 64
  *                      pairs per workgroup, 3 = fp4 operands, 64 x 128 pairs
- *                      per workgroup (64 x 64 when a class holds 2^16
- *                      samples or more); default: 3 from 16,384 SNPs and
+ *                      per workgroup, 4 = fp4 operands, 128 x 128 pairs per
+ *                      workgroup (3 and 4: 64 x 64 when a class holds 2^16
+ *                      samples or more); default: 4 from 16,384 SNPs and
  *                      8,192 samples on, else 2.
  * The table is written by an autotuner that times the candidates on the
  * device (fiuncho_amd.tune); it is process-wide and starts empty.  Batched
- * scans use its algorithm and, for the popcount kinds (orders 2-5 off the
- * matrix cores), its batch_unit: the work of one unit of the persistent
- * kernels, 0 = the default (about 40,000 wave instructions: that many
- * 64-SNP tiles of last SNPs per prefix group), k = 1..6 = the default times
- * 2^(k-3) (a quarter to 8 times). */
+ * scans use its algorithm and its batch_unit.  For the popcount kinds
+ * (orders 2-5 off the matrix cores) batch_unit is the work of one unit of the
+ * persistent kernels, 0 = the default (about 40,000 wave instructions: that
+ * many 64-SNP tiles of last SNPs per prefix group), k = 1..6 = the default
+ * times 2^(k-3) (a quarter to 8 times).  For FIU_ALGO_MFMA entries it is the
+ * batched tile: 0 = the one the variant names (the size rule for variant 0),
+ * 1 = 64 x 32 (order 3) / 64 x 64 (order 2), 2 = 64 x 64 / 64 x 128 (classes
+ * below 2^16 samples); above 2 is FIU_ERR_INVALID. */
 typedef struct {
     uint16_t order;
-    uint16_t batch_unit;              /* batched popcount kinds' unit size, 0..6 */
+    uint16_t batch_unit;              /* batched kinds' unit size (popcount) or tile (MFMA) */
     uint32_t snps_lo, snps_hi;        /* SNPs in [lo, hi) */
     uint32_t samples_lo, samples_hi;  /* cases + ctrls in [lo, hi) */
     int32_t algorithm;                /* fiu_algorithm (not AUTO) */

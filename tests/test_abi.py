@@ -107,7 +107,7 @@ def test_tuning_table_roundtrip_and_validation():
     good = [E(2, 0, 1000, 0, 4096, fa.ALGO_MARGINAL, 8, 1.5), E(3, 100, 200, 10, 20, fa.ALGO_MFMA, 3),
             E(4, 0, 50, 0, 100, fa.ALGO_DIRECT, 0), E(5, 0, 50, 0, 100, fa.ALGO_MARGINAL, 0),
             E(7, 0, 50, 0, 100, fa.ALGO_DIRECT, 0), E(6, 0, 50, 0, 100, fa.ALGO_DIRECT, 0),
-            E(4, 50, 500, 0, 100, fa.ALGO_MARGINAL, 0, 0.0, 6)]
+            E(4, 50, 500, 0, 100, fa.ALGO_MARGINAL, 0, 0.0, 6), E(2, 50, 500, 0, 100, fa.ALGO_MFMA, 2, 0.0, 2)]
     try:
         tune.set_table(good)
         got = tune.get_table()
@@ -116,7 +116,7 @@ def test_tuning_table_roundtrip_and_validation():
         assert abs(got[0].ms - 1.5) < 1e-6
         assert [e.batch_unit for e in got] == [e.batch_unit for e in good]
         bad = [E(3, 0, 10, 0, 10, fa.ALGO_MARGINAL, 8),     # 8 prefixes per wave: order 2 only
-               E(2, 0, 10, 0, 10, fa.ALGO_MFMA, 4),         # ring depth: order 3 only
+               E(2, 0, 10, 0, 10, fa.ALGO_MFMA, 5),         # ring depth: order 3 only
                E(3, 0, 10, 0, 10, fa.ALGO_MFMA, 7),         # no such tile / staging variant
                E(4, 0, 10, 0, 10, fa.ALGO_MFMA, 0),         # matrix cores: orders 2-3
                E(13, 0, 10, 0, 10, fa.ALGO_DIRECT, 0),      # orders 2-12 only
@@ -126,7 +126,8 @@ def test_tuning_table_roundtrip_and_validation():
                E(2, 10, 10, 0, 10, fa.ALGO_DIRECT, 0),      # empty SNP range
                E(2, 0, 10, 0, 10, fa.ALGO_AUTO, 0),         # AUTO is not a choice
                E(1, 0, 10, 0, 10, fa.ALGO_DIRECT, 0),
-               E(4, 0, 10, 0, 10, fa.ALGO_MARGINAL, 0, 0.0, 7)]     # batch unit 0..6
+               E(4, 0, 10, 0, 10, fa.ALGO_MARGINAL, 0, 0.0, 7),     # batch unit 0..6
+               E(3, 0, 10, 0, 10, fa.ALGO_MFMA, 0, 0.0, 3)]         # batch tile 0..2
         for b in bad:
             try:
                 tune.set_table([b])

@@ -2,9 +2,9 @@
 
 The marginal pair counts come from indicator dot products on the matrix cores
 -- int8 operands with int32 accumulation, or fp4 (e2m1) operands with fp32
-accumulation, both exact at these sizes; fp4 tiles of 64 x 128 pairs per
-workgroup (counts kept as 16-bit pairs between the classes, two-pass
-epilogue) or 64 x 64; everything downstream is the popcount scan's.  Bar: the same
+accumulation, both exact at these sizes; fp4 tiles of 128 x 128 or 64 x 128
+pairs per workgroup (counts kept as 16-bit pairs between the classes,
+two-pass epilogue) or 64 x 64; everything downstream is the popcount scan's.  Bar: the same
 top-N (combinations, order and float MI bit for bit) as the oracle's FP64
 ranking and as the popcount scan.
 """
@@ -22,8 +22,8 @@ pytestmark = pytest.mark.gpu
 
 
 # variant 0: the default (fp4, 64 x 64 pairs per workgroup at these sizes); 1:
-# int8 operands; 2: fp4, 64 x 64; 3: fp4, 64 x 128
-@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+# int8 operands; 2: fp4, 64 x 64; 3: fp4, 64 x 128; 4: fp4, 128 x 128
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4])
 @pytest.mark.parametrize("snps,cases,ctrls,outputs,seed", [
     (70, 129, 63, 40, 21),       # one partial tile, ragged classes
     (128, 512, 512, 100, 22),    # two full tiles per side
@@ -47,7 +47,7 @@ def test_mfma_matches_popcount_scan():
     print("case: mfma vs popcount 1500 SNPs", flush=True)
     d, _ = synth(1500, 3000, 2500, 31, planted=(100, 1200))
     ref = fa.GpuSearch(algorithm=fa.ALGO_MARGINAL).run_arrays(d, 2, fa.Distribution(1500, 1), 500)
-    for int8, variant in ((False, 0), (True, 0), (False, 2), (False, 3)):
+    for int8, variant in ((False, 0), (True, 0), (False, 2), (False, 3), (False, 4)):
         c, m = fa.GpuSearch(algorithm=fa.ALGO_MFMA, mfma_int8=int8, variant=variant).run_arrays(
             d, 2, fa.Distribution(1500, 1), 500)
         assert c.tolist() == ref[0].tolist() and np.array_equal(m, ref[1])

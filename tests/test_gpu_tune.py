@@ -105,3 +105,37 @@ def test_batch_unit_sizes_keep_results():
         tune.clear()
         for d in dss:
             d.close()
+
+
+def test_batch_tiles_keep_results():
+    # a matrix-core entry's batch tile (batch_unit 1 = narrow, 2 = wide)
+    # changes which tile the batched scans run, not what they find; small
+    # datasets (odd tile counts, ragged classes) take the wide tiles only this
+    # way
+    print("case: batch tiles, orders 2 and 3", flush=True)
+    sets = [synth(300, 900, 700, 150), synth(130, 1000, 1100, 151), synth(700, 800, 900, 152),
+            synth(517, 300, 1300, 153)]
+    scans = [(sets[0][0], 3, fa.Distribution(300, 2), 40), (sets[1][0], 3, fa.Distribution(130, 2), 40),
+             (sets[2][0], 2, fa.Distribution(700, 1), 60), (sets[3][0], 2, fa.Distribution(517, 1), 60)]
+    want = [fa.GpuSearch().run_arrays(*sc) for sc in scans]
+    for k, (d, od) in enumerate(sets[:2]):
+        oc, om, _ = O.search(od, 3, 40, threads=4, mi_mode=2)
+        assert want[k][0].tolist() == oc.tolist() and np.array_equal(want[k][1], om)
+    E = tune.Entry
+    try:
+        for tile in (1, 2):
+            tune.set_table([E(3, 0, 100000, 0, 1 << 20, fa.ALGO_MFMA, 0, 0.0, tile),
+                            E(2, 0, 100000, 0, 1 << 20, fa.ALGO_MFMA, 0, 0.0, tile)])
+            got, _ = fa.search_batch(scans)
+            for (c, m), (wc, wm), sc in zip(got, want, scans):
+                assert c.tolist() == wc.tolist() and np.array_equal(m, wm), (tile, sc[0].snps, sc[1])
+        # the tuner times both tiles of a matrix-core class and keeps one
+        tune.clear()
+        table = tune.autotune_batch([tune.BatchClass(3, 0, 1000, 0, 100000, 300, 1600, count=2,
+                                                     algorithm=fa.ALGO_MFMA)], repeats=1)
+        entry = [e for e in table if e.order == 3]
+        assert len(entry) == 1 and entry[0].algorithm == fa.ALGO_MFMA and entry[0].batch_unit in (1, 2)
+    finally:
+        tune.clear()
+        for d, _ in sets:
+            d.close()
