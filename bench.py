@@ -53,7 +53,7 @@ VALU_PEAK = 256 * 64 * 2.4e9          # int32 VALU lane-ops per second (MI355X)
 # WRITE_SIZE x 1.0, divided by the triples of those dispatches
 # (profiles/k_scan3m_order3_10000x16000.md, profiles/k_mfma3w_order3_10000x16000.md;
 # k_mfma3, the 64 x 32 tile of variants 3 / 4, from its own earlier profile).
-TRAFFIC_BYTES_PER_TRIPLE = {"k_scan3m": 1563.0, "k_mfma3w": 89.7, "k_mfma3": 47.7}
+TRAFFIC_BYTES_PER_TRIPLE = {"k_scan3m": 1563.0, "k_mfma3w": 89.4, "k_mfma3": 47.7}
 
 
 def alg_ops_per_combo(order, cases, ctrls):
@@ -198,7 +198,8 @@ def combo_samples(sp):
 def bench_batch(count=64):
     """64 uneven datasets (seeds 100..163), top-50 each: every search one call
     (plain) against all of them in one fiu_search_batch (batched).  Datasets
-    are resident and their tables prepared before either is timed."""
+    are resident and their tables prepared before either is timed; each form
+    runs twice and its faster pass is reported."""
     import fiuncho_amd as fa
     specs = uneven_specs(count, 100, BATCH_SCALE)
     dss = []
@@ -211,15 +212,22 @@ def bench_batch(count=64):
         dss.append(d)
     print(f"[bench] batch: {count} datasets resident", file=sys.stderr, flush=True)
     scans = [(d, sp["order"], fa.Distribution(sp["snps"], sp["order"] - 1), BATCH_OUTPUTS) for d, sp in zip(dss, specs)]
-    t0 = time.perf_counter()
-    plain = [fa.GpuSearch().run_arrays(*s) for s in scans]
-    t_plain = time.perf_counter() - t0
-    print(f"[bench] batch: plain {t_plain:.2f} s", file=sys.stderr, flush=True)
-    t0 = time.perf_counter()
-    batched, st = fa.search_batch(scans)
-    t_batch = time.perf_counter() - t0
-    print(f"[bench] batch: batched {t_batch:.2f} s", file=sys.stderr, flush=True)
-    same = all(a[0].tolist() == b[0].tolist() and np.array_equal(a[1], b[1]) for a, b in zip(plain, batched))
+    # two passes of each form, alternating; the faster of each is kept (the
+    # first batched call also captures its step graph)
+    passes = {"plain": [], "batched": []}
+    same = True
+    for _ in range(2):
+        t0 = time.perf_counter()
+        plain = [fa.GpuSearch().run_arrays(*s) for s in scans]
+        passes["plain"].append(time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        batched, st = fa.search_batch(scans)
+        passes["batched"].append(time.perf_counter() - t0)
+        print(f"[bench] batch: plain {passes['plain'][-1]:.2f} s, batched {passes['batched'][-1]:.2f} s",
+              file=sys.stderr, flush=True)
+        same = same and all(a[0].tolist() == b[0].tolist() and np.array_equal(a[1], b[1])
+                            for a, b in zip(plain, batched))
+    t_plain, t_batch = min(passes["plain"]), min(passes["batched"])
     work = sum(combo_samples(sp) for sp in specs)
     orders = [sp["order"] for sp in specs]
     for d in dss:
@@ -228,7 +236,7 @@ def bench_batch(count=64):
                         "order 3 with 200..1,500 SNPs, order 4 (every 16th) with 100..300 SNPs and "
                         "1,000..5,000 samples; 1,000..50,000 samples, case fraction 0.3..0.7, top-50",
             "orders": {str(k): orders.count(k) for k in (2, 3, 4)},
-            "combo_samples": work, "plain_s": t_plain, "batched_s": t_batch,
+            "combo_samples": work, "plain_s": t_plain, "batched_s": t_batch, "passes_s": passes,
             "speedup": t_plain / t_batch, "batched_combo_samples_per_s": work / t_batch,
             "steps": st.steps, "device_s": st.device_ms / 1e3, "results_identical": same}
 

@@ -367,13 +367,16 @@ void Engine::plan_job(Job *j)
         (order == 2 || (order == 3 && d->pair_ready && j->step == 1))) {
         j->kind = order == 2 ? kKindMfma2 : kKindMfma3;
         // the tile: a tuning entry's batch tile (batch_unit 1 = the 64 x 32 /
-        // 64 x 64 tiles, 2 = the wide ones), else its variant, else the sizes
-        // where the single search takes the wide tiles (their counts wait as
-        // 16-bit pairs: classes below 2^16 samples)
+        // 64 x 64 tiles, 2 = the wide ones), else the sizes where the single
+        // search takes the wide tiles (their counts wait as 16-bit pairs:
+        // classes below 2^16 samples).  The entry's variant, timed on single
+        // searches, is not used: on the 64-dataset batch, wide tiles for
+        // the order-3 datasets of 512-2,048 SNPs that the single-search
+        // tuner puts on them made the batch 9 % slower (finer units balance
+        // the persistent kernel better)
         const uint64_t samples = (uint64_t)d->cases + d->ctrls;
         bool wide = samples >= 8192 && (uint64_t)n >= (order == 3 ? 2048u : 16384u);
         if (tuned_algo == FIU_ALGO_MFMA && te.batch_unit) wide = te.batch_unit == 2;
-        else if (tuned_algo == FIU_ALGO_MFMA && te.variant) wide = order == 3 ? te.variant >= 5 : te.variant >= 3;
         j->wide = wide && d->cases < (1u << 16) && d->ctrls < (1u << 16);
         j->pw = 1;
         j->n_pref = order == 2 ? mfma2_superblocks(T) : std::max(0, n - 2);

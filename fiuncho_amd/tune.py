@@ -44,7 +44,7 @@ class Entry:
     variant: int = 0
     ms: float = 0.0
     batch_unit: int = 0   # batched popcount kinds: 0 = default unit, k = default x 2^(k-3);
-                          # matrix cores: batched tile, 0 = the variant's, 1 = narrow, 2 = wide
+                          # matrix cores: batched tile, 0 = by size, 1 = narrow, 2 = wide
 
 
 @dataclass
@@ -196,6 +196,7 @@ DEFAULT_BATCH_CLASSES: List[BatchClass] = [
     BatchClass(4, 0, 512, 0, 8192, 200, 3000),
     BatchClass(4, 0, 512, 8192, INF, 160, 16000),
     BatchClass(5, 0, 128, 0, INF, 50, 4000),
+    BatchClass(3, 512, 2048, 8192, INF, 1000, 16000, count=4, algorithm=ALGO_MFMA),
     BatchClass(3, 2048, INF, 8192, INF, 2500, 16000, count=3, algorithm=ALGO_MFMA),
     BatchClass(2, 16384, INF, 8192, INF, 20000, 16000, count=3, algorithm=ALGO_MFMA),
 ]

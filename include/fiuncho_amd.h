@@ -309,9 +309,10 @@ fiu_status fiu_write_results(const char *path, const int32_t *combos,
  * persistent kernels, 0 = the default (about 40,000 wave instructions: that
  * many 64-SNP tiles of last SNPs per prefix group), k = 1..6 = the default
  * times 2^(k-3) (a quarter to 8 times).  For FIU_ALGO_MFMA entries it is the
- * batched tile: 0 = the one the variant names (the size rule for variant 0),
- * 1 = 64 x 32 (order 3) / 64 x 64 (order 2), 2 = 64 x 64 / 64 x 128 (classes
- * below 2^16 samples); above 2 is FIU_ERR_INVALID. */
+ * batched tile: 0 = by size (64 x 64 / 64 x 128 from 2,048 / 16,384 SNPs and
+ * 8,192 samples on, as the single search's default), 1 = 64 x 32 (order 3)
+ * / 64 x 64 (order 2), 2 = 64 x 64 / 64 x 128 (classes below 2^16 samples);
+ * above 2 is FIU_ERR_INVALID.  The variant does not steer batched scans. */
 typedef struct {
     uint16_t order;
     uint16_t batch_unit;              /* batched kinds' unit size (popcount) or tile (MFMA) */
