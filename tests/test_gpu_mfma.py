@@ -54,20 +54,25 @@ def test_mfma_matches_popcount_scan():
         assert c[0].tolist() == [100, 1200]
 
 
-def test_mfma_distribution_and_chunks():
+@pytest.mark.parametrize("variant", [0, 3, 4])
+def test_mfma_distribution_and_chunks(variant):
     # distribution shares (masked in the epilogue) and tiny tile chunks with
-    # the candidate re-scan
+    # the candidate re-scan, on the 64 x 64 (default here), 64 x 128 and
+    # 128 x 128 tiles
+    print(f"case: mfma shares and chunks variant={variant}", flush=True)
     d, od = synth(90, 300, 280, 41)   # 4005 pairs: every one fits in the top-4096
     full = fa.GpuSearch(algorithm=fa.ALGO_MARGINAL).run_arrays(d, 2, fa.Distribution(90, 1), 4096)
     ref = {tuple(c): v for c, v in zip(full[0].tolist(), full[1])}
     seen = {}
     for off in range(3):
-        c, m = fa.GpuSearch(algorithm=fa.ALGO_MFMA).run_arrays(d, 2, fa.Distribution(90, 1).layer(3, off), 4096)
+        c, m = fa.GpuSearch(algorithm=fa.ALGO_MFMA, variant=variant).run_arrays(
+            d, 2, fa.Distribution(90, 1).layer(3, off), 4096)
         for cc, mm in zip(c.tolist(), m):
             assert cc[0] % 3 == off and tuple(cc) not in seen
             seen[tuple(cc)] = mm
     assert seen == ref
-    s = fa.GpuSearch(algorithm=fa.ALGO_MFMA, chunk_combos=4096 * 2, candidate_cap=256, sample_threshold=False)
+    s = fa.GpuSearch(algorithm=fa.ALGO_MFMA, variant=variant, chunk_combos=4096 * 2, candidate_cap=256,
+                     sample_threshold=False)
     c, m = s.run_arrays(d, 2, fa.Distribution(90, 1), 64)
     oc, om, _ = O.search(od, 2, 64, threads=2, mi_mode=2)
     assert c.tolist() == oc.tolist() and np.array_equal(m, om)
